@@ -1,0 +1,293 @@
+"""CPU oracle for the single-partition SPECFEM3D solver loop (TEST INFRASTRUCTURE ONLY).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
+import this module, and only as the checker / CPU baseline. The product path never
+calls it.
+
+``OracleSolver`` drives the C restatement in ``specfem_oracle.c`` through one time
+step in exactly the order of the reference host loop at NPROC=1:
+
+* ``iterate_time.F90:219-330``: Newmark predictor (acoustic, then elastic), acoustic
+  forces, elastic forces, seismograms, kernels;
+* ``compute_forces_acoustic_calling_routine.f90:54-343``: free surface, Stacey,
+  coupling (uses the predicted displacement), sources, element loop (all elements are
+  "inner" at NPROC=1, so the element loop runs after the boundary terms), mass +
+  corrector, free surface again;
+* ``compute_forces_viscoelastic_calling_routine.F90:30-435``: Stacey, coupling
+  (uses the updated fluid potential), sources, element loop, mass + corrector.
+
+Precision: ``"f32"`` is the FP32 oracle (same operation order as the reference's
+CUSTOM_REAL=4 CPU build, no fused multiply-add); ``"f64"`` is the FP64 oracle on the
+same (float32) database values.
+
+Parity status of each restated function is recorded in tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build_oracle(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    src = os.path.join(_HERE, "specfem_oracle.c")
+    if force or not os.path.exists(path) or os.path.getmtime(path) < os.path.getmtime(src):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = C.CDLL(build_oracle())
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+@dataclass
+class AttenuationState:
+    alphaval: np.ndarray
+    betaval: np.ndarray
+    gammaval: np.ndarray
+    factor_common: np.ndarray        # (nspec*125*3)
+    factor_common_kappa: np.ndarray
+
+
+class OracleSolver:
+    """Forward (and adjoint-with-stored-forward) time stepping on the CPU."""
+
+    def __init__(self, db, sources=None, receivers=None, precision: str = "f32", attenuation=None,
+                 compute_strain: bool = False, store_eps_trace3: bool = False):
+        self.db = db
+        self.prec = precision
+        self.dt_np = np.float32 if precision == "f32" else np.float64
+        self.sfx = "_f32" if precision == "f32" else "_f64"
+        self.L = lib()
+        R = self.dt_np
+        from specfem3d_amd.gll import get_gll
+        g = get_gll()
+        self.hprime = np.ascontiguousarray(g.hprime, dtype=R)
+        self.hprimewgll = np.ascontiguousarray(g.hprimewgll, dtype=R)
+        self.wgllwgll = np.ascontiguousarray(g.wgllwgll_xy, dtype=R)
+        self.ibool = np.ascontiguousarray(db.ibool.reshape(-1), dtype=np.int32)
+        self.irr = np.ascontiguousarray(db.irregular_element_number, dtype=np.int32)
+        self.jac9 = np.ascontiguousarray(db.jac9.reshape(-1), dtype=R)
+        self.jac = np.ascontiguousarray(db.jacobian.reshape(-1), dtype=R)
+        self.kappa = np.ascontiguousarray(db.kappastore.reshape(-1), dtype=R)
+        self.mu = np.ascontiguousarray(db.mustore.reshape(-1), dtype=R)
+        self.rho = np.ascontiguousarray(db.rhostore.reshape(-1), dtype=R)
+        self.rho_vp = np.ascontiguousarray(db.rho_vp.reshape(-1), dtype=R)
+        self.rho_vs = np.ascontiguousarray(db.rho_vs.reshape(-1), dtype=R)
+        self.rmassx = db.rmassx.astype(R)
+        self.rmassy = db.rmassy.astype(R)
+        self.rmassz = db.rmassz.astype(R)
+        self.rmass_ac = db.rmass_acoustic.astype(R)
+        self.is_el = np.ascontiguousarray(db.ispec_is_elastic.astype(np.uint8))
+        self.is_ac = np.ascontiguousarray(db.ispec_is_acoustic.astype(np.uint8))
+        self.el_elems = db.elastic_elements()
+        self.ac_elems = db.acoustic_elements()
+        self.elastic = len(self.el_elems) > 0
+        self.acoustic = len(self.ac_elems) > 0
+        f = db.abs_faces
+        self.abs = (f.n, np.ascontiguousarray(f.ispec, np.int32), np.ascontiguousarray(f.ijk, np.int32),
+                    np.ascontiguousarray(f.normal, R), np.ascontiguousarray(f.jacobian2Dw, R))
+        f = db.free_faces
+        self.free = (f.n, np.ascontiguousarray(f.ispec, np.int32), np.ascontiguousarray(f.ijk, np.int32))
+        f = db.coupling_faces
+        self.cpl = (f.n, np.ascontiguousarray(f.ispec, np.int32), np.ascontiguousarray(f.ijk, np.int32),
+                    np.ascontiguousarray(f.normal, R), np.ascontiguousarray(f.jacobian2Dw, R))
+        self.stacey = db.stacey and db.abs_faces.n > 0
+        dt = db.dt
+        self.deltat = R(np.float32(dt)) if precision == "f32" else R(dt)
+        # reference: deltat = real(DT); deltatover2 = deltat/2; deltatsqover2 = deltat*deltat/2
+        self.deltatover2 = R(self.deltat / R(2))
+        self.deltatsqover2 = R(self.deltat * self.deltat / R(2))
+        n = db.nglob
+        self.displ = np.zeros(3 * n, R)
+        self.veloc = np.zeros(3 * n, R)
+        self.accel = np.zeros(3 * n, R)
+        self.pot = np.zeros(n, R)
+        self.potd = np.zeros(n, R)
+        self.potdd = np.zeros(n, R)
+        self.sources = sources
+        self.receivers = receivers
+        if sources is not None and sources.n:
+            self.src_ispec = np.ascontiguousarray(sources.ispec, np.int32)
+            self.src_arr = np.ascontiguousarray(sources.sourcearrays.reshape(-1), R)
+        if receivers is not None and receivers.n:
+            self.rec_ispec = np.ascontiguousarray(receivers.ispec, np.int32)
+            self.hx = np.ascontiguousarray(receivers.hxir, np.float64)
+            self.hy = np.ascontiguousarray(receivers.hetar, np.float64)
+            self.hz = np.ascontiguousarray(receivers.hgammar, np.float64)
+            self.nu = np.ascontiguousarray(receivers.nu, np.float64)
+        self.att = attenuation
+        self.compute_strain = compute_strain or (attenuation is not None)
+        ne = db.nspec * 125
+        if self.compute_strain:
+            self.eps = [np.zeros(ne, R) for _ in range(6)]  # xx,yy,xy,xz,yz,trace
+        else:
+            self.eps = [None] * 6
+        self.eps_tr3 = np.zeros(ne, R) if store_eps_trace3 else None
+        if attenuation is not None:
+            self.R = [np.zeros(ne * 3, R) for _ in range(6)]  # xx,yy,xy,xz,yz,trace
+            self.alpha = attenuation.alphaval.astype(R)
+            self.beta = attenuation.betaval.astype(R)
+            self.gamma = attenuation.gammaval.astype(R)
+            self.fc = np.ascontiguousarray(attenuation.factor_common.reshape(-1), R)
+            self.fck = np.ascontiguousarray(attenuation.factor_common_kappa.reshape(-1), R)
+        self.it = 0
+        self.adjoint_sign = 0   # 1 for the adjoint wavefield of a kernel run (coupling sign flip)
+        self.stf_table = None
+
+    # -- pieces ---------------------------------------------------------------------
+    def _call(self, name, *args):
+        return getattr(self.L, name + self.sfx)(*args)
+
+    def predictor(self):
+        R = self.dt_np
+        if self.acoustic:
+            self._call("orc_update_displ", C.c_int64(self.db.nglob), _p(self.pot), _p(self.potd), _p(self.potdd),
+                       self._r(self.deltat), self._r(self.deltatover2), self._r(self.deltatsqover2))
+        if self.elastic:
+            self._call("orc_update_displ", C.c_int64(3 * self.db.nglob), _p(self.displ), _p(self.veloc),
+                       _p(self.accel), self._r(self.deltat), self._r(self.deltatover2), self._r(self.deltatsqover2))
+
+    def _r(self, v):
+        return C.c_float(v) if self.prec == "f32" else C.c_double(v)
+
+    def forces_elastic(self):
+        db = self.db
+        att = self.att is not None
+        e = self.eps
+        if att:
+            Rr = self.R
+            attargs = (_p(self.alpha), _p(self.beta), _p(self.gamma), _p(self.fc), _p(self.fck),
+                       _p(Rr[0]), _p(Rr[1]), _p(Rr[2]), _p(Rr[3]), _p(Rr[4]), _p(Rr[5]))
+        else:
+            attargs = (None,) * 11
+        self._call("orc_forces_elastic", C.c_int(len(self.el_elems)), _p(self.el_elems), _p(self.ibool),
+                   _p(self.irr), self._r(db.xix_regular), self._r(db.jacobian_regular), _p(self.jac9),
+                   _p(self.jac), _p(self.hprime), _p(self.hprimewgll), _p(self.wgllwgll), _p(self.kappa),
+                   _p(self.mu), _p(self.displ), _p(self.veloc), _p(self.accel), self._r(self.deltat),
+                   C.c_int(int(att)), C.c_int(int(self.compute_strain)), *attargs,
+                   _p(e[0]), _p(e[1]), _p(e[2]), _p(e[3]), _p(e[4]), _p(e[5]), _p(self.eps_tr3))
+
+    def forces_acoustic(self):
+        db = self.db
+        self._call("orc_forces_acoustic", C.c_int(len(self.ac_elems)), _p(self.ac_elems), _p(self.ibool),
+                   _p(self.irr), self._r(db.xix_regular), self._r(db.jacobian_regular), _p(self.jac9),
+                   _p(self.jac), _p(self.hprime), _p(self.hprimewgll), _p(self.wgllwgll), _p(self.rho),
+                   _p(self.pot), _p(self.potdd))
+
+    def stf_values(self, it):
+        s = self.sources
+        if self.stf_table is not None:
+            v = self.stf_table[it - 1]
+        else:
+            v = s.stf(it)
+        return np.asarray(v, np.float64).astype(np.float32).astype(self.dt_np)  # stf_used = real(stf)
+
+    def step(self, adjoint_source=None):
+        """One forward time step (1-based step index self.it+1)."""
+        self.it += 1
+        it = self.it
+        self.predictor()
+        flip = C.c_int(self.adjoint_sign)
+        if self.acoustic:
+            nf, fis, fijk = self.free
+            if nf:
+                self._call("orc_free_surface", C.c_int(nf), _p(fis), _p(fijk), _p(self.ibool), _p(self.is_ac),
+                           _p(self.pot), _p(self.potd), _p(self.potdd))
+            if self.stacey:
+                n, isp, ijk, nrm, jw = self.abs
+                self._call("orc_stacey_acoustic", C.c_int(n), _p(isp), _p(ijk), _p(jw), _p(self.ibool),
+                           _p(self.is_ac), _p(self.rho), _p(self.kappa), _p(self.potd), _p(self.potdd))
+            if self.elastic and self.cpl[0]:
+                n, isp, ijk, nrm, jw = self.cpl
+                self._call("orc_coupling_ac_el", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
+                           _p(self.displ), _p(self.potdd), flip)
+            self.forces_acoustic()
+            self._call("orc_kernel3_acoustic", C.c_int64(self.db.nglob), _p(self.potd), _p(self.potdd),
+                       _p(self.rmass_ac), self._r(self.deltatover2))
+            nf, fis, fijk = self.free
+            if nf:
+                self._call("orc_free_surface", C.c_int(nf), _p(fis), _p(fijk), _p(self.ibool), _p(self.is_ac),
+                           _p(self.pot), _p(self.potd), _p(self.potdd))
+        if self.elastic:
+            if self.stacey:
+                n, isp, ijk, nrm, jw = self.abs
+                self._call("orc_stacey_elastic", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
+                           _p(self.is_el), _p(self.rho_vp), _p(self.rho_vs), _p(self.veloc), _p(self.accel), None)
+            if self.acoustic and self.cpl[0]:
+                n, isp, ijk, nrm, jw = self.cpl
+                self._call("orc_coupling_el_ac", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
+                           _p(self.potdd), _p(self.accel), flip)
+            if self.sources is not None and self.sources.n:
+                stf = np.ascontiguousarray(self.stf_values(it), self.dt_np)
+                self._call("orc_add_sources_el", C.c_int(self.sources.n), _p(self.src_ispec), _p(self.src_arr),
+                           _p(stf), _p(self.ibool), _p(self.is_el), _p(self.accel))
+            if adjoint_source is not None:
+                adj = np.ascontiguousarray(adjoint_source, self.dt_np)
+                self._call("orc_add_adjoint_sources_el", C.c_int(self.receivers.n), _p(self.rec_ispec),
+                           _p(self.hx), _p(self.hy), _p(self.hz), _p(adj), _p(self.ibool), _p(self.accel))
+            self.forces_elastic()
+            self._call("orc_kernel3_elastic", C.c_int64(self.db.nglob), _p(self.veloc), _p(self.accel),
+                       _p(self.rmassx), _p(self.rmassy), _p(self.rmassz), self._r(self.deltatover2))
+
+    def seismogram(self, field: str = "displ") -> np.ndarray:
+        out = np.zeros(3 * self.receivers.n, self.dt_np)
+        f = {"displ": self.displ, "veloc": self.veloc, "accel": self.accel}[field]
+        self._call("orc_seismo_elastic", C.c_int(self.receivers.n), _p(self.rec_ispec), _p(self.hx), _p(self.hy),
+                   _p(self.hz), _p(self.nu), _p(self.ibool), _p(f), _p(out))
+        return out.reshape(-1, 3)
+
+    def run(self, nstep: int, record: str = "displ", callback=None) -> np.ndarray:
+        """Run ``nstep`` steps; returns seismograms (nstep, nrec, 3)."""
+        nrec = self.receivers.n if self.receivers is not None else 0
+        seis = np.zeros((nstep, nrec, 3), self.dt_np)
+        if self.sources is not None and self.sources.n:
+            self.stf_table = self.sources.stf_table(self.it + nstep) if self.stf_table is None else self.stf_table
+        for s in range(nstep):
+            self.step()
+            if nrec:
+                seis[s] = self.seismogram(record)
+            if callback is not None:
+                callback(self)
+        return seis
+
+    def max_norm(self) -> float:
+        return float(self._call_ret("orc_max_norm", self.displ))
+
+    def _call_ret(self, name, f):
+        fn = getattr(self.L, name + self.sfx)
+        fn.restype = C.c_float if self.prec == "f32" else C.c_double
+        return fn(C.c_int64(self.db.nglob), _p(f))
+
+
+def element_strain(db, displ, precision="f32"):
+    """Deviatoric strain and trace/3 of a displacement field (``compute_element_strain``)."""
+    R = np.float32 if precision == "f32" else np.float64
+    sfx = "_f32" if precision == "f32" else "_f64"
+    from specfem3d_amd.gll import get_gll
+    g = get_gll()
+    L = lib()
+    ne = db.nspec * 125
+    out = [np.zeros(ne, R) for _ in range(6)]
+    elems = db.elastic_elements()
+    fn = getattr(L, "orc_element_strain" + sfx)
+    rarg = (C.c_float if precision == "f32" else C.c_double)(db.xix_regular)
+    fn(C.c_int(len(elems)), _p(elems), _p(np.ascontiguousarray(db.ibool.reshape(-1), np.int32)),
+       _p(np.ascontiguousarray(db.irregular_element_number, np.int32)), rarg,
+       _p(np.ascontiguousarray(db.jac9.reshape(-1), R)), _p(np.ascontiguousarray(g.hprime, R)),
+       _p(np.ascontiguousarray(displ, R)), *[_p(o) for o in out])
+    return out  # xx, yy, xy, xz, yz, trace/3

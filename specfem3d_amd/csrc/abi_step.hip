@@ -1,0 +1,686 @@
+// abi_step.hip -- per-time-step entry points of the Fortran-callable C-ABI, plus transfers.
+//
+// Call order within a step is fixed by the host (iterate_time.F90, the *_calling routines);
+// every entry point enqueues on the mesh's stream and returns (only device->host transfers and
+// norms synchronise). FORWARD_OR_ADJOINT: 0 = both fields, 1 = forward (or adjoint) field,
+// 3 = backward/reconstructed field.
+//
+// kernel_3 (mass division + corrector) is deferred: the next update_displacement on the same
+// field runs it fused with the predictor (one pass over u, v, a instead of two). Any entry
+// point that reads v or a (or writes the field) runs a pending corrector first, so results
+// are identical to the eager order.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+
+namespace sf {
+realw* stage_to_device(Mesh* m, const realw* vals, int n, realw* dst);
+realw* stf_device_buffer(Mesh* m, int n);
+
+static inline int field_of(int foa) { return foa == 3 ? 1 : 0; }
+
+static void flush_el(Mesh* m, int f) {
+  if (m->pending_k3_el[f]) {
+    launch_corrector(m->stream, 3 * (int64_t)m->nglob, m->el[f].veloc, m->el[f].accel, m->rmass, m->pending_dt2_el[f]);
+    m->pending_k3_el[f] = 0;
+  }
+}
+static void flush_ac(Mesh* m, int f) {
+  if (m->pending_k3_ac[f]) {
+    launch_corrector(m->stream, m->nglob, m->ac[f].potd, m->ac[f].potdd, m->rmass_ac, m->pending_dt2_ac[f]);
+    m->pending_k3_ac[f] = 0;
+  }
+}
+void flush_all(Mesh* m) {
+  for (int f = 0; f < 2; f++) {
+    flush_el(m, f);
+    flush_ac(m, f);
+  }
+}
+
+static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
+  ElasticArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nelem = m->nspec_elastic;
+  a.elems = m->elastic_elems;
+  a.ibool = m->ibool;
+  a.irregular = m->irregular;
+  a.xix_regular = m->xix_regular;
+  a.jacobian_regular = m->jacobian_regular;
+  a.jac = m->jac;
+  a.jac_plane = (int64_t)m->nspec_irregular * 125;
+  a.kappamu = m->kappamu;
+  a.displ = m->el[f].displ;
+  a.veloc = m->el[f].veloc;
+  a.accel = m->el[f].accel;
+  a.deltat = deltat;
+  a.R = m->att[f].R;
+  a.eps = m->att[f].eps;
+  a.fc = m->factor_common;
+  a.fck = m->factor_common_kappa;
+  a.eps_tr3 = m->simulation_type == 3 ? m->eps_trace_over_3[f] : nullptr;
+  a.nloc = (int64_t)m->nspec * 125;
+  for (int s = 0; s < 3; s++) {
+    a.alpha[s] = f ? m->b_alphaval[s] : m->alphaval[s];
+    a.beta[s] = f ? m->b_betaval[s] : m->betaval[s];
+    a.gamma[s] = f ? m->b_gammaval[s] : m->gammaval[s];
+  }
+  a.local_out = m->asm_buf;
+  return a;
+}
+
+void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
+  ElasticArgs a = elastic_args(m, f, deltat);
+  const bool ordered = m->assembly == 1 && m->asm_ready;
+  launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered);
+  if (ordered)
+    launch_ordered_gather(m->stream, m->nglob, 3, m->asm_ptr, m->asm_idx, m->asm_buf, a.nloc, m->el[f].accel);
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+static void check_field(Mesh* m, int f, const char* where) {
+  if (!m->el[f].displ) fatal(where, f ? "backward elastic field not prepared" : "elastic field not prepared");
+}
+
+// ---------------------------------------------------------------- Newmark (update_displacement_cuda.cu)
+extern "C" void FC_FUNC_(update_displacement_cuda, UPDATE_DISPLACMENT_CUDA)(long* Mesh_pointer, realw* deltat_F,
+                                                                            realw* deltatover2_F,
+                                                                            realw* deltatsqover2_F,
+                                                                            int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  const int f = field_of(*FORWARD_OR_ADJOINT);
+  check_field(m, f, "update_displacement_cuda");
+  const int64_t n = 3 * (int64_t)m->nglob;
+  Wavefield& w = m->el[f];
+  if (m->pending_k3_el[f]) {
+    launch_corrector_predictor(m->stream, n, w.displ, w.veloc, w.accel, m->rmass, m->pending_dt2_el[f], *deltat_F,
+                               *deltatover2_F, *deltatsqover2_F);
+    m->pending_k3_el[f] = 0;
+  } else {
+    launch_predictor(m->stream, n, w.displ, w.veloc, w.accel, *deltat_F, *deltatover2_F, *deltatsqover2_F);
+  }
+}
+
+extern "C" void FC_FUNC_(update_displacement_ac_cuda, UPDATE_DISPLACEMENT_AC_CUDA)(long* Mesh_pointer,
+                                                                                  realw* deltat_F,
+                                                                                  realw* deltatover2_F,
+                                                                                  realw* deltatsqover2_F,
+                                                                                  int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  const int f = field_of(*FORWARD_OR_ADJOINT);
+  AcField& a = m->ac[f];
+  if (!a.pot) fatal("update_displacement_ac_cuda", "acoustic field not prepared");
+  if (m->pending_k3_ac[f]) {
+    launch_corrector_predictor(m->stream, m->nglob, a.pot, a.potd, a.potdd, m->rmass_ac, m->pending_dt2_ac[f],
+                               *deltat_F, *deltatover2_F, *deltatsqover2_F);
+    m->pending_k3_ac[f] = 0;
+  } else {
+    launch_predictor(m->stream, m->nglob, a.pot, a.potd, a.potdd, *deltat_F, *deltatover2_F, *deltatsqover2_F);
+  }
+}
+
+extern "C" void FC_FUNC_(kernel_3_a_cuda, KERNEL_3_A_CUDA)(long* Mesh_pointer, realw* deltatover2_F,
+                                                           realw* b_deltatover2_F, int* APPROXIMATE_OCEAN_LOAD,
+                                                           int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*APPROXIMATE_OCEAN_LOAD) fatal("kernel_3_a_cuda", "ocean load is not supported");
+  const int foa = *FORWARD_OR_ADJOINT;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->el[f].displ) continue;
+    flush_el(m, f);
+    m->pending_k3_el[f] = 1;
+    m->pending_dt2_el[f] = f ? *b_deltatover2_F : *deltatover2_F;
+  }
+}
+
+extern "C" void FC_FUNC_(kernel_3_b_cuda, KERNEL_3_B_CUDA)(long* Mesh_pointer, realw* deltatover2_F,
+                                                           realw* b_deltatover2_F, int* FORWARD_OR_ADJOINT) {
+  (void)Mesh_pointer; (void)deltatover2_F; (void)b_deltatover2_F; (void)FORWARD_OR_ADJOINT;
+  fatal("kernel_3_b_cuda", "ocean load is not supported");
+}
+
+extern "C" void FC_FUNC_(kernel_3_acoustic_cuda, KERNEL_3_ACOUSTIC_CUDA)(long* Mesh_pointer, realw* deltatover2_F,
+                                                                         realw* b_deltatover2_F,
+                                                                         int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->ac[f].pot) continue;
+    flush_ac(m, f);
+    m->pending_k3_ac[f] = 1;
+    m->pending_dt2_ac[f] = f ? *b_deltatover2_F : *deltatover2_F;
+  }
+}
+
+// ---------------------------------------------------------------- forces
+// compute_forces_viscoelastic_cuda.cu:1115
+extern "C" void FC_FUNC_(compute_forces_viscoelastic_cuda, COMPUTE_FORCES_VISCOELASTIC_CUDA)(
+    long* Mesh_pointer, int* iphase, realw* deltat, int* nspec_outer_elastic, int* nspec_inner_elastic,
+    int* COMPUTE_AND_STORE_STRAIN, int* ATTENUATION, int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  // NPROC=1: every element is "inner"; all elements are computed in phase 2
+  (void)nspec_outer_elastic; (void)nspec_inner_elastic;
+  if (*iphase != 2) return;
+  const bool strain = *COMPUTE_AND_STORE_STRAIN != 0;
+  const bool att = *ATTENUATION != 0;
+  if (att && !m->att[0].R) fatal("compute_forces_viscoelastic_cuda", "attenuation arrays not prepared");
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  // FORWARD_OR_ADJOINT = 0 computes the adjoint field, then the backward field
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (foa == 0 && !m->el[1].displ && f == 1) continue;
+    check_field(m, f, "compute_forces_viscoelastic_cuda");
+    flush_el(m, f);
+    if (strain && !m->att[f].eps) fatal("compute_forces_viscoelastic_cuda", "strain arrays not prepared");
+    elastic_forces(m, f, *deltat, strain, att && m->att[f].R != nullptr);
+  }
+}
+
+// compute_forces_acoustic_cuda.cu:238
+extern "C" void FC_FUNC_(compute_forces_acoustic_cuda, COMPUTE_FORCES_ACOUSTIC_CUDA)(long* Mesh_pointer, int* iphase,
+                                                                                   int* nspec_outer_acoustic,
+                                                                                   int* nspec_inner_acoustic,
+                                                                                   int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)nspec_outer_acoustic; (void)nspec_inner_acoustic;
+  if (*iphase != 2) return;
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->ac[f].pot) continue;
+    flush_ac(m, f);
+    AcousticArgs a;
+    a.nelem = m->nspec_acoustic;
+    a.elems = m->acoustic_elems;
+    a.ibool = m->ibool;
+    a.irregular = m->irregular;
+    a.xix_regular = m->xix_regular;
+    a.jacobian_regular = m->jacobian_regular;
+    a.jac = m->jac;
+    a.jac_plane = (int64_t)m->nspec_irregular * 125;
+    a.rho = m->rho;
+    a.pot = m->ac[f].pot;
+    a.potdd = m->ac[f].potdd;
+    a.local_out = m->asm_ac_buf;
+    const bool ordered = m->assembly == 1 && m->asm_ac_buf;
+    launch_acoustic(m->stream, a, m->exact != 0, ordered);
+    if (ordered)
+      launch_ordered_gather(m->stream, m->nglob, 1, m->asm_ac_ptr, m->asm_ac_idx, m->asm_ac_buf,
+                            (int64_t)m->nspec * 125, m->ac[f].potdd);
+  }
+}
+
+// compute_forces_acoustic_cuda.cu:334 -- zero chi, chi', chi'' on the fluid free surface. It
+// commutes with a deferred acoustic corrector (zeroed points stay zero through it).
+extern "C" void FC_FUNC_(acoustic_enforce_free_surf_cuda, ACOUSTIC_ENFORCE_FREE_SURF_CUDA)(
+    long* Mesh_pointer, int* ABSORB_INSTEAD_OF_FREE_SURFACE, int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*ABSORB_INSTEAD_OF_FREE_SURFACE) return;
+  const int foa = *FORWARD_OR_ADJOINT;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->ac[f].pot) continue;
+    launch_free_surface(m->stream, m->free_ac, m->ac[f].pot, m->ac[f].potd, m->ac[f].potdd);
+  }
+}
+
+// ---------------------------------------------------------------- boundaries
+// compute_stacey_viscoelastic_cuda.cu:39. Forward field: apply (and, when saving the forward
+// run, fill b_absorb_field(3,25,nfaces) for the host to write). Backward field of a kernel
+// run: subtract the record the host read into b_absorb_field.
+extern "C" void FC_FUNC_(compute_stacey_viscoelastic_cuda, COMPUTE_STACEY_VISCOELASTIC_CUDA)(
+    long* Mesh_pointer, int* iphasef, realw* b_absorb_field, int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*iphasef != 1 || !m->stacey || m->abs_el.n == 0) return;
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  const int64_t rec_len = 3 * 25 * (int64_t)m->num_abs_boundary_faces;
+  if (foa == 0 || foa == 1) {
+    flush_el(m, 0);
+    const bool store = m->simulation_type == 1 && m->save_forward && b_absorb_field;
+    launch_stacey_elastic(m->stream, m->abs_el, m->el[0].veloc, m->el[0].accel, m->b_absorb_field, store ? 1 : 0);
+    if (store) {
+      SF_CHECK(hipMemcpyAsync(b_absorb_field, m->b_absorb_field, sizeof(realw) * rec_len, hipMemcpyDeviceToHost,
+                              m->stream));
+      SF_CHECK(hipStreamSynchronize(m->stream));
+    }
+  }
+  if ((foa == 0 || foa == 3) && m->simulation_type == 3 && m->el[1].displ) {
+    flush_el(m, 1);
+    SF_CHECK(hipMemcpyAsync(m->b_absorb_field, b_absorb_field, sizeof(realw) * rec_len, hipMemcpyHostToDevice,
+                            m->stream));
+    launch_stacey_elastic_replay(m->stream, m->abs_el, m->b_absorb_field, m->el[1].accel);
+    SF_CHECK(hipStreamSynchronize(m->stream));  // the host buffer is reused for the next record
+  }
+}
+
+// compute_stacey_viscoelastic_cuda.cu:224 -- undo-attenuation / checkpoint runs step the
+// reconstructed field forward in time, so Stacey is applied to it directly.
+extern "C" void FC_FUNC_(compute_stacey_viscoelastic_undoatt_cuda, COMPUTE_STACEY_VISCOELASTIC_UNDOATT_CUDA)(
+    long* Mesh_pointer, int* iphasef, int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*iphasef != 1 || !m->stacey || m->abs_el.n == 0) return;
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->el[f].displ) continue;
+    flush_el(m, f);
+    launch_stacey_elastic(m->stream, m->abs_el, m->el[f].veloc, m->el[f].accel, nullptr, 0);
+  }
+}
+
+extern "C" void FC_FUNC_(compute_stacey_acoustic_cuda, COMPUTE_STACEY_ACOUSTIC_CUDA)(long* Mesh_pointer,
+                                                                                   int* iphasef,
+                                                                                   realw* h_b_absorb_potential,
+                                                                                   int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*iphasef != 1 || !m->stacey || m->abs_ac.n == 0) return;
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  const int64_t rec_len = 25 * (int64_t)m->num_abs_boundary_faces;
+  if (foa == 0 || foa == 1) {
+    flush_ac(m, 0);
+    const bool store = m->simulation_type == 1 && m->save_forward && h_b_absorb_potential;
+    launch_stacey_acoustic(m->stream, m->abs_ac, m->ac[0].potd, m->ac[0].potdd, m->b_absorb_potential, store ? 1 : 0);
+    if (store) {
+      SF_CHECK(hipMemcpyAsync(h_b_absorb_potential, m->b_absorb_potential, sizeof(realw) * rec_len,
+                              hipMemcpyDeviceToHost, m->stream));
+      SF_CHECK(hipStreamSynchronize(m->stream));
+    }
+  }
+  if ((foa == 0 || foa == 3) && m->simulation_type == 3 && m->ac[1].pot) {
+    flush_ac(m, 1);
+    SF_CHECK(hipMemcpyAsync(m->b_absorb_potential, h_b_absorb_potential, sizeof(realw) * rec_len,
+                            hipMemcpyHostToDevice, m->stream));
+    launch_stacey_acoustic_replay(m->stream, m->abs_ac, m->b_absorb_potential, m->ac[1].potdd);
+    SF_CHECK(hipStreamSynchronize(m->stream));
+  }
+}
+
+extern "C" void FC_FUNC_(compute_stacey_acoustic_undoatt_cuda, COMPUTE_STACEY_ACOUSTIC_UNDOATT_CUDA)(
+    long* Mesh_pointer, int* iphasef, int* FORWARD_OR_ADJOINT_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*iphasef != 1 || !m->stacey || m->abs_ac.n == 0) return;
+  const int foa = *FORWARD_OR_ADJOINT_f;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->ac[f].pot) continue;
+    flush_ac(m, f);
+    launch_stacey_acoustic(m->stream, m->abs_ac, m->ac[f].potd, m->ac[f].potdd, nullptr, 0);
+  }
+}
+
+// compute_coupling_cuda.cu:39 -- fluid side: chi'' += w u.n (adjoint field: u -> -u)
+extern "C" void FC_FUNC_(compute_coupling_ac_el_cuda, COMPUTE_COUPLING_AC_EL_CUDA)(long* Mesh_pointer, int* iphasef,
+                                                                                 int* num_coupling_ac_el_facesf,
+                                                                                 int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*iphasef != 1 || *num_coupling_ac_el_facesf == 0 || m->coupling.n == 0) return;
+  const int foa = *FORWARD_OR_ADJOINT;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->ac[f].pot || !m->el[f].displ) continue;
+    flush_ac(m, f);
+    const int flip = (m->simulation_type != 1 && f == 0) ? 1 : 0;
+    launch_coupling_ac_el(m->stream, m->coupling, m->el[f].displ, m->ac[f].potdd, flip);
+  }
+}
+
+// compute_coupling_cuda.cu:135 -- solid side: a += w n (-chi'') (adjoint field: sign flip)
+extern "C" void FC_FUNC_(compute_coupling_el_ac_cuda, COMPUTE_COUPLING_EL_AC_CUDA)(long* Mesh_pointer, int* iphasef,
+                                                                                 int* num_coupling_ac_el_facesf,
+                                                                                 int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (*iphasef != 1 || *num_coupling_ac_el_facesf == 0 || m->coupling.n == 0) return;
+  const int foa = *FORWARD_OR_ADJOINT;
+  for (int f = 0; f < 2; f++) {
+    if (foa == 1 && f == 1) continue;
+    if (foa == 3 && f == 0) continue;
+    if (!m->ac[f].pot || !m->el[f].displ) continue;
+    flush_ac(m, f);
+    flush_el(m, f);
+    const int flip = (m->simulation_type != 1 && f == 0) ? 1 : 0;
+    launch_coupling_el_ac(m->stream, m->coupling, m->ac[f].potdd, m->el[f].accel, flip);
+  }
+}
+
+// ---------------------------------------------------------------- sources
+// compute_add_sources_viscoelastic_cuda.cu:39; stf_used = real(stf) per source (double on the host)
+static void add_sources_el(Mesh* m, int f, const double* h_stf, int nsrc) {
+  if (m->src_pts.n == 0 || nsrc <= 0) return;
+  std::vector<realw> s(nsrc);
+  bool any = false;
+  for (int i = 0; i < nsrc; i++) {
+    s[i] = (realw)h_stf[i];
+    any = any || h_stf[i] != 0.0;
+  }
+  if (!any) return;  // as the reference's GPU wrapper: nothing to add when all STFs are zero
+  flush_el(m, f);
+  realw* d = stf_device_buffer(m, nsrc);
+  stage_to_device(m, s.data(), nsrc, d);
+  launch_add_sources_el(m->stream, m->src_pts, m->src_array, d, m->el[f].accel);
+}
+
+extern "C" void FC_FUNC_(compute_add_sources_el_cuda, COMPUTE_ADD_SOURCES_EL_CUDA)(long* Mesh_pointer,
+                                                                                 double* h_stf_pre_compute,
+                                                                                 int* h_NSOURCES) {
+  Mesh* m = from_handle(Mesh_pointer);
+  add_sources_el(m, 0, h_stf_pre_compute, *h_NSOURCES);
+}
+
+extern "C" void FC_FUNC_(compute_add_sources_el_s3_cuda, COMPUTE_ADD_SOURCES_EL_S3_CUDA)(long* Mesh_pointer,
+                                                                                       double* h_stf_pre_compute,
+                                                                                       int* h_NSOURCES) {
+  Mesh* m = from_handle(Mesh_pointer);
+  check_field(m, 1, "compute_add_sources_el_s3_cuda");
+  add_sources_el(m, 1, h_stf_pre_compute, *h_NSOURCES);
+}
+
+// compute_add_sources_viscoelastic_cuda.cu:224 -- adjoint sources at the receivers for step it:
+// h_source_adjoint(3, nadj_rec_local, NTSTEP_BETWEEN_READ_ADJSRC), sample
+// NTSTEP_BETWEEN_READ_ADJSRC - mod(it-1, NTSTEP_BETWEEN_READ_ADJSRC)
+extern "C" void FC_FUNC_(add_sources_el_sim_type_2_or_3, ADD_SOURCES_EL_SIM_TYPE_2_OR_3)(
+    long* Mesh_pointer, realw* h_source_adjoint, int* nrec, int* nadj_rec_local, int* NTSTEP_BETWEEN_READ_ADJSRC,
+    int* it) {
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)nrec;
+  const int nadj = *nadj_rec_local;
+  if (nadj <= 0 || m->adj_pts.n == 0) return;
+  const int nt = *NTSTEP_BETWEEN_READ_ADJSRC;
+  const int isample = nt - ((*it - 1) % nt);  // 1-based
+  const realw* src = h_source_adjoint + (int64_t)3 * nadj * (isample - 1);
+  flush_el(m, 0);
+  stage_to_device(m, src, 3 * nadj, m->adj_src);
+  launch_adjoint_sources_el(m->stream, m->adj_pts, m->adj_h, m->adj_src, m->el[0].accel);
+}
+
+extern "C" void FC_FUNC_(compute_add_sources_ac_cuda, COMPUTE_ADD_SOURCES_AC_CUDA)(long* Mesh_pointer,
+                                                                                 int* NSOURCESf,
+                                                                                 double* h_stf_pre_compute,
+                                                                                 int* run_number_of_the_source) {
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)h_stf_pre_compute; (void)run_number_of_the_source;
+  for (int s = 0; s < std::min(*NSOURCESf, m->nsources); s++)
+    if (!m->h_src_elastic[s]) fatal("compute_add_sources_ac_cuda", "sources inside acoustic elements are not supported");
+}
+
+extern "C" void FC_FUNC_(compute_add_sources_ac_s3_cuda, COMPUTE_ADD_SOURCES_AC_s3_CUDA)(long* Mesh_pointer,
+                                                                                       int* NSOURCESf,
+                                                                                       double* h_stf_pre_compute,
+                                                                                       int* run_number_of_the_source) {
+  FC_FUNC_(compute_add_sources_ac_cuda, X)(Mesh_pointer, NSOURCESf, h_stf_pre_compute, run_number_of_the_source);
+}
+
+extern "C" void FC_FUNC_(add_sources_ac_sim_2_or_3_cuda, ADD_SOURCES_AC_SIM_2_OR_3_CUDA)(
+    long* Mesh_pointer, realw* h_source_adjoint, int* nrec, int* nadj_rec_local, int* NTSTEP_BETWEEN_READ_ADJSRC,
+    int* it) {
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)h_source_adjoint; (void)nrec; (void)NTSTEP_BETWEEN_READ_ADJSRC; (void)it;
+  for (int r = 0; r < std::min(*nadj_rec_local, m->nadj_rec_local); r++)
+    if (!m->h_adj_elastic[r]) fatal("add_sources_ac_sim_2_or_3_cuda", "adjoint sources in acoustic elements are not supported");
+}
+
+// ---------------------------------------------------------------- seismograms (write_seismograms_cuda.cu:34)
+extern "C" void FC_FUNC_(compute_seismograms_cuda, COMPUTE_SEISMOGRAMS_CUDA)(
+    long* Mesh_pointer_f, realw* seismograms_d, realw* seismograms_v, realw* seismograms_a, realw* seismograms_p,
+    int* seismo_currentf, int* nlength_seismogramf, int* itf, int* it_endf, int* ACOUSTIC_SIMULATION,
+    int* ELASTIC_SIMULATION, int* USE_TRICK_FOR_BETTER_PRESSURE) {
+  Mesh* m = from_handle(Mesh_pointer_f);
+  (void)ACOUSTIC_SIMULATION; (void)ELASTIC_SIMULATION; (void)USE_TRICK_FOR_BETTER_PRESSURE;
+  if (m->nrec_local <= 0) return;
+  for (int r = 0; r < m->nrec_local; r++)
+    if (!m->h_rec_elastic[r]) fatal("compute_seismograms_cuda", "receivers in acoustic elements are not supported");
+  if (m->save_p) fatal("compute_seismograms_cuda", "pressure seismograms are not supported");
+  const int cur = *seismo_currentf;
+  // SIMULATION_TYPE 3 records the reconstructed (backward) field
+  const int f = (m->simulation_type == 3 && m->el[1].displ) ? 1 : 0;
+  if (m->save_v || m->save_a) flush_el(m, f);
+  const int64_t off = (int64_t)(cur - 1) * m->nrec_local * 3;
+  if (m->save_d) launch_seismo_el(m->stream, m->nrec_local, m->rec_iglob, m->rec_h, m->rec_nu, m->el[f].displ, m->seismo_d + off);
+  if (m->save_v) launch_seismo_el(m->stream, m->nrec_local, m->rec_iglob, m->rec_h, m->rec_nu, m->el[f].veloc, m->seismo_v + off);
+  if (m->save_a) launch_seismo_el(m->stream, m->nrec_local, m->rec_iglob, m->rec_h, m->rec_nu, m->el[f].accel, m->seismo_a + off);
+  if (cur == *nlength_seismogramf || *itf == *it_endf) {
+    const size_t bytes = sizeof(realw) * (size_t)cur * m->nrec_local * 3;
+    if (m->save_d) SF_CHECK(hipMemcpyAsync(seismograms_d, m->seismo_d, bytes, hipMemcpyDeviceToHost, m->stream));
+    if (m->save_v) SF_CHECK(hipMemcpyAsync(seismograms_v, m->seismo_v, bytes, hipMemcpyDeviceToHost, m->stream));
+    if (m->save_a) SF_CHECK(hipMemcpyAsync(seismograms_a, m->seismo_a, bytes, hipMemcpyDeviceToHost, m->stream));
+    SF_CHECK(hipStreamSynchronize(m->stream));
+  }
+  (void)seismograms_p;
+}
+
+// ---------------------------------------------------------------- kernels (compute_kernels_cuda.cu:39)
+extern "C" void FC_FUNC_(compute_kernels_elastic_cuda, COMPUTE_KERNELS_ELASTIC_CUDA)(long* Mesh_pointer,
+                                                                                   realw* deltat_f) {
+  Mesh* m = from_handle(Mesh_pointer);
+  check_field(m, 1, "compute_kernels_elastic_cuda");
+  if (!m->rho_kl) fatal("compute_kernels_elastic_cuda", "kernel arrays not prepared");
+  flush_el(m, 0);
+  flush_el(m, 1);
+  const int64_t nl = (int64_t)m->nspec * 125;
+  launch_compute_kernels_el(m->stream, m->nspec_elastic, m->elastic_elems, m->ibool, m->el[0].accel, m->el[1].displ,
+                            m->att[0].eps, m->eps_trace_over_3[0], m->att[1].eps, m->eps_trace_over_3[1], nl,
+                            *deltat_f, m->rho_kl, m->mu_kl, m->kappa_kl);
+}
+
+// compute_strain_cuda.cu: strain of the (backward) displacement for undo-attenuation runs
+extern "C" void FC_FUNC_(compute_strain_cuda, COMPUTE_strain_CUDA)(long* Mesh_pointer, int* FORWARD_OR_ADJOINT) {
+  Mesh* m = from_handle(Mesh_pointer);
+  const int f = field_of(*FORWARD_OR_ADJOINT);
+  check_field(m, f, "compute_strain_cuda");
+  if (!m->att[f].eps || !m->eps_trace_over_3[f]) fatal("compute_strain_cuda", "strain arrays not prepared");
+  flush_el(m, f);
+  ElasticArgs a = elastic_args(m, f, 0.f);
+  launch_element_strain(m->stream, a, m->att[f].eps, m->eps_trace_over_3[f], m->exact != 0);
+}
+
+extern "C" void FC_FUNC_(compute_kernels_acoustic_cuda, COMPUTE_KERNELS_ACOUSTIC_CUDA)(long* Mesh_pointer,
+                                                                                     realw* deltat_f) {
+  (void)Mesh_pointer; (void)deltat_f;
+  fatal("compute_kernels_acoustic_cuda", "acoustic sensitivity kernels are not supported");
+}
+
+// ---------------------------------------------------------------- norms (check_fields_cuda.cu:375-470)
+extern "C" void FC_FUNC_(get_norm_elastic_from_device, GET_NORM_ELASTIC_FROM_DEVICE)(realw* norm, long* Mesh_pointer,
+                                                                                   int* type) {
+  Mesh* m = from_handle(Mesh_pointer);
+  const int f = field_of(*type);
+  check_field(m, f, "get_norm_elastic_from_device");
+  launch_max_norm(m->stream, m->nglob, 3, m->el[f].displ, m->d_norm);
+  SF_CHECK(hipMemcpyAsync(m->h_norm, m->d_norm, sizeof(realw), hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  *norm = m->h_norm[0];
+}
+
+extern "C" void FC_FUNC_(get_norm_acoustic_from_device, GET_NORM_ACOUSTIC_FROM_DEVICE)(realw* norm,
+                                                                                     long* Mesh_pointer,
+                                                                                     int* sim_type) {
+  Mesh* m = from_handle(Mesh_pointer);
+  const int f = (*sim_type == 3) ? 1 : 0;
+  if (!m->ac[f].pot) fatal("get_norm_acoustic_from_device", "acoustic field not prepared");
+  launch_max_norm(m->stream, m->nglob, 1, m->ac[f].pot, m->d_norm);
+  SF_CHECK(hipMemcpyAsync(m->h_norm, m->d_norm, sizeof(realw), hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  *norm = m->h_norm[0];
+}
+
+// ---------------------------------------------------------------- transfers (transfer_fields_cuda.cu)
+static void h2d(Mesh* m, realw* d, const realw* h, int64_t n) {
+  SF_CHECK(hipMemcpyAsync(d, h, sizeof(realw) * (size_t)n, hipMemcpyHostToDevice, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+}
+static void d2h(Mesh* m, realw* h, const realw* d, int64_t n) {
+  SF_CHECK(hipMemcpyAsync(h, d, sizeof(realw) * (size_t)n, hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+}
+
+#define SF_XFER3(NAME, F, DIR)                                                                                  \
+  extern "C" void FC_FUNC_(NAME, X)(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer) { \
+    Mesh* m = from_handle(Mesh_pointer);                                                                      \
+    check_field(m, F, #NAME);                                                                                 \
+    flush_el(m, F);                                                                                           \
+    DIR(m, m->el[F].displ, displ, *size);                                                                     \
+    DIR(m, m->el[F].veloc, veloc, *size);                                                                     \
+    DIR(m, m->el[F].accel, accel, *size);                                                                     \
+  }
+#define SF_XFER1(NAME, F, MEMBER, DIR)                                                   \
+  extern "C" void FC_FUNC_(NAME, X)(int* size, realw* arr, long* Mesh_pointer) {        \
+    Mesh* m = from_handle(Mesh_pointer);                                                 \
+    check_field(m, F, #NAME);                                                            \
+    flush_el(m, F);                                                                      \
+    DIR(m, m->el[F].MEMBER, arr, *size);                                                 \
+  }
+#define H2D(m, d, h, n) h2d(m, d, h, n)
+#define D2H(m, d, h, n) d2h(m, h, d, n)
+
+SF_XFER3(transfer_fields_el_to_device, 0, H2D)
+SF_XFER3(transfer_fields_el_from_device, 0, D2H)
+SF_XFER3(transfer_b_fields_to_device, 1, H2D)
+SF_XFER3(transfer_b_fields_from_device, 1, D2H)
+SF_XFER1(transfer_displ_to_device, 0, displ, H2D)
+SF_XFER1(transfer_displ_from_device, 0, displ, D2H)
+SF_XFER1(transfer_veloc_to_device, 0, veloc, H2D)
+SF_XFER1(transfer_veloc_from_device, 0, veloc, D2H)
+SF_XFER1(transfer_accel_to_device, 0, accel, H2D)
+SF_XFER1(transfer_accel_from_device, 0, accel, D2H)
+SF_XFER1(transfer_b_displ_to_device, 1, displ, H2D)
+SF_XFER1(transfer_b_displ_from_device, 1, displ, D2H)
+SF_XFER1(transfer_b_veloc_to_device, 1, veloc, H2D)
+SF_XFER1(transfer_b_veloc_from_device, 1, veloc, D2H)
+SF_XFER1(transfer_b_accel_to_device, 1, accel, H2D)
+SF_XFER1(transfer_b_accel_from_device, 1, accel, D2H)
+
+#define SF_XFERAC3(NAME, F, DIR)                                                                                 \
+  extern "C" void FC_FUNC_(NAME, X)(int* size, realw* p, realw* pd, realw* pdd, long* Mesh_pointer) {           \
+    Mesh* m = from_handle(Mesh_pointer);                                                                       \
+    if (!m->ac[F].pot) fatal(#NAME, "acoustic field not prepared");                                            \
+    flush_ac(m, F);                                                                                            \
+    DIR(m, m->ac[F].pot, p, *size);                                                                            \
+    DIR(m, m->ac[F].potd, pd, *size);                                                                          \
+    DIR(m, m->ac[F].potdd, pdd, *size);                                                                        \
+  }
+#define SF_XFERAC1(NAME, F, MEMBER, DIR)                                          \
+  extern "C" void FC_FUNC_(NAME, X)(int* size, realw* arr, long* Mesh_pointer) { \
+    Mesh* m = from_handle(Mesh_pointer);                                          \
+    if (!m->ac[F].pot) fatal(#NAME, "acoustic field not prepared");               \
+    flush_ac(m, F);                                                               \
+    DIR(m, m->ac[F].MEMBER, arr, *size);                                          \
+  }
+SF_XFERAC3(transfer_fields_ac_to_device, 0, H2D)
+SF_XFERAC3(transfer_fields_ac_from_device, 0, D2H)
+SF_XFERAC3(transfer_b_fields_ac_to_device, 1, H2D)
+SF_XFERAC3(transfer_b_fields_ac_from_device, 1, D2H)
+SF_XFERAC1(transfer_potential_ac_from_device, 0, pot, D2H)
+SF_XFERAC1(transfer_b_potential_ac_from_device, 1, pot, D2H)
+SF_XFERAC1(transfer_b_potential_ac_to_device, 1, pot, H2D)
+SF_XFERAC1(transfer_b_potential_dot_dot_ac_from_device, 1, potdd, D2H)
+SF_XFERAC1(transfer_b_potential_dot_dot_ac_to_device, 1, potdd, H2D)
+SF_XFERAC1(transfer_dot_dot_to_device, 0, potdd, H2D)
+SF_XFERAC1(transfer_dot_dot_from_device, 0, potdd, D2H)
+SF_XFERAC1(transfer_b_dot_dot_from_device, 1, potdd, D2H)
+
+// transfer_fields_cuda.cu: sensitivity kernels (5,5,5,NSPEC_AB) back to the host
+extern "C" void FC_FUNC_(transfer_kernels_el_to_host, TRANSFER_KERNELS_EL_TO_HOST)(long* Mesh_pointer,
+                                                                                 realw* h_rho_kl, realw* h_mu_kl,
+                                                                                 realw* h_kappa_kl,
+                                                                                 realw* h_cijkl_kl, int* NSPEC_AB) {
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)h_cijkl_kl;
+  if (!m->rho_kl) fatal("transfer_kernels_el_to_host", "kernel arrays not prepared");
+  const int64_t n = (int64_t)*NSPEC_AB * 125;
+  d2h(m, h_rho_kl, m->rho_kl, n);
+  d2h(m, h_mu_kl, m->mu_kl, n);
+  d2h(m, h_kappa_kl, m->kappa_kl, n);
+}
+
+extern "C" void FC_FUNC_(transfer_strain_from_device, TRANSFER_STRAIN_FROM_DEVICE)(
+    long* Mesh_pointer, realw* epsilondev_xx, realw* epsilondev_yy, realw* epsilondev_xy, realw* epsilondev_xz,
+    realw* epsilondev_yz, realw* epsilondev_trace, int* size_epsilondev) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (!m->att[0].eps) fatal("transfer_strain_from_device", "strain arrays not prepared");
+  const int64_t nl = (int64_t)m->nspec * 125, n = *size_epsilondev;
+  realw* dst[6] = {epsilondev_xx, epsilondev_yy, epsilondev_xy, epsilondev_xz, epsilondev_yz, epsilondev_trace};
+  for (int c = 0; c < 6; c++)
+    if (dst[c]) d2h(m, dst[c], m->att[0].eps + c * nl, n);
+}
+
+extern "C" void FC_FUNC_(transfer_b_strain_to_device, TRANSFER_B_strain_TO_DEVICE)(
+    long* Mesh_pointer, realw* b_epsilondev_xx, realw* b_epsilondev_yy, realw* b_epsilondev_xy,
+    realw* b_epsilondev_xz, realw* b_epsilondev_yz, realw* b_epsilondev_trace, int* size_epsilondev) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (!m->att[1].eps) fatal("transfer_b_strain_to_device", "backward strain arrays not prepared");
+  const int64_t nl = (int64_t)m->nspec * 125, n = *size_epsilondev;
+  realw* src[6] = {b_epsilondev_xx, b_epsilondev_yy, b_epsilondev_xy, b_epsilondev_xz, b_epsilondev_yz,
+                   b_epsilondev_trace};
+  for (int c = 0; c < 6; c++)
+    if (src[c]) h2d(m, m->att[1].eps + c * nl, src[c], n);
+}
+
+static void rmem_copy(Mesh* m, int f, realw* const* arrs, int64_t nR, bool to_device) {
+  if (!m->att[f].R) fatal("transfer_rmemory", "attenuation arrays not prepared");
+  const int64_t nl = (int64_t)m->nspec * 125;
+  if (nR != 3 * nl) fatal("transfer_rmemory", "size_R must be N_SLS*125*NSPEC");
+  std::vector<realw> plane(18 * (size_t)nl);
+  if (!to_device) d2h(m, plane.data(), m->att[f].R, 18 * nl);
+  for (int c = 0; c < 6; c++)
+    for (int64_t q = 0; q < nl; q++)
+      for (int s = 0; s < 3; s++) {
+        realw& pv = plane[((size_t)c * 3 + s) * nl + q];
+        if (to_device) pv = arrs[c][3 * q + s];
+        else arrs[c][3 * q + s] = pv;
+      }
+  if (to_device) h2d(m, m->att[f].R, plane.data(), 18 * nl);
+}
+
+extern "C" void FC_FUNC_(transfer_rmemory_from_device, TRANSFER_RMEMORY_FROM_DEVICE)(long* Mesh_pointer, realw* R_xx,
+                                                                                   realw* R_yy, realw* R_xy,
+                                                                                   realw* R_xz, realw* R_yz,
+                                                                                   realw* R_trace, int* size_R) {
+  Mesh* m = from_handle(Mesh_pointer);
+  realw* a[6] = {R_xx, R_yy, R_xy, R_xz, R_yz, R_trace};
+  rmem_copy(m, 0, a, *size_R, false);
+}
+
+extern "C" void FC_FUNC_(transfer_b_rmemory_to_device, TRANSFER_B_RMEMORY_TO_DEVICE)(long* Mesh_pointer,
+                                                                                   realw* b_R_xx, realw* b_R_yy,
+                                                                                   realw* b_R_xy, realw* b_R_xz,
+                                                                                   realw* b_R_yz, realw* b_R_trace,
+                                                                                   int* size_R) {
+  Mesh* m = from_handle(Mesh_pointer);
+  realw* a[6] = {b_R_xx, b_R_yy, b_R_xy, b_R_xz, b_R_yz, b_R_trace};
+  rmem_copy(m, 1, a, *size_R, true);
+}
+
+extern "C" void FC_FUNC_(transfer_kernels_ac_to_host, TRANSFER_KERNELS_AC_TO_HOST)(long* Mesh_pointer,
+                                                                                 realw* h_rho_ac_kl,
+                                                                                 realw* h_kappa_ac_kl,
+                                                                                 int* NSPEC_AB) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (!m->rho_ac_kl) fatal("transfer_kernels_ac_to_host", "acoustic kernel arrays not prepared");
+  const int64_t n = (int64_t)*NSPEC_AB * 125;
+  d2h(m, h_rho_ac_kl, m->rho_ac_kl, n);
+  d2h(m, h_kappa_ac_kl, m->kappa_ac_kl, n);
+}
+
+// ---------------------------------------------------------------- library extensions (no reference counterpart)
+// sf_set_options: exact = 1 runs the element kernels without fused multiply-add (the CPU
+// reference's rounding); assembly = 1 sums element contributions in element order.
+extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly);
+extern "C" void sf_synchronize(long* Mesh_pointer) {
+  Mesh* m = from_handle(Mesh_pointer);
+  flush_all(m);
+  SF_CHECK(hipStreamSynchronize(m->stream));
+}

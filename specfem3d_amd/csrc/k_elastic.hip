@@ -1,0 +1,413 @@
+// k_elastic.hip -- isotropic elastic / viscoelastic (3 SLS) element stiffness for NGLL=5.
+//
+// Restates compute_forces_viscoelastic.F90:33-962 (with the memory update of
+// compute_element_att_memory.f90:28-156) as one gfx950 kernel:
+//   * a 128-thread workgroup (two wave64s) processes a tile of 5 elements; lane t works on
+//     element t/25 and on the GLL column (i, j) = (t%5, (t%25)/5), holding the 5 points of the
+//     column (k = 0..4) in registers, so 125 of the 128 lanes are busy;
+//   * the gamma-direction contractions run in registers; the xi/eta ones read the element
+//     tile staged once in LDS (6 planes x 128 floats per element: u and dt*v, later the
+//     stress terms t1/t2);
+//   * the element-to-global scatter is a float atomic add (global_atomic_add_f32) or, in
+//     ordered-assembly mode, a store of the local contributions that a gather kernel later
+//     sums in the CPU's element order;
+//   * workgroup ids are remapped so each XCD (8 per chip, dispatched round-robin) walks one
+//     contiguous range of elements and keeps the shared points of neighbours in its own L2.
+// The arithmetic order of every sum follows the reference (the five terms of each
+// contraction left to right, etc.). This file is compiled twice: with -ffp-contract=off
+// (SF_EXACT=1, same rounding as the CPU reference) and with FMA contraction (SF_EXACT=0).
+#include "kernels.h"
+
+#ifndef SF_EXACT
+#define SF_EXACT 1
+#endif
+#if SF_EXACT
+#define SFX(name) name##_exact
+#define SF_VNS v_exact
+#else
+#define SFX(name) name##_fast
+#define SF_VNS v_fast
+#endif
+
+namespace sf {
+// each build variant keeps its kernels in its own namespace (template instantiations of the
+// two variants would otherwise be merged by the linker)
+namespace SF_VNS {
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  // bijection: blocks b, b+8, b+16, ... (one XCD) get one contiguous logical range
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+template <bool STRAIN, bool ATT, bool ORDERED>
+__global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
+  __shared__ realw s_buf[5][6][128];
+  const int tid = threadIdx.x;
+  const int el = tid / 25;
+  const int c = tid - el * 25;
+  const int i = c % 5, j = c / 5;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int ie = blk * 5 + el;
+  const bool active = (tid < 125) && (ie < A.nelem);
+  const int e = active ? A.elems[ie] : 0;
+  const int elx = el < 5 ? el : 4;
+
+  realw hpi[5], hpj[5], hpwi[5], hpwj[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    hpi[l] = G.hp[i * 5 + l];
+    hpj[l] = G.hp[j * 5 + l];
+    hpwi[l] = G.hpw[l * 5 + i];
+    hpwj[l] = G.hpw[l * 5 + j];
+  }
+
+  int g[5];
+  realw ux[5], uy[5], uz[5];
+  realw vx[5], vy[5], vz[5];
+  if (active) {
+    const int* ib = A.ibool + (int64_t)e * 125 + c;
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      g[k] = ib[25 * k];
+      const realw* d = A.displ + 3 * (int64_t)g[k];
+      ux[k] = d[0]; uy[k] = d[1]; uz[k] = d[2];
+      s_buf[el][0][c + 25 * k] = ux[k];
+      s_buf[el][1][c + 25 * k] = uy[k];
+      s_buf[el][2][c + 25 * k] = uz[k];
+      if (ATT) {
+        const realw* v = A.veloc + 3 * (int64_t)g[k];
+        vx[k] = A.deltat * v[0]; vy[k] = A.deltat * v[1]; vz[k] = A.deltat * v[2];
+        s_buf[el][3][c + 25 * k] = vx[k];
+        s_buf[el][4][c + 25 * k] = vy[k];
+        s_buf[el][5][c + 25 * k] = vz[k];
+      }
+    }
+  }
+  __syncthreads();
+
+  const int irr = active ? A.irregular[e] : 0;
+  realw t1[3][5], t2[3][5], t3[3][5];
+  const realw ONE_THIRD = (realw)1 / (realw)3;
+  const realw FOUR_THIRDS = (realw)4 / (realw)3;
+
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    if (!active) continue;
+    const int p = c + 25 * k;
+    const realw* sx = s_buf[elx][0];
+    const realw* sy = s_buf[elx][1];
+    const realw* sz = s_buf[elx][2];
+    const int b1 = 5 * j + 25 * k;  // f(l, j, k) = s[b1 + l]
+    const int b2 = i + 25 * k;      // f(i, l, k) = s[b2 + 5 l]
+    realw tx1 = hpi[0] * sx[b1], ty1 = hpi[0] * sy[b1], tz1 = hpi[0] * sz[b1];
+    realw tx2 = hpj[0] * sx[b2], ty2 = hpj[0] * sy[b2], tz2 = hpj[0] * sz[b2];
+    realw tx3 = G.hp[k * 5] * ux[0], ty3 = G.hp[k * 5] * uy[0], tz3 = G.hp[k * 5] * uz[0];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      tx1 = tx1 + hpi[l] * sx[b1 + l]; ty1 = ty1 + hpi[l] * sy[b1 + l]; tz1 = tz1 + hpi[l] * sz[b1 + l];
+      tx2 = tx2 + hpj[l] * sx[b2 + 5 * l]; ty2 = ty2 + hpj[l] * sy[b2 + 5 * l]; tz2 = tz2 + hpj[l] * sz[b2 + 5 * l];
+      tx3 = tx3 + G.hp[k * 5 + l] * ux[l]; ty3 = ty3 + G.hp[k * 5 + l] * uy[l]; tz3 = tz3 + G.hp[k * 5 + l] * uz[l];
+    }
+
+    realw xixl = 0, xiyl = 0, xizl = 0, etaxl = 0, etayl = 0, etazl = 0, gaxl = 0, gayl = 0, gazl = 0, jacl;
+    realw duxdx, duxdy, duxdz, duydx, duydy, duydz, duzdx, duzdy, duzdz;
+    if (irr) {
+      const int64_t q = (int64_t)(irr - 1) * 125 + p;
+      const realw* J = A.jac;
+      const int64_t P = A.jac_plane;
+      xixl = J[q]; xiyl = J[P + q]; xizl = J[2 * P + q];
+      etaxl = J[3 * P + q]; etayl = J[4 * P + q]; etazl = J[5 * P + q];
+      gaxl = J[6 * P + q]; gayl = J[7 * P + q]; gazl = J[8 * P + q];
+      jacl = J[9 * P + q];
+      duxdx = xixl * tx1 + etaxl * tx2 + gaxl * tx3;
+      duxdy = xiyl * tx1 + etayl * tx2 + gayl * tx3;
+      duxdz = xizl * tx1 + etazl * tx2 + gazl * tx3;
+      duydx = xixl * ty1 + etaxl * ty2 + gaxl * ty3;
+      duydy = xiyl * ty1 + etayl * ty2 + gayl * ty3;
+      duydz = xizl * ty1 + etazl * ty2 + gazl * ty3;
+      duzdx = xixl * tz1 + etaxl * tz2 + gaxl * tz3;
+      duzdy = xiyl * tz1 + etayl * tz2 + gayl * tz3;
+      duzdz = xizl * tz1 + etazl * tz2 + gazl * tz3;
+    } else {
+      const realw xr = A.xix_regular;
+      jacl = A.jacobian_regular;
+      duxdx = xr * tx1; duxdy = xr * tx2; duxdz = xr * tx3;
+      duydx = xr * ty1; duydy = xr * ty2; duydz = xr * ty3;
+      duzdx = xr * tz1; duzdy = xr * tz2; duzdz = xr * tz3;
+    }
+
+    const int64_t ep = (int64_t)e * 125 + p;
+    realw e_xx = 0, e_yy = 0, e_xy = 0, e_xz = 0, e_yz = 0, e_tr = 0;
+    if (STRAIN) {
+      realw sxx, sxy, sxz, syx, syy, syz, szx, szy, szz;
+      if (ATT) {
+        // strain at u + dt v: continue each contraction with the dt*v tile
+        const realw* wx = s_buf[elx][3];
+        const realw* wy = s_buf[elx][4];
+        const realw* wz = s_buf[elx][5];
+        realw a1 = tx1, a2 = tx2, a3 = tx3, c1 = ty1, c2 = ty2, c3 = ty3, d1 = tz1, d2 = tz2, d3 = tz3;
+#pragma unroll
+        for (int l = 0; l < 5; l++) {
+          const realw h1 = hpi[l], h2 = hpj[l], h3 = G.hp[k * 5 + l];
+          a1 = a1 + wx[b1 + l] * h1; c1 = c1 + wy[b1 + l] * h1; d1 = d1 + wz[b1 + l] * h1;
+          a2 = a2 + wx[b2 + 5 * l] * h2; c2 = c2 + wy[b2 + 5 * l] * h2; d2 = d2 + wz[b2 + 5 * l] * h2;
+          a3 = a3 + vx[l] * h3; c3 = c3 + vy[l] * h3; d3 = d3 + vz[l] * h3;
+        }
+        if (irr) {
+          sxx = xixl * a1 + etaxl * a2 + gaxl * a3;
+          sxy = xiyl * a1 + etayl * a2 + gayl * a3;
+          sxz = xizl * a1 + etazl * a2 + gazl * a3;
+          syx = xixl * c1 + etaxl * c2 + gaxl * c3;
+          syy = xiyl * c1 + etayl * c2 + gayl * c3;
+          syz = xizl * c1 + etazl * c2 + gazl * c3;
+          szx = xixl * d1 + etaxl * d2 + gaxl * d3;
+          szy = xiyl * d1 + etayl * d2 + gayl * d3;
+          szz = xizl * d1 + etazl * d2 + gazl * d3;
+        } else {
+          const realw xr = A.xix_regular;
+          sxx = xr * a1; sxy = xr * a2; sxz = xr * a3;
+          syx = xr * c1; syy = xr * c2; syz = xr * c3;
+          szx = xr * d1; szy = xr * d2; szz = xr * d3;
+        }
+      } else {
+        sxx = duxdx; sxy = duxdy; sxz = duxdz; syx = duydx; syy = duydy; syz = duydz;
+        szx = duzdx; szy = duzdy; szz = duzdz;
+      }
+      e_tr = (sxx + syy + szz);
+      const realw templ = ONE_THIRD * (sxx + syy + szz);
+      if (A.eps_tr3) A.eps_tr3[ep] = templ;
+      e_xx = sxx - templ;
+      e_yy = syy - templ;
+      e_xy = (realw)0.5 * (sxy + syx);
+      e_xz = (realw)0.5 * (szx + sxz);
+      e_yz = (realw)0.5 * (szy + syz);
+    }
+
+    const float2 km = A.kappamu[ep];
+    const realw kappal = km.x, mul = km.y;
+    const realw lp2m = kappal + FOUR_THIRDS * mul;
+    const realw lambdal = lp2m - (realw)2 * mul;
+    realw sigma_xx = lp2m * duxdx + lambdal * (duydy + duzdz);
+    realw sigma_yy = lp2m * duydy + lambdal * (duxdx + duzdz);
+    realw sigma_zz = lp2m * duzdz + lambdal * (duxdx + duydy);
+    realw sigma_xy = mul * (duxdy + duydx);
+    realw sigma_xz = mul * (duzdx + duxdz);
+    realw sigma_yz = mul * (duzdy + duydz);
+
+    if (ATT) {
+      const int64_t nl = A.nloc;
+      realw* R = A.R;
+      realw Rv[6][3];
+#pragma unroll
+      for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+        for (int s = 0; s < 3; s++) Rv[cc][s] = R[(cc * 3 + s) * nl + ep];
+      const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
+      const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
+      const realw rtr = (Rv[5][0] + Rv[5][1]) + Rv[5][2];
+      sigma_xx = sigma_xx - rxx - rtr;
+      sigma_yy = sigma_yy - ryy - rtr;
+      sigma_zz = sigma_zz + rxx + ryy - rtr;
+      sigma_xy = sigma_xy - ((Rv[2][0] + Rv[2][1]) + Rv[2][2]);
+      sigma_xz = sigma_xz - ((Rv[3][0] + Rv[3][1]) + Rv[3][2]);
+      sigma_yz = sigma_yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
+      // memory-variable update (second-order RK) with old strain Sn and new strain Snp1
+      realw* Ep = A.eps;
+      const realw Sn[6] = {Ep[0 * nl + ep], Ep[1 * nl + ep], Ep[2 * nl + ep], Ep[3 * nl + ep], Ep[4 * nl + ep],
+                           Ep[5 * nl + ep]};
+      const realw Snp1[6] = {e_xx, e_yy, e_xy, e_xz, e_yz, e_tr};
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        const realw flk = kappal * A.fck[s * nl + ep];
+        R[(5 * 3 + s) * nl + ep] = A.alpha[s] * Rv[5][s] + flk * (A.beta[s] * Sn[5] + A.gamma[s] * Snp1[5]);
+        const realw flm = mul * A.fc[s * nl + ep];
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++)
+          R[(cc * 3 + s) * nl + ep] = A.alpha[s] * Rv[cc][s] + flm * (A.beta[s] * Sn[cc] + A.gamma[s] * Snp1[cc]);
+      }
+    }
+    if (STRAIN) {
+      realw* Ep = A.eps;
+      const int64_t nl = A.nloc;
+      Ep[0 * nl + ep] = e_xx; Ep[1 * nl + ep] = e_yy; Ep[2 * nl + ep] = e_xy;
+      Ep[3 * nl + ep] = e_xz; Ep[4 * nl + ep] = e_yz;
+      if (ATT) Ep[5 * nl + ep] = e_tr;
+    }
+
+    const realw syx = sigma_xy, szx = sigma_xz, szy = sigma_yz;
+    if (irr) {
+      t1[0][k] = jacl * (sigma_xx * xixl + syx * xiyl + szx * xizl);
+      t1[1][k] = jacl * (sigma_xy * xixl + sigma_yy * xiyl + szy * xizl);
+      t1[2][k] = jacl * (sigma_xz * xixl + sigma_yz * xiyl + sigma_zz * xizl);
+      t2[0][k] = jacl * (sigma_xx * etaxl + syx * etayl + szx * etazl);
+      t2[1][k] = jacl * (sigma_xy * etaxl + sigma_yy * etayl + szy * etazl);
+      t2[2][k] = jacl * (sigma_xz * etaxl + sigma_yz * etayl + sigma_zz * etazl);
+      t3[0][k] = jacl * (sigma_xx * gaxl + syx * gayl + szx * gazl);
+      t3[1][k] = jacl * (sigma_xy * gaxl + sigma_yy * gayl + szy * gazl);
+      t3[2][k] = jacl * (sigma_xz * gaxl + sigma_yz * gayl + sigma_zz * gazl);
+    } else {
+      const realw xr = A.xix_regular;
+      t1[0][k] = jacl * sigma_xx * xr;
+      t1[1][k] = jacl * sigma_xy * xr;
+      t1[2][k] = jacl * sigma_xz * xr;
+      t2[0][k] = jacl * syx * xr;
+      t2[1][k] = jacl * sigma_yy * xr;
+      t2[2][k] = jacl * sigma_yz * xr;
+      t3[0][k] = jacl * szx * xr;
+      t3[1][k] = jacl * szy * xr;
+      t3[2][k] = jacl * sigma_zz * xr;
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) {
+        s_buf[el][cc][c + 25 * k] = t1[cc][k];
+        s_buf[el][3 + cc][c + 25 * k] = t2[cc][k];
+      }
+  }
+  __syncthreads();
+  if (!active) return;
+
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const int b1 = 5 * j + 25 * k;
+    const int b2 = i + 25 * k;
+    realw f[3];
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++) {
+      const realw* s1 = s_buf[el][cc];
+      const realw* s2 = s_buf[el][3 + cc];
+      realw n1 = hpwi[0] * s1[b1];
+      realw n2 = hpwj[0] * s2[b2];
+      realw n3 = G.hpw[k] * t3[cc][0];
+#pragma unroll
+      for (int l = 1; l < 5; l++) {
+        n1 = n1 + hpwi[l] * s1[b1 + l];
+        n2 = n2 + hpwj[l] * s2[b2 + 5 * l];
+        n3 = n3 + G.hpw[l * 5 + k] * t3[cc][l];
+      }
+      const realw fac1 = G.wxy[j * 5 + k], fac2 = G.wxy[i * 5 + k], fac3 = G.wxy[i * 5 + j];
+      f[cc] = fac1 * n1 + fac2 * n2 + fac3 * n3;
+    }
+    if (ORDERED) {
+      const int64_t ep = (int64_t)e * 125 + c + 25 * k;
+      A.local_out[ep] = f[0];
+      A.local_out[A.nloc + ep] = f[1];
+      A.local_out[2 * A.nloc + ep] = f[2];
+    } else {
+      realw* a = A.accel + 3 * (int64_t)g[k];
+      atomicAdd(a + 0, -f[0]);
+      atomicAdd(a + 1, -f[1]);
+      atomicAdd(a + 2, -f[2]);
+    }
+  }
+}
+
+// element strain of a displacement field (compute_element_strain.F90:28-130); one lane per point
+__global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, realw* eps, realw* eps_tr3) {
+  __shared__ realw s_buf[5][3][128];
+  const int tid = threadIdx.x;
+  const int el = tid / 25;
+  const int c = tid - el * 25;
+  const int i = c % 5, j = c / 5;
+  const int ie = blockIdx.x * 5 + el;
+  const bool active = (tid < 125) && (ie < A.nelem);
+  const int e = active ? A.elems[ie] : 0;
+  realw ux[5], uy[5], uz[5];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const int g = A.ibool[(int64_t)e * 125 + c + 25 * k];
+      ux[k] = A.displ[3 * (int64_t)g]; uy[k] = A.displ[3 * (int64_t)g + 1]; uz[k] = A.displ[3 * (int64_t)g + 2];
+      s_buf[el][0][c + 25 * k] = ux[k]; s_buf[el][1][c + 25 * k] = uy[k]; s_buf[el][2][c + 25 * k] = uz[k];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int irr = A.irregular[e];
+  const realw ONE_THIRD = (realw)1 / (realw)3;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const int p = c + 25 * k;
+    const int b1 = 5 * j + 25 * k, b2 = i + 25 * k;
+    const realw* sx = s_buf[el][0];
+    const realw* sy = s_buf[el][1];
+    const realw* sz = s_buf[el][2];
+    realw tx1 = G.hp[i * 5] * sx[b1], ty1 = G.hp[i * 5] * sy[b1], tz1 = G.hp[i * 5] * sz[b1];
+    realw tx2 = G.hp[j * 5] * sx[b2], ty2 = G.hp[j * 5] * sy[b2], tz2 = G.hp[j * 5] * sz[b2];
+    realw tx3 = G.hp[k * 5] * ux[0], ty3 = G.hp[k * 5] * uy[0], tz3 = G.hp[k * 5] * uz[0];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      tx1 = tx1 + G.hp[i * 5 + l] * sx[b1 + l]; ty1 = ty1 + G.hp[i * 5 + l] * sy[b1 + l];
+      tz1 = tz1 + G.hp[i * 5 + l] * sz[b1 + l];
+      tx2 = tx2 + G.hp[j * 5 + l] * sx[b2 + 5 * l]; ty2 = ty2 + G.hp[j * 5 + l] * sy[b2 + 5 * l];
+      tz2 = tz2 + G.hp[j * 5 + l] * sz[b2 + 5 * l];
+      tx3 = tx3 + G.hp[k * 5 + l] * ux[l]; ty3 = ty3 + G.hp[k * 5 + l] * uy[l]; tz3 = tz3 + G.hp[k * 5 + l] * uz[l];
+    }
+    realw duxdx, duxdy, duxdz, duydx, duydy, duydz, duzdx, duzdy, duzdz;
+    if (irr) {
+      const int64_t q = (int64_t)(irr - 1) * 125 + p;
+      const int64_t P = A.jac_plane;
+      const realw* J = A.jac;
+      const realw xixl = J[q], xiyl = J[P + q], xizl = J[2 * P + q], etaxl = J[3 * P + q], etayl = J[4 * P + q];
+      const realw etazl = J[5 * P + q], gaxl = J[6 * P + q], gayl = J[7 * P + q], gazl = J[8 * P + q];
+      duxdx = xixl * tx1 + etaxl * tx2 + gaxl * tx3;
+      duxdy = xiyl * tx1 + etayl * tx2 + gayl * tx3;
+      duxdz = xizl * tx1 + etazl * tx2 + gazl * tx3;
+      duydx = xixl * ty1 + etaxl * ty2 + gaxl * ty3;
+      duydy = xiyl * ty1 + etayl * ty2 + gayl * ty3;
+      duydz = xizl * ty1 + etazl * ty2 + gazl * ty3;
+      duzdx = xixl * tz1 + etaxl * tz2 + gaxl * tz3;
+      duzdy = xiyl * tz1 + etayl * tz2 + gayl * tz3;
+      duzdz = xizl * tz1 + etazl * tz2 + gazl * tz3;
+    } else {
+      const realw xr = A.xix_regular;
+      duxdx = xr * tx1; duxdy = xr * tx2; duxdz = xr * tx3;
+      duydx = xr * ty1; duydy = xr * ty2; duydz = xr * ty3;
+      duzdx = xr * tz1; duzdy = xr * tz2; duzdz = xr * tz3;
+    }
+    const realw templ = ONE_THIRD * (duxdx + duydy + duzdz);
+    const int64_t ep = (int64_t)e * 125 + p;
+    const int64_t nl = A.nloc;
+    eps_tr3[ep] = templ;
+    eps[0 * nl + ep] = duxdx - templ;
+    eps[1 * nl + ep] = duydy - templ;
+    eps[2 * nl + ep] = (realw)0.5 * (duxdy + duydx);
+    eps[3 * nl + ep] = (realw)0.5 * (duzdx + duxdz);
+    eps[4 * nl + ep] = (realw)0.5 * (duzdy + duydz);
+  }
+}
+
+}  // namespace SF_VNS
+using namespace SF_VNS;
+
+void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool ordered) {
+  if (a.nelem <= 0) return;
+  dim3 grid((a.nelem + 4) / 5), block(128);
+  const GLLk G = g_gll_host;
+#define SF_K(S, T, O) hipLaunchKernelGGL((k_elastic<S, T, O>), grid, block, 0, s, a, G)
+  if (ordered) {
+    if (att) SF_K(true, true, true);
+    else if (strain) SF_K(true, false, true);
+    else SF_K(false, false, true);
+  } else {
+    if (att) SF_K(true, true, false);
+    else if (strain) SF_K(true, false, false);
+    else SF_K(false, false, false);
+  }
+#undef SF_K
+  SF_LAUNCH_CHECK("k_elastic");
+}
+
+void SFX(launch_element_strain)(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3) {
+  if (a.nelem <= 0) return;
+  dim3 grid((a.nelem + 4) / 5), block(128);
+  hipLaunchKernelGGL(k_element_strain, grid, block, 0, s, a, g_gll_host, eps, eps_tr3);
+  SF_LAUNCH_CHECK("k_element_strain");
+}
+
+}  // namespace sf

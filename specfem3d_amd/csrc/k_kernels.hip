@@ -1,0 +1,101 @@
+// k_kernels.hip -- sensitivity-kernel accumulation, ordered assembly gather, max-norm.
+// Built with -ffp-contract=off (reference rounding).
+#include "kernels.h"
+
+namespace sf {
+
+// compute_kernels.f90:151-186 (isotropic), one lane per local point of the elastic elements:
+//   rho_kl   += dt (a . b_u)
+//   mu_kl    += dt (e1 b1 + e2 b2 + (e1+e2)(b1+b2) + 2 (e3 b3 + e4 b4 + e5 b5))
+//   kappa_kl += dt (9 tr/3 b_tr/3)
+__global__ void k_compute_kernels_el(int nelem, const int* __restrict__ elems, const int* __restrict__ ibool,
+                                     const realw* __restrict__ accel, const realw* __restrict__ b_displ,
+                                     const realw* __restrict__ eps, const realw* __restrict__ etr3,
+                                     const realw* __restrict__ beps, const realw* __restrict__ btr3, int64_t nl,
+                                     realw deltat, realw* rho_kl, realw* mu_kl, realw* kappa_kl) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nelem * 125) return;
+  const int e = elems[t / 125];
+  const int p = (int)(t % 125);
+  const int64_t ep = (int64_t)e * 125 + p;
+  const int64_t g = ibool[ep];
+  const realw dot = accel[3 * g] * b_displ[3 * g] + accel[3 * g + 1] * b_displ[3 * g + 1] +
+                    accel[3 * g + 2] * b_displ[3 * g + 2];
+  rho_kl[ep] = rho_kl[ep] + deltat * dot;
+  const realw e1 = eps[ep], e2 = eps[nl + ep], e3 = eps[2 * nl + ep], e4 = eps[3 * nl + ep], e5 = eps[4 * nl + ep];
+  const realw b1 = beps[ep], b2 = beps[nl + ep], b3 = beps[2 * nl + ep], b4 = beps[3 * nl + ep],
+              b5 = beps[4 * nl + ep];
+  mu_kl[ep] = mu_kl[ep] + deltat * (e1 * b1 + e2 * b2 + (e1 + e2) * (b1 + b2) + (realw)2 * (e3 * b3 + e4 * b4 + e5 * b5));
+  kappa_kl[ep] = kappa_kl[ep] + deltat * ((realw)9 * etr3[ep] * btr3[ep]);
+}
+
+// ordered assembly: for each global point, subtract its element contributions in increasing
+// element order (the order of the reference's serial element loop). ptr/idx is a CSR list of
+// local slots e*125+p per global point, sorted by element.
+__global__ void k_ordered_gather(int nglob, int ncomp, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                 const realw* __restrict__ local, int64_t nl, realw* out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nglob) return;
+  const int b = ptr[g], e = ptr[g + 1];
+  if (b == e) return;
+  for (int c = 0; c < ncomp; c++) {
+    realw a = out[(int64_t)g * ncomp + c];
+    for (int q = b; q < e; q++) a = a - local[c * nl + idx[q]];
+    out[(int64_t)g * ncomp + c] = a;
+  }
+}
+
+// max over points of the vector norm (check_stability.f90); NaN propagates as +inf bits
+__global__ void k_max_norm(int64_t npts, int ncomp, const realw* __restrict__ f, unsigned int* out) {
+  __shared__ realw sm[256];
+  realw m = 0.f;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < npts; t += (int64_t)gridDim.x * blockDim.x) {
+    realw v;
+    if (ncomp == 3) {
+      const realw x = f[3 * t], y = f[3 * t + 1], z = f[3 * t + 2];
+      v = sqrtf(x * x + y * y + z * z);
+    } else {
+      v = fabsf(f[t]);
+    }
+    if (v != v) v = INFINITY;
+    m = fmaxf(m, v);
+  }
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(sm[0]));  // non-negative floats order as uints
+}
+
+__global__ void k_zero_u32(unsigned int* p) { *p = 0u; }
+
+void launch_compute_kernels_el(hipStream_t s, int nelem, const int* elems, const int* ibool, const realw* accel,
+                               const realw* b_displ, const realw* eps, const realw* eps_tr3, const realw* b_eps,
+                               const realw* b_eps_tr3, int64_t nloc, realw deltat, realw* rho_kl, realw* mu_kl,
+                               realw* kappa_kl) {
+  if (nelem <= 0) return;
+  hipLaunchKernelGGL(k_compute_kernels_el, sf_grid((int64_t)nelem * 125, 256), dim3(256), 0, s, nelem, elems, ibool,
+                     accel, b_displ, eps, eps_tr3, b_eps, b_eps_tr3, nloc, deltat, rho_kl, mu_kl, kappa_kl);
+  SF_LAUNCH_CHECK("k_compute_kernels_el");
+}
+
+void launch_ordered_gather(hipStream_t s, int nglob, int ncomp, const int* ptr, const int* idx, const realw* local,
+                           int64_t nloc, realw* out) {
+  hipLaunchKernelGGL(k_ordered_gather, sf_grid(nglob, 256), dim3(256), 0, s, nglob, ncomp, ptr, idx, local, nloc,
+                     out);
+  SF_LAUNCH_CHECK("k_ordered_gather");
+}
+
+void launch_max_norm(hipStream_t s, int64_t npts, int ncomp, const realw* f, realw* out) {
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, reinterpret_cast<unsigned int*>(out));
+  int64_t nb = (npts + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_max_norm, dim3((unsigned)nb), dim3(256), 0, s, npts, ncomp, f,
+                     reinterpret_cast<unsigned int*>(out));
+  SF_LAUNCH_CHECK("k_max_norm");
+}
+
+}  // namespace sf

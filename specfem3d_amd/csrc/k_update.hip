@@ -1,0 +1,118 @@
+// k_update.hip -- Newmark time-marching streams (pure HBM-bound, float4 vectorised).
+//
+// predictor   (update_displacement_scheme.f90:359-410, :206-227):
+//     u += dt v + dt^2/2 a ; v += dt/2 a ; a = 0
+// corrector   (compute_forces_viscoelastic_calling_routine.F90:368-374 + update_veloc_elastic):
+//     a *= rmass^-1 ; v += dt/2 a
+// corrector+predictor fused: the corrector of step n followed by the predictor of step n+1
+// in one pass (same arithmetic, one read and one write of u, v, a instead of two).
+// All arrays are flat (3*nglob for vector fields, nglob for potentials) and the inverse
+// mass is stored with the same layout, so each thread handles 4 consecutive floats.
+// Built with -ffp-contract=off: identical rounding to the CPU reference.
+#include "kernels.h"
+
+namespace sf {
+
+__global__ void k_predictor(int64_t n4, int64_t n, realw* __restrict__ u, realw* __restrict__ v,
+                            realw* __restrict__ a, realw dt, realw dt2, realw dtsq2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n4) {
+    float4 U = reinterpret_cast<float4*>(u)[t];
+    float4 V = reinterpret_cast<float4*>(v)[t];
+    const float4 A = reinterpret_cast<float4*>(a)[t];
+    U.x = U.x + dt * V.x + dtsq2 * A.x; V.x = V.x + dt2 * A.x;
+    U.y = U.y + dt * V.y + dtsq2 * A.y; V.y = V.y + dt2 * A.y;
+    U.z = U.z + dt * V.z + dtsq2 * A.z; V.z = V.z + dt2 * A.z;
+    U.w = U.w + dt * V.w + dtsq2 * A.w; V.w = V.w + dt2 * A.w;
+    reinterpret_cast<float4*>(u)[t] = U;
+    reinterpret_cast<float4*>(v)[t] = V;
+    reinterpret_cast<float4*>(a)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // scalar tail
+  const int64_t r = 4 * n4 + t;
+  if (t < 4 && r < n) {
+    u[r] = u[r] + dt * v[r] + dtsq2 * a[r];
+    v[r] = v[r] + dt2 * a[r];
+    a[r] = 0.f;
+  }
+}
+
+__global__ void k_corrector(int64_t n4, int64_t n, realw* __restrict__ v, realw* __restrict__ a,
+                            const realw* __restrict__ rm, realw dt2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n4) {
+    float4 V = reinterpret_cast<float4*>(v)[t];
+    float4 A = reinterpret_cast<float4*>(a)[t];
+    const float4 M = reinterpret_cast<const float4*>(rm)[t];
+    A.x = A.x * M.x; A.y = A.y * M.y; A.z = A.z * M.z; A.w = A.w * M.w;
+    V.x = V.x + dt2 * A.x; V.y = V.y + dt2 * A.y; V.z = V.z + dt2 * A.z; V.w = V.w + dt2 * A.w;
+    reinterpret_cast<float4*>(v)[t] = V;
+    reinterpret_cast<float4*>(a)[t] = A;
+  }
+  const int64_t r = 4 * n4 + t;
+  if (t < 4 && r < n) {
+    a[r] = a[r] * rm[r];
+    v[r] = v[r] + dt2 * a[r];
+  }
+}
+
+__global__ void k_corrector_predictor(int64_t n4, int64_t n, realw* __restrict__ u, realw* __restrict__ v,
+                                      realw* __restrict__ a, const realw* __restrict__ rm, realw dt2p, realw dt,
+                                      realw dt2, realw dtsq2) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n4) {
+    float4 U = reinterpret_cast<float4*>(u)[t];
+    float4 V = reinterpret_cast<float4*>(v)[t];
+    float4 A = reinterpret_cast<float4*>(a)[t];
+    const float4 M = reinterpret_cast<const float4*>(rm)[t];
+#define SF_CP(c)                               \
+  A.c = A.c * M.c;                             \
+  V.c = V.c + dt2p * A.c;                      \
+  U.c = U.c + dt * V.c + dtsq2 * A.c;          \
+  V.c = V.c + dt2 * A.c;
+    SF_CP(x) SF_CP(y) SF_CP(z) SF_CP(w)
+#undef SF_CP
+    reinterpret_cast<float4*>(u)[t] = U;
+    reinterpret_cast<float4*>(v)[t] = V;
+    reinterpret_cast<float4*>(a)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const int64_t r = 4 * n4 + t;
+  if (t < 4 && r < n) {
+    realw ar = a[r] * rm[r];
+    realw vr = v[r] + dt2p * ar;
+    u[r] = u[r] + dt * vr + dtsq2 * ar;
+    v[r] = vr + dt2 * ar;
+    a[r] = 0.f;
+  }
+}
+
+static dim3 grid_for(int64_t n4, int block) {
+  int64_t b = (n4 + block - 1) / block;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+void launch_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, realw dt, realw dt2, realw dtsq2) {
+  if (n <= 0) return;
+  const int64_t n4 = n / 4;
+  hipLaunchKernelGGL(k_predictor, grid_for(n4, 256), dim3(256), 0, s, n4, n, u, v, a, dt, dt2, dtsq2);
+  SF_LAUNCH_CHECK("k_predictor");
+}
+
+void launch_corrector(hipStream_t s, int64_t n, realw* v, realw* a, const realw* rmass, realw dt2) {
+  if (n <= 0) return;
+  const int64_t n4 = n / 4;
+  hipLaunchKernelGGL(k_corrector, grid_for(n4, 256), dim3(256), 0, s, n4, n, v, a, rmass, dt2);
+  SF_LAUNCH_CHECK("k_corrector");
+}
+
+void launch_corrector_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, const realw* rmass,
+                                realw dt2_prev, realw dt, realw dt2, realw dtsq2) {
+  if (n <= 0) return;
+  const int64_t n4 = n / 4;
+  hipLaunchKernelGGL(k_corrector_predictor, grid_for(n4, 256), dim3(256), 0, s, n4, n, u, v, a, rmass, dt2_prev,
+                     dt, dt2, dtsq2);
+  SF_LAUNCH_CHECK("k_corrector_predictor");
+}
+
+}  // namespace sf

@@ -1,0 +1,95 @@
+// kernels.h -- launchers of the device kernels (all asynchronous on the given stream).
+#pragma once
+
+#include "mesh.h"
+
+namespace sf {
+
+// GLL constants passed to the element kernels by value (kernel-argument segment, read through
+// the scalar cache): hp[i*5+l] = l'_l(x_i), hpw[i*5+l] = hp[i*5+l] w_i, wxy[a*5+b] = w_a w_b.
+struct GLLk {
+  realw hp[25];
+  realw hpw[25];
+  realw wxy[25];
+};
+extern GLLk g_gll_host;
+void set_gll_constants(const realw* hprime, const realw* hprimewgll, const realw* wgllwgll);
+
+// Newmark pieces (k_update.hip)
+void launch_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, realw dt, realw dt2, realw dtsq2);
+void launch_corrector(hipStream_t s, int64_t n, realw* v, realw* a, const realw* rmass, realw dt2);
+void launch_corrector_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, const realw* rmass,
+                                realw dt2_prev, realw dt, realw dt2, realw dtsq2);
+
+// element stiffness (k_elastic.hip / k_acoustic.hip)
+struct ElasticArgs {
+  int nelem;
+  const int* elems;
+  const int* ibool;
+  const int* irregular;
+  realw xix_regular, jacobian_regular;
+  const realw* jac;        // 10 planes
+  int64_t jac_plane;       // nspec_irr*125
+  const float2* kappamu;
+  const realw* displ;
+  const realw* veloc;
+  realw* accel;
+  realw deltat;
+  realw* R;                // 18 planes of nloc
+  realw* eps;              // 6 planes of nloc
+  const realw* fc;         // 3 planes
+  const realw* fck;        // 3 planes
+  realw* eps_tr3;          // nloc or null
+  int64_t nloc;            // nspec*125
+  realw alpha[3], beta[3], gamma[3];
+  realw* local_out;        // ordered assembly: 3 planes of nloc
+};
+void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, bool ordered);
+void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool exact);
+
+struct AcousticArgs {
+  int nelem;
+  const int* elems;
+  const int* ibool;
+  const int* irregular;
+  realw xix_regular, jacobian_regular;
+  const realw* jac;
+  int64_t jac_plane;
+  const realw* rho;
+  const realw* pot;
+  realw* potdd;
+  realw* local_out;        // ordered assembly: 1 plane of nloc
+};
+void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, bool ordered);
+
+// ordered assembly: out[g*ncomp + c] -= contributions in element order
+void launch_ordered_gather(hipStream_t s, int nglob, int ncomp, const int* ptr, const int* idx,
+                           const realw* local, int64_t nloc, realw* out);
+
+// boundary terms (k_boundary.hip)
+void launch_stacey_elastic(hipStream_t s, const FacePoints& f, const realw* veloc, realw* accel, realw* b_absorb,
+                           int mode);
+void launch_stacey_elastic_replay(hipStream_t s, const FacePoints& f, const realw* rec, realw* b_accel);
+void launch_stacey_acoustic(hipStream_t s, const FacePoints& f, const realw* potd, realw* potdd, realw* b_absorb,
+                            int mode);
+void launch_stacey_acoustic_replay(hipStream_t s, const FacePoints& f, const realw* rec, realw* b_potdd);
+void launch_coupling_ac_el(hipStream_t s, const FacePoints& f, const realw* displ, realw* potdd, int flip);
+void launch_coupling_el_ac(hipStream_t s, const FacePoints& f, const realw* potdd, realw* accel, int flip);
+void launch_free_surface(hipStream_t s, const FacePoints& f, realw* pot, realw* potd, realw* potdd);
+
+// sources / receivers (k_sources.hip); FacePoints.ispec holds the owning source / receiver
+void launch_add_sources_el(hipStream_t s, const FacePoints& f, const realw* arr, const realw* stf, realw* accel);
+void launch_add_sources_ac(hipStream_t s, const FacePoints& f, const realw* arr, const realw* kappa,
+                           const realw* stf, realw* potdd);
+void launch_adjoint_sources_el(hipStream_t s, const FacePoints& f, const realw* h, const realw* adj, realw* accel);
+void launch_seismo_el(hipStream_t s, int nrec, const int* iglob, const double* h, const double* nu,
+                      const realw* field, realw* out);
+
+// kernels / norms (k_kernels.hip)
+void launch_compute_kernels_el(hipStream_t s, int nelem, const int* elems, const int* ibool, const realw* accel,
+                               const realw* b_displ, const realw* eps, const realw* eps_tr3, const realw* b_eps,
+                               const realw* b_eps_tr3, int64_t nloc, realw deltat, realw* rho_kl, realw* mu_kl,
+                               realw* kappa_kl);
+void launch_max_norm(hipStream_t s, int64_t npts, int ncomp, const realw* f, realw* out);
+
+}  // namespace sf

@@ -1,0 +1,167 @@
+// mesh.h -- the opaque handle behind the reference's `long* Mesh_pointer`.
+//
+// Device data layout (chosen for gfx950, not copied from the reference's Mesh struct):
+//   * wavefields displ/veloc/accel: float[3*nglob], xyz interleaved (same as Fortran
+//     displ(3,NGLOB), so host transfers are plain copies);
+//   * ibool: int32[nspec*125] 0-based, i fastest (Fortran ibool(i,j,k,ispec) order);
+//   * kappa/mu: one float2 per local point so each point needs one 8-byte load;
+//   * irregular Jacobians: 10 SoA planes [t][nspec_irr*125] (xix..gammaz, jacobian) so a
+//     wavefront's 64 lanes read 64 consecutive floats of one plane;
+//   * attenuation memory: SoA planes [comp*3 + sls][nspec*125], comp = xx,yy,xy,xz,yz,trace;
+//   * inverse mass: float[3*nglob] interleaved (rmassx,rmassy,rmassz) so the corrector is a
+//     pure float4 stream over the flat 3*nglob arrays;
+//   * boundary faces are flattened to face points with the global index, normal, weight and
+//     the per-point impedances gathered once at prepare time.
+#pragma once
+
+#include "common.h"
+
+namespace sf {
+
+struct Wavefield {
+  realw* displ = nullptr;
+  realw* veloc = nullptr;
+  realw* accel = nullptr;
+};
+
+struct AcField {
+  realw* pot = nullptr;
+  realw* potd = nullptr;
+  realw* potdd = nullptr;
+};
+
+struct AttField {            // memory variables + stored strain of one elastic wavefield
+  realw* R = nullptr;        // 18 planes of nloc (comp-major, sls-minor)
+  realw* eps = nullptr;      // 6 planes of nloc: xx,yy,xy,xz,yz,trace
+};
+
+struct FacePoints {          // flattened boundary/coupling face points
+  int n = 0;                 // number of face points (faces * 25)
+  int* iglob = nullptr;
+  int* ispec = nullptr;      // element of the face point (0-based)
+  int* ploc = nullptr;       // local point index p = i + 5j + 25k
+  realw* normal = nullptr;   // 3*n
+  realw* jacw = nullptr;     // n
+  realw* imp_p = nullptr;    // rho*vp (elastic Stacey) or 1/(rho*c) factor (acoustic)
+  realw* imp_s = nullptr;    // rho*vs (elastic Stacey)
+  // Points are sorted into "rounds": round r holds the r-th occurrence of each global point in
+  // the host's face order, so one launch per round can update without atomics and the sum at
+  // shared edge/corner points keeps the reference's face order.
+  int nrounds = 0;
+  int round_off[17] = {0};
+  int* slot = nullptr;       // original host slot (face*25 + igll) of each point, for records
+};
+
+struct Mesh {
+  int nspec = 0, nglob = 0, nspec_irregular = 0;
+  int simulation_type = 1;
+  int save_forward = 0;
+  int myrank = 0;
+
+  // geometry
+  int* ibool = nullptr;
+  int* irregular = nullptr;
+  realw xix_regular = 0.f, jacobian_regular = 0.f;
+  realw* jac = nullptr;            // 10 planes of nspec_irr*125
+  int jac_from_host = 0;           // 1 if the determinant came from the host (sf_set_jacobian)
+  int* h_ibool = nullptr;          // host copy (0-based) for host-side index work
+
+  // domains
+  int nspec_elastic = 0, nspec_acoustic = 0;
+  int* elastic_elems = nullptr;    // element lists in processing order
+  int* acoustic_elems = nullptr;
+  unsigned char* h_is_elastic = nullptr;
+  unsigned char* h_is_acoustic = nullptr;
+
+  // elastic material and mass
+  float2* kappamu = nullptr;
+  realw* rho = nullptr;            // rhostore (acoustic forces, kernels)
+  realw* kappa_ac = nullptr;       // kappastore for acoustic elements
+  realw* rmass = nullptr;          // 3*nglob inverse mass (elastic)
+  realw* rmass_ac = nullptr;       // nglob inverse mass (acoustic)
+
+  // wavefields: [0] forward/adjoint, [1] backward/reconstructed
+  Wavefield el[2];
+  AcField ac[2];
+  int compute_and_store_strain = 0;
+  int attenuation = 0;
+  AttField att[2];
+  realw* factor_common = nullptr;  // 3 planes (mu)
+  realw* factor_common_kappa = nullptr;  // 3 planes
+  realw alphaval[N_SLS], betaval[N_SLS], gammaval[N_SLS];
+  realw b_alphaval[N_SLS], b_betaval[N_SLS], b_gammaval[N_SLS];
+  realw* eps_trace_over_3[2] = {nullptr, nullptr};
+
+  // boundaries
+  int stacey = 0;
+  FacePoints abs_el, abs_ac;       // Stacey face points in elastic / acoustic elements
+  realw* b_absorb_field = nullptr; // device buffer for one step of stored tractions (3*abs_el.n)
+  realw* b_absorb_potential = nullptr;
+  FacePoints free_ac;              // acoustic free-surface points
+  FacePoints coupling;             // fluid-solid coupling points (on acoustic elements)
+  int num_abs_boundary_faces = 0;
+  int* h_abs_face_point_el = nullptr;  // host map from record slot (face*25+igll) to abs_el index
+
+  // sources
+  int nsources = 0;
+  FacePoints src_pts;              // (source, point) pairs of sources in elastic elements
+  realw* src_array = nullptr;      // nsrc*125*3
+  unsigned char* h_src_elastic = nullptr;
+  unsigned char* h_src_acoustic = nullptr;
+  realw* src_kappa = nullptr;      // nsrc*125 kappa at source points (acoustic sources)
+
+  // receivers
+  int nrec_local = 0;
+  int* rec_iglob = nullptr;        // nrec*125
+  double* rec_h = nullptr;         // nrec*125 products hxi*heta*hgamma (double)
+  double* rec_nu = nullptr;        // nrec*9
+  int nlength_seismogram = 0;
+  realw* seismo_d = nullptr;       // device buffers [nlength][nrec][3]
+  realw* seismo_v = nullptr;
+  realw* seismo_a = nullptr;
+  realw* seismo_p = nullptr;
+  int save_d = 0, save_v = 0, save_a = 0, save_p = 0;
+  unsigned char* h_rec_elastic = nullptr;
+  int* h_rec_ispec = nullptr;
+
+  // adjoint sources (SIMULATION_TYPE 2/3)
+  int nadj_rec_local = 0;
+  int ntstep_between_read_adjsrc = 0;
+  realw* adj_h = nullptr;          // nadj*125 float products (CUSTOM_REAL hlagrange)
+  FacePoints adj_pts;              // (adjoint receiver, point) pairs
+  realw* adj_src = nullptr;        // one step of adjoint sources (3*nadj)
+  unsigned char* h_adj_elastic = nullptr;
+
+  // sensitivity kernels
+  realw* rho_kl = nullptr;
+  realw* mu_kl = nullptr;
+  realw* kappa_kl = nullptr;
+  realw* rho_ac_kl = nullptr;
+  realw* kappa_ac_kl = nullptr;
+
+  // execution
+  hipStream_t stream = nullptr;
+  int exact = 1;                   // 1: kernels built without fused multiply-add (oracle order)
+  int assembly = 0;                // 0 atomic scatter-add, 1 ordered gather (oracle order)
+  // deferred corrector (kernel_3) per field, fused into the next predictor when possible
+  int pending_k3_el[2] = {0, 0};
+  realw pending_dt2_el[2] = {0.f, 0.f};
+  int pending_k3_ac[2] = {0, 0};
+  realw pending_dt2_ac[2] = {0.f, 0.f};
+  // ordered-assembly structures (CSR of contributions per global point, sorted by element)
+  int* asm_ptr = nullptr;          // nglob+1
+  int* asm_idx = nullptr;          // slots e*125+p
+  realw* asm_buf = nullptr;        // 3*nspec*125 per-element local forces
+  int asm_ready = 0;
+  int* asm_ac_ptr = nullptr;       // same for the acoustic elements (1 component)
+  int* asm_ac_idx = nullptr;
+  realw* asm_ac_buf = nullptr;
+
+  // scratch for reductions
+  realw* d_norm = nullptr;
+  realw* h_norm = nullptr;         // pinned
+};
+
+Mesh* from_handle(long* p);
+
+}  // namespace sf

@@ -1,0 +1,392 @@
+"""Solver database: the arrays the Fortran host hands to the C-ABI.
+
+The arrays follow the reference's semantics (``src/generate_databases/``):
+
+* ``ibool``: global point number of each local GLL point, stored here 0-based with
+  shape ``(nspec, 5, 5, 5)`` indexed ``[ispec, k, j, i]`` -- the same memory order as
+  Fortran ``ibool(i,j,k,ispec)``; the C-ABI receives it 1-based.
+* regular elements (axis-aligned cubes of one edge length) carry no Jacobian
+  arrays, only the scalars ``xix_regular`` and ``jacobian_regular``
+  (``calc_jacobian.f90:184-345``, ``check_element_regularity``).
+* material arrays ``kappastore = rho (vp^2 - 4/3 vs^2)``, ``mustore = rho vs^2``,
+  ``rho_vp``, ``rho_vs`` computed in float32 (``get_model.F90:289-301``).
+* mass matrices: elastic ``rmass = sum jacobian w_i w_j w_k rho`` accumulated in
+  float64-per-term then float32 (``define_mass_matrices.f90:30-60``); the Stacey
+  contribution ``rmassx/y/z`` (``define_mass_matrices.f90:353-442``); combined and
+  inverted as in ``prepare_timerun.F90:246-345``.
+* boundary faces carry 25 points each with outward unit normals and the weighted
+  surface Jacobian ``jacobian2Dw`` (``get_jacobian_boundaries.f90:303-365``).
+* coupling faces are stored on the acoustic element with normals pointing away
+  from the fluid (``get_coupling_surfaces.f90:326``).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ..gll import get_gll, NGLL
+
+# face ids as in the reference: 1 xmin, 2 xmax, 3 ymin, 4 ymax, 5 bottom (zmin), 6 top (zmax)
+FACE_XMIN, FACE_XMAX, FACE_YMIN, FACE_YMAX, FACE_ZMIN, FACE_ZMAX = range(1, 7)
+
+
+def face_ijk(iface: int) -> np.ndarray:
+    """(25, 3) 0-based (i, j, k) of the GLL points of element face ``iface``.
+
+    Point order: the two in-face indices run with the first one fastest.
+    """
+    a, b = np.meshgrid(np.arange(NGLL), np.arange(NGLL), indexing="xy")
+    a = a.ravel()
+    b = b.ravel()
+    z = np.zeros(25, dtype=np.int32)
+    e = np.full(25, NGLL - 1, dtype=np.int32)
+    if iface == FACE_XMIN:
+        return np.stack([z, a, b], 1).astype(np.int32)
+    if iface == FACE_XMAX:
+        return np.stack([e, a, b], 1).astype(np.int32)
+    if iface == FACE_YMIN:
+        return np.stack([a, z, b], 1).astype(np.int32)
+    if iface == FACE_YMAX:
+        return np.stack([a, e, b], 1).astype(np.int32)
+    if iface == FACE_ZMIN:
+        return np.stack([a, b, z], 1).astype(np.int32)
+    if iface == FACE_ZMAX:
+        return np.stack([a, b, e], 1).astype(np.int32)
+    raise ValueError(iface)
+
+
+@dataclass
+class Faces:
+    """A set of element faces with 25 GLL points each (0-based indices)."""
+    ispec: np.ndarray                       # (nf,) int32
+    ijk: np.ndarray                         # (nf, 25, 3) int32, [i, j, k]
+    normal: np.ndarray                      # (nf, 25, 3) float32
+    jacobian2Dw: np.ndarray                 # (nf, 25) float32
+
+    @property
+    def n(self) -> int:
+        return int(self.ispec.shape[0])
+
+    @staticmethod
+    def empty() -> "Faces":
+        return Faces(np.zeros(0, np.int32), np.zeros((0, 25, 3), np.int32),
+                     np.zeros((0, 25, 3), np.float32), np.zeros((0, 25), np.float32))
+
+
+@dataclass
+class Database:
+    nspec: int
+    nglob: int
+    ibool: np.ndarray                       # (nspec,5,5,5) int32 0-based, [e,k,j,i]
+    xstore: np.ndarray                      # (nglob,) float32
+    ystore: np.ndarray
+    zstore: np.ndarray
+    irregular_element_number: np.ndarray    # (nspec,) int32: 0 regular, else 1-based irregular index
+    nspec_irregular: int
+    xix_regular: np.float32
+    jacobian_regular: np.float32
+    # irregular Jacobian terms (nspec_irregular, 125) float32, order [xix,xiy,xiz,etax,...,gammaz]
+    jac9: np.ndarray
+    jacobian: np.ndarray                    # (nspec_irregular,125)
+    rhostore: np.ndarray                    # (nspec,125) float32
+    kappastore: np.ndarray
+    mustore: np.ndarray
+    rho_vp: np.ndarray
+    rho_vs: np.ndarray
+    ispec_is_elastic: np.ndarray            # (nspec,) bool
+    ispec_is_acoustic: np.ndarray
+    anchors: np.ndarray                     # (nspec, 8, 3) float64 HEX8 anchor coordinates
+    abs_faces: Faces = field(default_factory=Faces.empty)
+    free_faces: Faces = field(default_factory=Faces.empty)
+    coupling_faces: Faces = field(default_factory=Faces.empty)
+    # mass matrices (inverted, float32): filled by finalize()
+    rmassx: np.ndarray | None = None
+    rmassy: np.ndarray | None = None
+    rmassz: np.ndarray | None = None
+    rmass_acoustic: np.ndarray | None = None
+    # attenuation (filled by attenuation.setup_attenuation)
+    attenuation: object | None = None
+    qmu: np.ndarray | None = None           # (nspec,125) float32 or None
+    qkappa: np.ndarray | None = None
+    dt: float | None = None
+    stacey: bool = True
+    meta: dict = field(default_factory=dict)
+
+    # -- derived helpers --------------------------------------------------------
+    @property
+    def nspec_elastic(self) -> int:
+        return int(self.ispec_is_elastic.sum())
+
+    @property
+    def nspec_acoustic(self) -> int:
+        return int(self.ispec_is_acoustic.sum())
+
+    def elastic_elements(self) -> np.ndarray:
+        """0-based elastic element list in increasing order (``phase_ispec_inner_elastic``)."""
+        return np.nonzero(self.ispec_is_elastic)[0].astype(np.int32)
+
+    def acoustic_elements(self) -> np.ndarray:
+        return np.nonzero(self.ispec_is_acoustic)[0].astype(np.int32)
+
+    def gll_coords(self, ispec: int) -> np.ndarray:
+        """(125, 3) float64 coordinates of the GLL points of an element."""
+        ib = self.ibool[ispec].ravel()
+        return np.stack([self.xstore[ib], self.ystore[ib], self.zstore[ib]], 1).astype(np.float64)
+
+    def element_midpoints(self) -> np.ndarray:
+        ib = self.ibool[:, 2, 2, 2]
+        return np.stack([self.xstore[ib], self.ystore[ib], self.zstore[ib]], 1).astype(np.float64)
+
+    # -- mass matrices ------------------------------------------------------------
+    def finalize(self, dt: float, stacey: bool = True) -> "Database":
+        """Mass matrices with the implicit Stacey term, inverted (``prepare_timerun.F90:246-345``)."""
+        self.dt = float(dt)
+        self.stacey = bool(stacey)
+        gll = get_gll()
+        w = gll.wgll
+        w3 = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel()  # [k,j,i] flattened, i fastest
+        nglob = self.nglob
+        ib = self.ibool.reshape(self.nspec, 125)
+        jac = np.empty((self.nspec, 125), dtype=np.float32)
+        reg = self.irregular_element_number == 0
+        jac[reg] = self.jacobian_regular
+        if self.nspec_irregular:
+            irr = ~reg
+            jac[irr] = self.jacobian[self.irregular_element_number[irr] - 1]
+        # elastic mass: per-term real(dble(jac)*w*dble(rho)) summed in float32 in element order
+        if self.nspec_elastic:
+            el = self.ispec_is_elastic
+            terms = (jac[el].astype(np.float64) * w3[None, :] * self.rhostore[el].astype(np.float64)).astype(np.float32)
+            rmass = _ordered_scatter_add(nglob, ib[el], terms)
+            if stacey and self.abs_faces.n:
+                sx, sy, sz = self._stacey_mass_elastic(dt)
+                rmassx, rmassy, rmassz = rmass + sx, rmass + sy, rmass + sz
+            else:
+                rmassx = rmassy = rmassz = rmass
+            out = []
+            for r in (rmassx, rmassy, rmassz):
+                r = r.copy()
+                r[r <= 0] = 1.0
+                out.append((np.float32(1.0) / r).astype(np.float32))
+            self.rmassx, self.rmassy, self.rmassz = out
+        else:
+            one = np.ones(nglob, np.float32)
+            self.rmassx = self.rmassy = self.rmassz = one
+        if self.nspec_acoustic:
+            ac = self.ispec_is_acoustic
+            terms = (jac[ac].astype(np.float64) * w3[None, :] / self.kappastore[ac].astype(np.float64)).astype(np.float32)
+            rma = _ordered_scatter_add(nglob, ib[ac], terms)
+            if stacey and self.abs_faces.n:
+                rma = rma + self._stacey_mass_acoustic(dt)
+            rma[rma <= 0] = 1.0
+            self.rmass_acoustic = (np.float32(1.0) / rma).astype(np.float32)
+        else:
+            self.rmass_acoustic = np.ones(nglob, np.float32)
+        return self
+
+    def _stacey_mass_elastic(self, dt: float):
+        f = self.abs_faces
+        sel = self.ispec_is_elastic[f.ispec]
+        nglob = self.nglob
+        dto2 = np.float32(0.5 * dt)
+        ijk = f.ijk[sel]
+        isp = f.ispec[sel]
+        n = f.normal[sel]
+        jw = f.jacobian2Dw[sel]
+        loc = ijk[..., 0] + 5 * ijk[..., 1] + 25 * ijk[..., 2]
+        rvp = self.rho_vp[isp[:, None], loc]
+        rvs = self.rho_vs[isp[:, None], loc]
+        nx, ny, nz = n[..., 0], n[..., 1], n[..., 2]
+        vn = dto2 * ((nx + ny) + nz)
+        tx = rvp * vn * nx + rvs * (dto2 - vn * nx)
+        ty = rvp * vn * ny + rvs * (dto2 - vn * ny)
+        tz = rvp * vn * nz + rvs * (dto2 - vn * nz)
+        ig = self.ibool.reshape(self.nspec, 125)[isp[:, None], loc]
+        sx = _ordered_scatter_add(nglob, ig, (tx * jw).astype(np.float32))
+        sy = _ordered_scatter_add(nglob, ig, (ty * jw).astype(np.float32))
+        sz = _ordered_scatter_add(nglob, ig, (tz * jw).astype(np.float32))
+        return sx, sy, sz
+
+    def _stacey_mass_acoustic(self, dt: float):
+        f = self.abs_faces
+        sel = self.ispec_is_acoustic[f.ispec]
+        dto2 = np.float32(0.5 * dt)
+        ijk = f.ijk[sel]
+        isp = f.ispec[sel]
+        loc = ijk[..., 0] + 5 * ijk[..., 1] + 25 * ijk[..., 2]
+        rvp = self.rho_vp[isp[:, None], loc]
+        sn = dto2 / rvp
+        ig = self.ibool.reshape(self.nspec, 125)[isp[:, None], loc]
+        return _ordered_scatter_add(self.nglob, ig, (f.jacobian2Dw[sel] * sn).astype(np.float32))
+
+    def summary(self) -> dict:
+        return dict(nspec=self.nspec, nglob=self.nglob, nspec_irregular=self.nspec_irregular,
+                    nspec_elastic=self.nspec_elastic, nspec_acoustic=self.nspec_acoustic,
+                    n_abs_faces=self.abs_faces.n, n_free_faces=self.free_faces.n,
+                    n_coupling_faces=self.coupling_faces.n,
+                    attenuation=self.attenuation is not None)
+
+
+def _ordered_scatter_add(n: int, idx: np.ndarray, vals: np.ndarray) -> np.ndarray:
+    """float32 scatter-add in the serial order of ``idx.ravel()`` (like the reference's loops).
+
+    ``np.add.at`` on a float32 output accumulates in float32 in index order, which
+    is what the Fortran element loop does.
+    """
+    out = np.zeros(n, dtype=np.float32)
+    np.add.at(out, idx.ravel(), vals.ravel().astype(np.float32))
+    return out
+
+
+# ------------------------------------------------------------------------------------
+# geometry helpers shared by the box generator and the external-mesh reader
+# ------------------------------------------------------------------------------------
+
+# HEX8 anchor reference coordinates in SPECFEM order: bottom face (gamma=-1) counter-
+# clockwise, then top face.
+HEX8_REF = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                     [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=np.float64)
+
+
+def hex8_shape(xi, eta, gamma):
+    """Trilinear shape functions N_a and derivatives dN_a/d(xi,eta,gamma) (float64)."""
+    xi = np.asarray(xi, np.float64)
+    eta = np.asarray(eta, np.float64)
+    gamma = np.asarray(gamma, np.float64)
+    r = HEX8_REF
+    fx = 1.0 + np.multiply.outer(xi, r[:, 0])
+    fy = 1.0 + np.multiply.outer(eta, r[:, 1])
+    fz = 1.0 + np.multiply.outer(gamma, r[:, 2])
+    N = 0.125 * fx * fy * fz
+    dN = np.stack([0.125 * r[:, 0] * fy * fz, 0.125 * fx * r[:, 1] * fz, 0.125 * fx * fy * r[:, 2]], axis=-2)
+    return N, dN  # N (..., 8), dN (..., 3, 8)
+
+
+def gll_shape_tables():
+    """Shape function values/derivatives at the 125 GLL points, order [k, j, i] (i fastest)."""
+    x = get_gll().xigll
+    G, E, X = np.meshgrid(x, x, x, indexing="ij")  # [k, j, i]
+    N, dN = hex8_shape(X.ravel(), E.ravel(), G.ravel())
+    return N, dN  # (125, 8), (125, 3, 8)
+
+
+def jacobian_terms(anchors: np.ndarray, chunk: int = 65536):
+    """Inverse-Jacobian terms and det at the GLL points of HEX8 elements.
+
+    Returns float32 arrays jac9 (n, 9, 125) [xix,xiy,xiz,etax,etay,etaz,gammax,gammay,gammaz]
+    and jacobian (n, 125), computed in float64 as ``calc_jacobian.f90:1-100``.
+    """
+    _, dN = gll_shape_tables()
+    n = anchors.shape[0]
+    jac9 = np.empty((n, 9, 125), np.float32)
+    jdet = np.empty((n, 125), np.float32)
+    for s in range(0, n, chunk):
+        a = anchors[s:s + chunk]  # (m, 8, 3)
+        # J[m, p, d, c] = d x_c / d xi_d
+        J = np.einsum("pda,mac->mpdc", dN, a)
+        xxi, yxi, zxi = J[..., 0, 0], J[..., 0, 1], J[..., 0, 2]
+        xeta, yeta, zeta = J[..., 1, 0], J[..., 1, 1], J[..., 1, 2]
+        xga, yga, zga = J[..., 2, 0], J[..., 2, 1], J[..., 2, 2]
+        det = xxi * (yeta * zga - yga * zeta) - xeta * (yxi * zga - yga * zxi) + xga * (yxi * zeta - yeta * zxi)
+        if np.any(det <= 0):
+            raise ValueError("negative or null 3D Jacobian found")
+        terms = [
+            (yeta * zga - yga * zeta) / det, (xga * zeta - xeta * zga) / det, (xeta * yga - xga * yeta) / det,
+            (yga * zxi - yxi * zga) / det, (xxi * zga - xga * zxi) / det, (xga * yxi - xxi * yga) / det,
+            (yxi * zeta - yeta * zxi) / det, (xeta * zxi - xxi * zeta) / det, (xxi * yeta - xeta * yxi) / det,
+        ]
+        jac9[s:s + chunk] = np.stack(terms, 1).astype(np.float32)
+        jdet[s:s + chunk] = det.astype(np.float32)
+    return jac9, jdet
+
+
+def classify_regular(anchors: np.ndarray):
+    """Regular/irregular classification following ``check_element_regularity``.
+
+    An element is regular if it is an axis-aligned cube whose edge matches the
+    first regular cube found; returns (irregular_element_number, nspec_irregular,
+    xix_regular, jacobian_regular).
+    """
+    a = anchors.astype(np.float32).astype(np.float64)  # the reference tests float32 anchors
+    n = a.shape[0]
+    x, y, z = a[..., 0], a[..., 1], a[..., 2]
+    eq = lambda p, q: np.abs(p - q) <= 1e-5 * np.maximum(np.abs(p), np.abs(q)) + 0.0
+    ok = (eq(x[:, 0], x[:, 3]) & eq(x[:, 0], x[:, 4]) & eq(x[:, 0], x[:, 7]) & eq(x[:, 1], x[:, 2])
+          & eq(x[:, 1], x[:, 5]) & eq(x[:, 1], x[:, 6]) & eq(y[:, 0], y[:, 1]) & eq(y[:, 0], y[:, 4])
+          & eq(y[:, 0], y[:, 5]) & eq(y[:, 2], y[:, 3]) & eq(y[:, 2], y[:, 6]) & eq(y[:, 2], y[:, 7])
+          & eq(z[:, 0], z[:, 1]) & eq(z[:, 0], z[:, 2]) & eq(z[:, 0], z[:, 3]) & eq(z[:, 4], z[:, 5])
+          & eq(z[:, 4], z[:, 6]) & eq(z[:, 4], z[:, 7]))
+    d1 = ((a[:, 1] - a[:, 0]) ** 2).sum(-1)
+    d2 = ((a[:, 4] - a[:, 0]) ** 2).sum(-1)
+    d3 = ((a[:, 3] - a[:, 0]) ** 2).sum(-1)
+    thr = 1e-5 * d1
+    cube = ok & (np.abs(d2 - d1) < thr) & (np.abs(d3 - d1) < thr) & (x[:, 1] > x[:, 0]) & (y[:, 3] > y[:, 0]) & (z[:, 4] > z[:, 0])
+    irregular = np.zeros(n, np.int32)
+    xix_reg = np.float32(0.0)
+    jac_reg = np.float32(0.0)
+    if cube.any():
+        first = int(np.argmax(cube))
+        edge2 = d1[first]
+        same = cube & (np.abs(d1 - edge2) <= 1e-5 * edge2)
+        h = np.sqrt(edge2)
+        # xix = 2/h and det = (h/2)^3 in float64, cast to float32 like calc_jacobian
+        xix_reg = np.float32(1.0 / (0.5 * h))
+        jac_reg = np.float32((0.5 * h) ** 3)
+    else:
+        same = cube
+    irr_mask = ~same
+    idx = np.cumsum(irr_mask).astype(np.int32)
+    irregular[irr_mask] = idx[irr_mask]
+    nirr = int(irr_mask.sum())
+    return irregular, nirr, xix_reg, jac_reg
+
+
+def face_geometry(anchors: np.ndarray, ispec: np.ndarray, iface: np.ndarray, xstore, ystore, zstore, ibool):
+    """Outward unit normals and weighted 2-D Jacobians of element faces.
+
+    Uses the element's GLL point coordinates (float32, as the reference reads them
+    from ``xstore``) and bilinear face geometry, per ``compute_jacobian_2D_face``.
+    Returns ijk (nf,25,3), normal (nf,25,3), jacobian2Dw (nf,25).
+    """
+    gll = get_gll()
+    x = gll.xigll
+    w = gll.wgll
+    nf = len(ispec)
+    ijk_all = np.empty((nf, 25, 3), np.int32)
+    nrm = np.empty((nf, 25, 3), np.float32)
+    jw = np.empty((nf, 25), np.float32)
+    A, B = np.meshgrid(x, x, indexing="xy")
+    WA, WB = np.meshgrid(w, w, indexing="xy")
+    A = A.ravel(); B = B.ravel(); WW = (WA * WB).ravel()
+    # bilinear 4-node face shape derivative tables
+    dNa = np.stack([-(1 - B), (1 - B), (1 + B), -(1 + B)], -1) * 0.25  # corners (-,-),(+,-),(+,+),(-,+)
+    dNb = np.stack([-(1 - A), -(1 + A), (1 + A), (1 - A)], -1) * 0.25
+    for fid in range(1, 7):
+        sel = np.nonzero(iface == fid)[0]
+        if len(sel) == 0:
+            continue
+        ijk = face_ijk(fid)
+        # face corners (a, b) = (0,0), (4,0), (4,4), (0,4); point index = a + 5 b
+        e = ispec[sel]
+        corners = []
+        for (ca, cb) in ((0, 0), (4, 0), (4, 4), (0, 4)):
+            i, j, k = ijk[ca + 5 * cb]
+            g = ibool[e, k, j, i]
+            corners.append(np.stack([xstore[g], ystore[g], zstore[g]], -1).astype(np.float64))
+        C = np.stack(corners, 1)  # (m, 4, 3)
+        ta = np.einsum("pa,mac->mpc", dNa, C)
+        tb = np.einsum("pa,mac->mpc", dNb, C)
+        un = np.cross(ta, tb)
+        jac = np.linalg.norm(un, axis=-1)
+        un = un / jac[..., None]
+        # orient outward: compare with vector from element centre to face centre
+        ctr = anchors[e].mean(1)
+        fc = C.mean(1)
+        s = np.sign(((fc - ctr)[:, None, :] * un).sum(-1))
+        s[s == 0] = 1.0
+        un = un * s[..., None]
+        ijk_all[sel] = ijk[None]
+        nrm[sel] = un.astype(np.float32)
+        jw[sel] = (jac * WW[None, :]).astype(np.float32)
+    return ijk_all, nrm, jw
