@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark of the single-partition solver loop on one MI355X.
+
+Workload (default): the single-shot GPU configuration -- 96 x 96 x 48 elements (NGLOB =
+28,607,425 GLL points), top 6 element layers acoustic coupled to a viscoelastic (Q_mu = 100,
+3 SLS) elastic half-space with a depth gradient x (1 + 5% Gaussian random field), one seeded
+moment-tensor source, 100 sea-floor receivers. One "step" = one Newmark time step driven
+through the reference's C-ABI call sequence (update_displacement, acoustic forces + Stacey +
+coupling, elastic forces + Stacey + coupling + source, kernel_3, seismograms).
+
+Metric: GLL-point updates/s = NGLOB x steps / wall time of the timed steps (inputs resident
+in HBM). Prints one JSON line with the roofline of the dominant kernel (viscoelastic element
+stiffness, HBM bound, timed with HIP events on its stream) and the CPU oracle timed on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stiffness_bytes_per_launch(db, attenuation: bool) -> float:
+    """Algorithmic HBM bytes of one element-stiffness launch (unique-traffic model).
+
+    Per global point of the elastic domain: displ read (12 B), accel read + write (24 B),
+    plus veloc read (12 B) with attenuation. Per local point of the elastic elements:
+    ibool + kappa + mu (12 B); with attenuation the 18 memory variables read + written
+    (144 B), the 6 stored strains read + written (48 B) and the 6 SLS factors read (24 B).
+    Regular elements carry no Jacobian arrays; irregular ones add 40 B per local point.
+    """
+    import numpy as np
+    el = db.ispec_is_elastic
+    nloc_el = int(el.sum()) * 125
+    nglob_el = int(np.unique(db.ibool[el]).size)
+    per_glob = 12 + 24 + (12 if attenuation else 0)
+    per_loc = 12 + (144 + 48 + 24 if attenuation else 0)
+    nirr_el = int((db.irregular_element_number[el] > 0).sum()) * 125
+    return float(per_glob * nglob_el + per_loc * nloc_el + 40 * nirr_el)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--scale", type=float, default=1.0, help="shrink the mesh (tests only)")
+    ap.add_argument("--exact", type=int, default=0, help="1: kernels without FMA (CPU rounding)")
+    ap.add_argument("--assembly", type=int, default=0, help="1: ordered (deterministic) assembly")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.solver import GPUSolver
+
+    t0 = time.time()
+    nstep_total = args.warmup + args.steps
+    w = W.config2(scale=args.scale, nstep=max(nstep_total, 1))
+    db = w.db
+    log(f"[bench] built {w.name}: {db.summary()} in {time.time() - t0:.1f}s")
+
+    g = GPUSolver(db, w.sources, w.receivers, nstep=nstep_total, attenuation=w.attenuation,
+                  exact=bool(args.exact), assembly=args.assembly)
+    log("[bench] prepared device arrays")
+    for _ in range(args.warmup):
+        g.step()
+    g.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done")
+    import ctypes as C
+    g.L.sf_timing_start(g.MP, C.c_int(args.steps))
+    g.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        g.step()
+    g.synchronize()
+    wall = time.perf_counter() - t
+    tot = C.c_double(0.0)
+    cnt = C.c_int(0)
+    g.L.sf_timing_read(g.MP, C.byref(tot), C.byref(cnt))
+    seis = g.seis_d[: g.seismo_current]
+    finite = bool(np.isfinite(seis).all())
+    g.close()
+    log(f"[bench] timed {args.steps} steps: {wall:.3f}s, stiffness kernel {tot.value / max(cnt.value, 1):.3f} ms avg "
+        f"over {cnt.value} launches, seismograms finite={finite}")
+
+    nglob = db.nglob
+    value = nglob * args.steps / wall
+    kbytes = stiffness_bytes_per_launch(db, w.attenuation is not None)
+    kavg_s = tot.value / max(cnt.value, 1) / 1e3
+    achieved = kbytes / kavg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_elastic (viscoelastic 3-SLS stiffness)", "bytes_per_launch": kbytes,
+                "avg_launch_ms": round(kavg_s * 1e3, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline and args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        ora = oracle.OracleSolver(db, w.sources, w.receivers, "f32", attenuation=w.attenuation)
+        tc = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            ora.step()
+        ct = time.perf_counter() - tc
+        cpu = {"value": nglob * args.cpu_steps / ct, "unit": "GLL-point updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} time steps of the same {w.name} mesh (FP32 C oracle, 1 thread)"}
+        log(f"[bench] CPU oracle {args.cpu_steps} steps in {ct:.1f}s")
+
+    out = {
+        "metric": "GLL-point updates/s/GPU (forward, single shot)",
+        "value": value,
+        "unit": "GLL-point updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded gradient + Gaussian random field model, random CMT source)",
+        "config": {"workload": w.name, "nglob": nglob, "nspec": db.nspec, "nspec_acoustic": db.nspec_acoustic,
+                   "attenuation": "3 SLS, Q_mu=100", "receivers": w.receivers.n, "exact": args.exact,
+                   "assembly": "ordered" if args.assembly else "atomic"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -139,6 +139,9 @@ class OracleSolver:
             self.eps = [None] * 6
         self.eps_tr3 = np.zeros(ne, R) if store_eps_trace3 else None
         if attenuation is not None:
+            # moduli scaled to the attenuation reference frequency (prepare_attenuation.f90)
+            self.kappa = np.ascontiguousarray(attenuation.kappastore.reshape(-1), dtype=R)
+            self.mu = np.ascontiguousarray(attenuation.mustore.reshape(-1), dtype=R)
             self.R = [np.zeros(ne * 3, R) for _ in range(6)]  # xx,yy,xy,xz,yz,trace
             self.alpha = attenuation.alphaval.astype(R)
             self.beta = attenuation.betaval.astype(R)

@@ -770,6 +770,10 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   dfree(m->asm_ac_ptr); dfree(m->asm_ac_idx); dfree(m->asm_ac_buf);
   dfree(m->d_norm);
   if (m->h_norm) (void)hipHostFree(m->h_norm);
+  if (m->timing_ev) {
+    for (int i = 0; i < 2 * m->timing_max; i++) (void)hipEventDestroy(m->timing_ev[i]);
+    delete[] m->timing_ev;
+  }
   delete[] m->h_src_elastic; delete[] m->h_rec_elastic; delete[] m->h_rec_ispec; delete[] m->h_adj_elastic;
   drop_host(m);
   SF_CHECK(hipStreamDestroy(m->stream));

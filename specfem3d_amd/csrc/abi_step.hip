@@ -74,7 +74,13 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
 void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
   ElasticArgs a = elastic_args(m, f, deltat);
   const bool ordered = m->assembly == 1 && m->asm_ready;
+  const bool timed = m->timing_count < m->timing_max;
+  if (timed) SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count], m->stream));
   launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered);
+  if (timed) {
+    SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count + 1], m->stream));
+    m->timing_count++;
+  }
   if (ordered)
     launch_ordered_gather(m->stream, m->nglob, 3, m->asm_ptr, m->asm_idx, m->asm_buf, a.nloc, m->el[f].accel);
 }
@@ -683,4 +689,32 @@ extern "C" void sf_synchronize(long* Mesh_pointer) {
   Mesh* m = from_handle(Mesh_pointer);
   flush_all(m);
   SF_CHECK(hipStreamSynchronize(m->stream));
+}
+
+// Time the next `max_launches` element-stiffness launches with HIP events on the mesh stream.
+extern "C" void sf_timing_start(long* Mesh_pointer, int max_launches) {
+  Mesh* m = from_handle(Mesh_pointer);
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  if (m->timing_ev) {
+    for (int i = 0; i < 2 * m->timing_max; i++) (void)hipEventDestroy(m->timing_ev[i]);
+    delete[] m->timing_ev;
+  }
+  m->timing_max = max_launches > 0 ? max_launches : 0;
+  m->timing_count = 0;
+  m->timing_ev = m->timing_max ? new hipEvent_t[2 * m->timing_max] : nullptr;
+  for (int i = 0; i < 2 * m->timing_max; i++) SF_CHECK(hipEventCreate(&m->timing_ev[i]));
+}
+
+// Total milliseconds and number of timed launches since sf_timing_start (synchronises).
+extern "C" void sf_timing_read(long* Mesh_pointer, double* total_ms, int* count) {
+  Mesh* m = from_handle(Mesh_pointer);
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  double t = 0.0;
+  for (int i = 0; i < m->timing_count; i++) {
+    float ms = 0.f;
+    SF_CHECK(hipEventElapsedTime(&ms, m->timing_ev[2 * i], m->timing_ev[2 * i + 1]));
+    t += ms;
+  }
+  *total_ms = t;
+  *count = m->timing_count;
 }

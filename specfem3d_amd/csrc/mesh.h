@@ -160,6 +160,11 @@ struct Mesh {
   // scratch for reductions
   realw* d_norm = nullptr;
   realw* h_norm = nullptr;         // pinned
+
+  // optional HIP-event timing of the element-stiffness launches (sf_timing_*)
+  int timing_max = 0;
+  int timing_count = 0;
+  hipEvent_t* timing_ev = nullptr; // 2 * timing_max events (start, stop)
 };
 
 Mesh* from_handle(long* p);

@@ -135,18 +135,31 @@ def build_box(nx: int, ny: int, nz: int,
     # materials at GLL points
     if material is None:
         material = homogeneous(2300.0, 2800.0, 1500.0)
+    # the models are continuous inside each domain, so they are evaluated once per global point
+    # of the domain and gathered to the elements' local points
     ib = ibool.reshape(nspec, 125)
-    X = xstore[ib]; Y = ystore[ib]; Z = zstore[ib]
     rho = np.empty((nspec, 125), np.float32)
     vp = np.empty((nspec, 125), np.float32)
     vs = np.empty((nspec, 125), np.float32)
+
+    def fill(mask, fn, fluid):
+        pts = np.unique(ib[mask])
+        r, p, s = fn(xstore[pts], ystore[pts], zstore[pts])
+        pos = np.searchsorted(pts, ib[mask])
+        rho[mask] = np.asarray(r, np.float32)[pos]
+        vp[mask] = np.asarray(p, np.float32)[pos]
+        vs[mask] = 0.0 if fluid else np.asarray(s, np.float32)[pos]
+
     if is_el.any():
-        r, p, s = material(X[is_el], Y[is_el], Z[is_el])
-        rho[is_el], vp[is_el], vs[is_el] = r, p, s
+        if is_el.all():
+            r, p, s = material(xstore, ystore, zstore)
+            rho[:] = np.asarray(r, np.float32)[ib]
+            vp[:] = np.asarray(p, np.float32)[ib]
+            vs[:] = np.asarray(s, np.float32)[ib]
+        else:
+            fill(is_el, material, False)
     if is_ac.any():
-        am = acoustic_material or homogeneous(1020.0, 1500.0, 0.0)
-        r, p, s = am(X[is_ac], Y[is_ac], Z[is_ac])
-        rho[is_ac], vp[is_ac], vs[is_ac] = r, p, 0.0
+        fill(is_ac, acoustic_material or homogeneous(1020.0, 1500.0, 0.0), True)
     f43 = np.float32(4.0 / 3.0)
     kappa = (rho * (vp * vp - f43 * vs * vs)).astype(np.float32)
     mu = (rho * vs * vs).astype(np.float32)

