@@ -1,0 +1,260 @@
+/*
+ * specfem3d_amd.h -- C-ABI of libspecfem3d_amd.so, the MI355X (gfx950) replacement of
+ * SPECFEM3D Cartesian's GPU interface for the single-partition (NPROC=1) solver and adjoint loops.
+ *
+ * The symbols are the Fortran-mangled names (lower case + trailing underscore) the reference's
+ * Fortran host already calls through implicit interfaces; every argument is passed by reference,
+ * Fortran logicals arrive as int, arrays are column-major with 1-based index values, `realw` is
+ * float. Each declaration names the reference interface it replaces (the authoritative signature
+ * list is src/gpu/specfem3D_gpu_cuda_method_stubs.c). The sf_* functions at the end have no
+ * counterpart in the reference.
+ */
+#ifndef SPECFEM3D_AMD_H
+#define SPECFEM3D_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef float realw;
+
+/* ---- device and set-up ------------------------------------------------------------------- */
+/* src/gpu/initialize_gpu.cu:535 */
+void initialize_gpu_device_(int* myrank_f, int* ncuda_devices);
+/* src/gpu/prepare_mesh_constants_cuda.cu:70 (stubs.c:481) */
+void prepare_constants_device_(long* Mesh_pointer, int* h_NGLLX, int* NSPEC_AB, int* NGLOB_AB, int* NSPEC_IRREGULAR,
+                               int* h_irregular_element_number, realw* h_xix, realw* h_xiy, realw* h_xiz,
+                               realw* h_etax, realw* h_etay, realw* h_etaz, realw* h_gammax, realw* h_gammay,
+                               realw* h_gammaz, realw* xix_regular, realw* jacobian_regular, int* h_ibool,
+                               int* num_interfaces_ext_mesh, int* max_nibool_interfaces_ext_mesh,
+                               int* h_nibool_interfaces_ext_mesh, int* h_ibool_interfaces_ext_mesh,
+                               realw* h_hprime_xx, realw* h_hprimewgll_xx, realw* h_wgllwgll_xy,
+                               realw* h_wgllwgll_xz, realw* h_wgllwgll_yz, int* STACEY_ABSORBING_CONDITIONS,
+                               int* h_abs_boundary_ispec, int* h_abs_boundary_ijk, realw* h_abs_boundary_normal,
+                               realw* h_abs_boundary_jacobian2Dw, int* h_num_abs_boundary_faces,
+                               int* h_ispec_is_inner, int* NSOURCES, int* nsources_local_f, realw* h_sourcearrays,
+                               int* h_islice_selected_source, int* h_ispec_selected_source,
+                               int* h_ispec_selected_rec, int* nrec, int* nrec_local, int* SIMULATION_TYPE,
+                               int* USE_MESH_COLORING_GPU_f, int* nspec_acoustic, int* nspec_elastic,
+                               int* ispec_is_acoustic, int* ispec_is_elastic, int* h_myrank, int* SAVE_FORWARD,
+                               realw* h_xir, realw* h_etar, realw* h_gammar, double* nu_rec, double* nu_source,
+                               int* h_islice_selected_rec, int* nlength_seismogram,
+                               int* SAVE_SEISMOGRAMS_DISPLACEMENT, int* SAVE_SEISMOGRAMS_VELOCITY,
+                               int* SAVE_SEISMOGRAMS_ACCELERATION, int* SAVE_SEISMOGRAMS_PRESSURE,
+                               int* h_NB_RUNS_ACOUSTIC_GPU, int* FAULT_SIMULATION,
+                               int* UNDO_ATTENUATION_AND_OR_PML, int* PML_CONDITIONS);
+/* src/gpu/prepare_mesh_constants_cuda.cu:657 (stubs.c:546) */
+void prepare_fields_elastic_device_(long* Mesh_pointer, realw* rmassx, realw* rmassy, realw* rmassz, realw* rho_vp,
+                                    realw* rho_vs, realw* h_kappav, realw* h_muv, int* num_phase_ispec_elastic,
+                                    int* phase_ispec_inner_elastic, realw* b_absorb_field, int* b_reclen_field,
+                                    int* COMPUTE_AND_STORE_STRAIN, realw* epsilondev_xx, realw* epsilondev_yy,
+                                    realw* epsilondev_xy, realw* epsilondev_xz, realw* epsilondev_yz,
+                                    int* ATTENUATION, int* R_size, realw* R_xx, realw* R_yy, realw* R_xy, realw* R_xz,
+                                    realw* R_yz, realw* factor_common, realw* R_trace, realw* epsilondev_trace,
+                                    realw* factor_common_kappa, realw* alphaval, realw* betaval, realw* gammaval,
+                                    int* APPROXIMATE_OCEAN_LOAD, realw* rmass_ocean_load, int* NOISE_TOMOGRAPHY,
+                                    realw* free_surface_normal, int* free_surface_ispec, int* free_surface_ijk,
+                                    int* num_free_surface_faces, int* ACOUSTIC_SIMULATION,
+                                    int* num_colors_outer_elastic, int* num_colors_inner_elastic,
+                                    int* num_elem_colors_elastic, int* ANISOTROPY, realw* c11store, realw* c12store,
+                                    realw* c13store, realw* c14store, realw* c15store, realw* c16store,
+                                    realw* c22store, realw* c23store, realw* c24store, realw* c25store,
+                                    realw* c26store, realw* c33store, realw* c34store, realw* c35store,
+                                    realw* c36store, realw* c44store, realw* c45store, realw* c46store,
+                                    realw* c55store, realw* c56store, realw* c66store);
+/* src/gpu/prepare_mesh_constants_cuda.cu:434 (stubs.c:523) */
+void prepare_fields_acoustic_device_(long* Mesh_pointer, realw* rmass_acoustic, realw* rhostore, realw* kappastore,
+                                     int* num_phase_ispec_acoustic, int* phase_ispec_inner_acoustic,
+                                     int* NOISE_TOMOGRAPHY, int* num_free_surface_faces, int* free_surface_ispec,
+                                     int* free_surface_ijk, int* b_reclen_potential, realw* b_absorb_potential,
+                                     int* ELASTIC_SIMULATION, int* num_coupling_ac_el_faces,
+                                     int* coupling_ac_el_ispec, int* coupling_ac_el_ijk,
+                                     realw* coupling_ac_el_normal, realw* coupling_ac_el_jacobian2Dw,
+                                     int* num_colors_outer_acoustic, int* num_colors_inner_acoustic,
+                                     int* num_elem_colors_acoustic);
+/* src/gpu/prepare_mesh_constants_cuda.cu:999 (stubs.c:584) */
+void prepare_fields_elastic_adj_dev_(long* Mesh_pointer, int* size_f, realw* epsilon_trace_over_3,
+                                     realw* b_epsilondev_xx, realw* b_epsilondev_yy, realw* b_epsilondev_xy,
+                                     realw* b_epsilondev_xz, realw* b_epsilondev_yz, realw* b_epsilon_trace_over_3,
+                                     int* R_size, realw* b_R_xx, realw* b_R_yy, realw* b_R_xy, realw* b_R_xz,
+                                     realw* b_R_yz, realw* b_R_trace, realw* b_epsilondev_trace, realw* b_alphaval,
+                                     realw* b_betaval, realw* b_gammaval, int* ANISOTROPIC_KL,
+                                     int* APPROXIMATE_HESS_KL);
+/* src/gpu/prepare_mesh_constants_cuda.cu:573 (stubs.c:542) */
+void prepare_fields_acoustic_adj_dev_(long* Mesh_pointer, int* APPROXIMATE_HESS_KL);
+/* src/gpu/prepare_mesh_constants_cuda.cu:1306 (stubs.c:611) */
+void prepare_sim2_or_3_const_device_(long* Mesh_pointer, int* nadj_rec_local, int* NTSTEP_BETWEEN_READ_ADJSRC,
+                                     realw* hxir_adjstore, realw* hetar_adjstore, realw* hgammar_adjstore,
+                                     int* nrec, int* h_islice_selected_rec, int* h_ispec_selected_rec);
+/* src/gpu/prepare_mesh_constants_cuda.cu:1552 (stubs.c:642) */
+void prepare_cleanup_device_(long* Mesh_pointer, int* ACOUSTIC_SIMULATION, int* ELASTIC_SIMULATION,
+                             int* NOISE_TOMOGRAPHY);
+
+/* ---- time marching ------------------------------------------------------------------------- */
+/* src/gpu/update_displacement_cuda.cu:39 (stubs.c:895) */
+void update_displacement_cuda_(long* Mesh_pointer, realw* deltat_F, realw* deltatover2_F, realw* deltatsqover2_F,
+                               int* FORWARD_OR_ADJOINT);
+/* src/gpu/update_displacement_cuda.cu:192 (stubs.c:902) */
+void update_displacement_ac_cuda_(long* Mesh_pointer, realw* deltat_F, realw* deltatover2_F, realw* deltatsqover2_F,
+                                  int* FORWARD_OR_ADJOINT);
+/* src/gpu/update_displacement_cuda.cu:287 (stubs.c:909) */
+void kernel_3_a_cuda_(long* Mesh_pointer, realw* deltatover2_F, realw* b_deltatover2_F, int* APPROXIMATE_OCEAN_LOAD,
+                      int* FORWARD_OR_ADJOINT);
+/* src/gpu/update_displacement_cuda.cu (stubs.c:916) -- ocean load only: stops with a message */
+void kernel_3_b_cuda_(long* Mesh_pointer, realw* deltatover2_F, realw* b_deltatover2_F, int* FORWARD_OR_ADJOINT);
+/* src/gpu/update_displacement_cuda.cu:448 (stubs.c:922) */
+void kernel_3_acoustic_cuda_(long* Mesh_pointer, realw* deltatover2_F, realw* b_deltatover2_F,
+                             int* FORWARD_OR_ADJOINT_f);
+
+/* ---- forces and boundary terms ---------------------------------------------------------- */
+/* src/gpu/compute_forces_viscoelastic_cuda.cu:1115 (stubs.c:243) */
+void compute_forces_viscoelastic_cuda_(long* Mesh_pointer, int* iphase, realw* deltat, int* nspec_outer_elastic,
+                                       int* nspec_inner_elastic, int* COMPUTE_AND_STORE_STRAIN, int* ATTENUATION,
+                                       int* FORWARD_OR_ADJOINT_f);
+/* src/gpu/compute_forces_acoustic_cuda.cu:238 (stubs.c:226) */
+void compute_forces_acoustic_cuda_(long* Mesh_pointer, int* iphase, int* nspec_outer_acoustic,
+                                   int* nspec_inner_acoustic, int* FORWARD_OR_ADJOINT_f);
+/* src/gpu/compute_forces_acoustic_cuda.cu:334 (stubs.c:233) */
+void acoustic_enforce_free_surf_cuda_(long* Mesh_pointer, int* ABSORB_INSTEAD_OF_FREE_SURFACE,
+                                      int* FORWARD_OR_ADJOINT);
+/* src/gpu/compute_stacey_viscoelastic_cuda.cu:39 (stubs.c:298) */
+void compute_stacey_viscoelastic_cuda_(long* Mesh_pointer, int* iphasef, realw* b_absorb_field,
+                                       int* FORWARD_OR_ADJOINT_f);
+/* src/gpu/compute_stacey_viscoelastic_cuda.cu:224 (stubs.c:304) */
+void compute_stacey_viscoelastic_undoatt_cuda_(long* Mesh_pointer, int* iphasef, int* FORWARD_OR_ADJOINT_f);
+/* src/gpu/compute_stacey_acoustic_cuda.cu (stubs.c:282) */
+void compute_stacey_acoustic_cuda_(long* Mesh_pointer, int* iphasef, realw* h_b_absorb_potential,
+                                   int* FORWARD_OR_ADJOINT_f);
+/* src/gpu/compute_stacey_acoustic_cuda.cu (stubs.c:288) */
+void compute_stacey_acoustic_undoatt_cuda_(long* Mesh_pointer, int* iphasef, int* FORWARD_OR_ADJOINT_f);
+/* src/gpu/compute_coupling_cuda.cu:39 (stubs.c:205) */
+void compute_coupling_ac_el_cuda_(long* Mesh_pointer, int* iphasef, int* num_coupling_ac_el_facesf,
+                                  int* FORWARD_OR_ADJOINT);
+/* src/gpu/compute_coupling_cuda.cu:135 (stubs.c:211) */
+void compute_coupling_el_ac_cuda_(long* Mesh_pointer, int* iphasef, int* num_coupling_ac_el_facesf,
+                                  int* FORWARD_OR_ADJOINT);
+
+/* ---- sources and receivers ----------------------------------------------------------------- */
+/* src/gpu/compute_add_sources_viscoelastic_cuda.cu:39 (stubs.c:176) */
+void compute_add_sources_el_cuda_(long* Mesh_pointer, double* h_stf_pre_compute, int* h_NSOURCES);
+/* src/gpu/compute_add_sources_viscoelastic_cuda.cu (stubs.c:181) */
+void compute_add_sources_el_s3_cuda_(long* Mesh_pointer, double* h_stf_pre_compute, int* h_NSOURCES);
+/* src/gpu/compute_add_sources_viscoelastic_cuda.cu:224 (stubs.c:192) */
+void add_sources_el_sim_type_2_or_3_(long* Mesh_pointer, realw* h_source_adjoint, int* nrec, int* nadj_rec_local,
+                                     int* NTSTEP_BETWEEN_READ_ADJSRC, int* it);
+/* src/gpu/compute_add_sources_acoustic_cuda.cu (stubs.c:153-170): sources/adjoint sources inside
+   fluid elements stop with a message */
+void compute_add_sources_ac_cuda_(long* Mesh_pointer, int* NSOURCESf, double* h_stf_pre_compute,
+                                  int* run_number_of_the_source);
+void compute_add_sources_ac_s3_cuda_(long* Mesh_pointer, int* NSOURCESf, double* h_stf_pre_compute,
+                                     int* run_number_of_the_source);
+void add_sources_ac_sim_2_or_3_cuda_(long* Mesh_pointer, realw* h_source_adjoint, int* nrec, int* nadj_rec_local,
+                                     int* NTSTEP_BETWEEN_READ_ADJSRC, int* it);
+/* src/gpu/write_seismograms_cuda.cu:34 (stubs.c:933) */
+void compute_seismograms_cuda_(long* Mesh_pointer_f, realw* seismograms_d, realw* seismograms_v,
+                               realw* seismograms_a, realw* seismograms_p, int* seismo_currentf,
+                               int* nlength_seismogramf, int* itf, int* it_endf, int* ACOUSTIC_SIMULATION,
+                               int* ELASTIC_SIMULATION, int* USE_TRICK_FOR_BETTER_PRESSURE);
+
+/* ---- sensitivity kernels, strain, stability norms ----------------------------------------- */
+/* src/gpu/compute_kernels_cuda.cu:39 (stubs.c:258) */
+void compute_kernels_elastic_cuda_(long* Mesh_pointer, realw* deltat_f);
+/* src/gpu/compute_kernels_cuda.cu (stubs.c:267): stops with a message (fluid kernels) */
+void compute_kernels_acoustic_cuda_(long* Mesh_pointer, realw* deltat_f);
+/* src/gpu/compute_strain_cuda.cu (stubs.c:314) */
+void compute_strain_cuda_(long* Mesh_pointer, int* FORWARD_OR_ADJOINT);
+/* src/gpu/check_fields_cuda.cu:375-470 (stubs.c:110-116) */
+void get_norm_elastic_from_device_(realw* norm, long* Mesh_pointer, int* type);
+void get_norm_acoustic_from_device_(realw* norm, long* Mesh_pointer, int* sim_type);
+/* src/gpu/check_fields_cuda.cu (stubs.c:104-108) */
+void get_free_device_memory_(realw* free, realw* used, realw* total);
+void output_free_device_memory_(int* myrank);
+
+/* ---- transfers (src/gpu/transfer_fields_cuda.cu:45-863, stubs.c:683-890) --------------- */
+void transfer_fields_el_to_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer);
+void transfer_fields_el_from_device_(int* size, realw* displ, realw* veloc, realw* accel, long* Mesh_pointer);
+void transfer_b_fields_to_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer);
+void transfer_b_fields_from_device_(int* size, realw* b_displ, realw* b_veloc, realw* b_accel, long* Mesh_pointer);
+void transfer_displ_to_device_(int* size, realw* displ, long* Mesh_pointer);
+void transfer_displ_from_device_(int* size, realw* displ, long* Mesh_pointer);
+void transfer_veloc_to_device_(int* size, realw* veloc, long* Mesh_pointer);
+void transfer_veloc_from_device_(int* size, realw* veloc, long* Mesh_pointer);
+void transfer_accel_to_device_(int* size, realw* accel, long* Mesh_pointer);
+void transfer_accel_from_device_(int* size, realw* accel, long* Mesh_pointer);
+void transfer_b_displ_to_device_(int* size, realw* b_displ, long* Mesh_pointer);
+void transfer_b_displ_from_device_(int* size, realw* b_displ, long* Mesh_pointer);
+void transfer_b_veloc_to_device_(int* size, realw* b_veloc, long* Mesh_pointer);
+void transfer_b_veloc_from_device_(int* size, realw* b_veloc, long* Mesh_pointer);
+void transfer_b_accel_to_device_(int* size, realw* b_accel, long* Mesh_pointer);
+void transfer_b_accel_from_device_(int* size, realw* b_accel, long* Mesh_pointer);
+void transfer_fields_ac_to_device_(int* size, realw* potential_acoustic, realw* potential_dot_acoustic,
+                                   realw* potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_fields_ac_from_device_(int* size, realw* potential_acoustic, realw* potential_dot_acoustic,
+                                     realw* potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_b_fields_ac_to_device_(int* size, realw* b_potential_acoustic, realw* b_potential_dot_acoustic,
+                                     realw* b_potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_b_fields_ac_from_device_(int* size, realw* b_potential_acoustic, realw* b_potential_dot_acoustic,
+                                       realw* b_potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_potential_ac_from_device_(int* size, realw* potential_acoustic, long* Mesh_pointer);
+void transfer_b_potential_ac_from_device_(int* size, realw* b_potential_acoustic, long* Mesh_pointer);
+void transfer_b_potential_ac_to_device_(int* size, realw* b_potential_acoustic, long* Mesh_pointer);
+void transfer_b_potential_dot_dot_ac_from_device_(int* size, realw* b_potential_dot_dot_acoustic,
+                                                  long* Mesh_pointer);
+void transfer_b_potential_dot_dot_ac_to_device_(int* size, realw* b_potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_dot_dot_to_device_(int* size, realw* potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_dot_dot_from_device_(int* size, realw* potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_b_dot_dot_from_device_(int* size, realw* b_potential_dot_dot_acoustic, long* Mesh_pointer);
+void transfer_kernels_el_to_host_(long* Mesh_pointer, realw* h_rho_kl, realw* h_mu_kl, realw* h_kappa_kl,
+                                  realw* h_cijkl_kl, int* NSPEC_AB);
+void transfer_kernels_ac_to_host_(long* Mesh_pointer, realw* h_rho_ac_kl, realw* h_kappa_ac_kl, int* NSPEC_AB);
+void transfer_strain_from_device_(long* Mesh_pointer, realw* epsilondev_xx, realw* epsilondev_yy,
+                                  realw* epsilondev_xy, realw* epsilondev_xz, realw* epsilondev_yz,
+                                  realw* epsilondev_trace, int* size_epsilondev);
+void transfer_b_strain_to_device_(long* Mesh_pointer, realw* b_epsilondev_xx, realw* b_epsilondev_yy,
+                                  realw* b_epsilondev_xy, realw* b_epsilondev_xz, realw* b_epsilondev_yz,
+                                  realw* b_epsilondev_trace, int* size_epsilondev);
+void transfer_rmemory_from_device_(long* Mesh_pointer, realw* R_xx, realw* R_yy, realw* R_xy, realw* R_xz,
+                                   realw* R_yz, realw* R_trace, int* size_R);
+void transfer_b_rmemory_to_device_(long* Mesh_pointer, realw* b_R_xx, realw* b_R_yy, realw* b_R_xy, realw* b_R_xz,
+                                   realw* b_R_yz, realw* b_R_trace, int* size_R);
+/* no-ops at NPROC=1 (src/gpu/assemble_MPI_*_cuda.cu, stubs.c:46-98) */
+void transfer_boun_pot_from_device_(long* Mesh_pointer, realw* send_potential_dot_dot_buffer,
+                                    const int* FORWARD_OR_ADJOINT);
+void transfer_asmbl_pot_to_device_(long* Mesh_pointer, realw* buffer_recv_scalar_ext_mesh,
+                                   const int* FORWARD_OR_ADJOINT);
+void transfer_boun_accel_from_device_(long* Mesh_pointer, realw* send_accel_buffer, const int* FORWARD_OR_ADJOINT);
+void transfer_boundary_from_device_a_(long* Mesh_pointer, const int* nspec_outer_elastic);
+void transfer_boundary_to_device_a_(long* Mesh_pointer, realw* buffer_recv_vector_ext_mesh,
+                                    const int* num_interfaces_ext_mesh, const int* max_nibool_interfaces_ext_mesh);
+void transfer_asmbl_accel_to_device_(long* Mesh_pointer, realw* buffer_recv_vector_ext_mesh,
+                                     const int* num_interfaces_ext_mesh, const int* max_nibool_interfaces_ext_mesh,
+                                     const int* nibool_interfaces_ext_mesh, const int* ibool_interfaces_ext_mesh,
+                                     const int* FORWARD_OR_ADJOINT);
+void transfer_sync_accel_to_device_(long* Mesh_pointer, realw* buffer_recv_vector_ext_mesh,
+                                    const int* num_interfaces_ext_mesh, const int* max_nibool_interfaces_ext_mesh,
+                                    const int* nibool_interfaces_ext_mesh, const int* ibool_interfaces_ext_mesh,
+                                    const int* FORWARD_OR_ADJOINT);
+void sync_copy_from_device_(long* Mesh_pointer, int* iphase, realw* send_buffer);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:884-890): nothing is pinned; the library stages copies */
+void register_host_array_(int* size, realw* h_array);
+void unregister_host_array_(realw* h_array);
+
+/* ---- library extensions (no counterpart in the reference interface) ---------------------- */
+/* exact = 1: element kernels without fused multiply-add (the CPU solver's rounding);
+   assembly = 1: element contributions summed per point in element order (deterministic). */
+void sf_set_options(long* Mesh_pointer, int exact, int assembly);
+/* jacobianstore (5,5,5,NSPEC_IRREGULAR) of the CPU solver for irregular elements (otherwise the
+   inverse of det(d xi/d x) is recomputed from the nine derivatives) */
+void sf_set_jacobian(long* Mesh_pointer, const realw* jacobianstore);
+/* run any deferred corrector and wait for the mesh stream */
+void sf_synchronize(long* Mesh_pointer);
+/* HIP-event timing of the element-stiffness launches */
+void sf_timing_start(long* Mesh_pointer, int max_launches);
+void sf_timing_read(long* Mesh_pointer, double* total_ms, int* count);
+/* message of the last fatal error */
+const char* sf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECFEM3D_AMD_H */

@@ -150,6 +150,9 @@ class OracleSolver:
             self.fck = np.ascontiguousarray(attenuation.factor_common_kappa.reshape(-1), R)
         self.it = 0
         self.adjoint_sign = 0   # 1 for the adjoint wavefield of a kernel run (coupling sign flip)
+        # tests plant a sign mistake in one coupling direction through these (never set otherwise)
+        self.plant_flip_ac_el = 0
+        self.plant_flip_el_ac = 0
         self.stf_table = None
 
     # -- pieces ---------------------------------------------------------------------
@@ -205,7 +208,6 @@ class OracleSolver:
         self.it += 1
         it = self.it
         self.predictor()
-        flip = C.c_int(self.adjoint_sign)
         if self.acoustic:
             nf, fis, fijk = self.free
             if nf:
@@ -218,7 +220,7 @@ class OracleSolver:
             if self.elastic and self.cpl[0]:
                 n, isp, ijk, nrm, jw = self.cpl
                 self._call("orc_coupling_ac_el", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
-                           _p(self.displ), _p(self.potdd), flip)
+                           _p(self.displ), _p(self.potdd), C.c_int(self.adjoint_sign ^ self.plant_flip_ac_el))
             self.forces_acoustic()
             self._call("orc_kernel3_acoustic", C.c_int64(self.db.nglob), _p(self.potd), _p(self.potdd),
                        _p(self.rmass_ac), self._r(self.deltatover2))
@@ -234,7 +236,7 @@ class OracleSolver:
             if self.acoustic and self.cpl[0]:
                 n, isp, ijk, nrm, jw = self.cpl
                 self._call("orc_coupling_el_ac", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
-                           _p(self.potdd), _p(self.accel), flip)
+                           _p(self.potdd), _p(self.accel), C.c_int(self.adjoint_sign ^ self.plant_flip_el_ac))
             if self.sources is not None and self.sources.n:
                 stf = np.ascontiguousarray(self.stf_values(it), self.dt_np)
                 self._call("orc_add_sources_el", C.c_int(self.sources.n), _p(self.src_ispec), _p(self.src_arr),

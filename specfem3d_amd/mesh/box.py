@@ -100,9 +100,16 @@ def build_box(nx: int, ny: int, nz: int,
     GZ, GY, GX = np.meshgrid(gz, gy, gx, indexing="ij")
     xs = GX.ravel(); ys = GY.ravel(); zs = GZ.ravel()
     if warp is not None:
-        xs, ys, zs = warp(xs, ys, zs)
+        # warped anchors; GLL points follow from the trilinear (HEX8) map of each element, which
+        # agrees on shared faces, so every global point gets one consistent position
+        from .database import gll_shape_tables
         ax, ay, az = warp(anchors[..., 0], anchors[..., 1], anchors[..., 2])
         anchors = np.stack([ax, ay, az], -1)
+        Nsh, _ = gll_shape_tables()
+        Xg = np.einsum("pa,eac->epc", Nsh, anchors)
+        pos = np.zeros((nglob, 3))
+        pos[ibool.reshape(nspec, 125).ravel()] = Xg.reshape(-1, 3)
+        xs, ys, zs = pos[:, 0], pos[:, 1], pos[:, 2]
     xstore = xs.astype(np.float32)
     ystore = ys.astype(np.float32)
     zstore = zs.astype(np.float32)

@@ -106,6 +106,9 @@ class GPUSolver:
             A.da(nu.ravel()), A.da(np.zeros(9 * max(nsrc, 1))), A.ia(np.zeros(max(nrec, 1), np.int32)),
             A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(0), A.i(1), A.i(0), A.i(0), A.i(0))
         L.sf_set_options(self.MP, C.c_int(int(exact)), C.c_int(int(assembly)))
+        if nirr:
+            # the CPU solver's jacobianstore (prepare_constants_device itself recomputes it)
+            L.sf_set_jacobian(self.MP, A.fa(db.jacobian.ravel()))
         nloc = db.nspec * 125
         if self.acoustic:
             n, ph = _fortran_phase_list(db.acoustic_elements())

@@ -72,9 +72,16 @@ class CMTSource:
     name: str = ""
 
 
-def read_cmtsolution(path: str, DT: float) -> list[CMTSource]:
-    """Parse a CMTSOLUTION file (one or more 13-line blocks)."""
-    lines = [l.rstrip("\n") for l in open(path)]
+def _lines(path_or_file):
+    if hasattr(path_or_file, "read"):
+        return path_or_file.read().splitlines()
+    with open(path_or_file) as f:
+        return f.read().splitlines()
+
+
+def read_cmtsolution(path, DT: float) -> list[CMTSource]:
+    """Parse a CMTSOLUTION file (path or file object; one or more 13-line blocks)."""
+    lines = _lines(path)
     blocks = []
     i = 0
     while i < len(lines):
@@ -120,9 +127,10 @@ class Station:
     burial: float
 
 
-def read_stations(path: str) -> list[Station]:
+def read_stations(path) -> list[Station]:
+    """Parse a STATIONS file (path or file object): ``STA NET lat(y) lon(x) elevation burial``."""
     st = []
-    for l in open(path):
+    for l in _lines(path):
         p = l.split()
         if len(p) < 6:
             continue
@@ -203,18 +211,20 @@ class Locator:
         s.discard(ispec)
         return sorted(s)
 
-    def locate(self, x, y, z) -> Located:
+    def locate(self, x, y, z, tie_break: str = "upper") -> Located:
         target = np.array([x, y, z], np.float64)
         d2 = ((self.mid - target) ** 2).sum(1)
         dmin = float(d2.min())
-        # Points on element corners/edges/faces are equidistant from several element midpoints.
-        # Among those ties the element with the highest midpoint, then the smallest x, then the
-        # smallest y is taken; for the 20^3 box with a source on an element corner this is the
-        # element the reference selects (its seismograms are reproduced to FP32 round-off).
+        # Points on element corners/edges/faces are equidistant from several element midpoints;
+        # the reference's pick among them depends on its element numbering (kd-tree order and,
+        # for NPROC > 1, the partition). tie_break="upper" takes the highest midpoint, then the
+        # smallest x, then the smallest y (what the serial internal-mesher run does for a source
+        # on a corner of the 20^3 box); "lower" takes the lowest midpoint first.
         tie = np.nonzero(d2 <= dmin * (1.0 + 1e-9) + 1e-12)[0]
         if len(tie) > 1:
             m = self.mid[tie]
-            order = np.lexsort((m[:, 1], m[:, 0], -m[:, 2]))
+            zkey = -m[:, 2] if tie_break == "upper" else m[:, 2]
+            order = np.lexsort((m[:, 1], m[:, 0], zkey))
             ispec = int(tie[order[0]])
         else:
             ispec = int(tie[0])
@@ -348,13 +358,14 @@ class SourceSet:
         raise ValueError(self.kind)
 
 
-def setup_cmt_sources(db: Database, cmts: list[CMTSource], DT: float, ricker: bool = False) -> SourceSet:
+def setup_cmt_sources(db: Database, cmts: list[CMTSource], DT: float, ricker: bool = False,
+                      tie_break: str = "upper") -> SourceSet:
     """Locate CMT sources and build source arrays, STF kind and t0."""
     loc = Locator(db)
     isp, arrs, located = [], [], []
     for s in cmts:
         ztop = surface_z(db, s.x, s.y)
-        L = loc.locate(s.x, s.y, ztop - s.depth_km * 1000.0)
+        L = loc.locate(s.x, s.y, ztop - s.depth_km * 1000.0, tie_break=tie_break)
         isp.append(L.ispec)
         arrs.append(cmt_source_array(db, L, s))
         located.append(L)

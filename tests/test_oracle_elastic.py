@@ -1,0 +1,147 @@
+"""Pins of the oracle's elastic path (compute_forces_viscoelastic.F90, Newmark, Stacey, sources,
+receivers) against mathematics and against the reference solver's own output.
+
+* an independent float64 assembly of the weak form (Kronecker-product derivative matrices,
+  numpy inverse Jacobians, sigma = lambda tr(eps) I + 2 mu eps) on irregular elements;
+* rigid translations produce no force, and the stiffness action is symmetric;
+* the Newmark scheme conserves the discrete energy of a closed box (no Stacey);
+* Stacey boundaries only remove energy;
+* config 1 (20^3 box, 2000 steps) reproduces the seismograms of the reference's FP32 and FP64
+  CPU builds (fixture tests/golden/cfg1_reference_seismograms.npz; first 400 steps here).
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_l2, small_elastic_box, smooth_warp
+from specfem3d_amd.gll import get_gll
+from specfem3d_amd.mesh.database import hex8_shape
+
+
+def oracle_force(oracle_mod, db, displ, prec="f64"):
+    """-(K u) through the oracle's element loop (accel starts at zero)."""
+    o = oracle_mod.OracleSolver(db, None, None, prec)
+    o.displ[:] = displ
+    o.accel[:] = 0
+    o.forces_elastic()
+    return o.accel.copy()
+
+
+def numpy_force(db, displ):
+    """Independent float64 weak-form stiffness action f = K u (not the reference's loop order)."""
+    g = get_gll()
+    D = g.D
+    I = np.eye(5)
+    Dr = [np.kron(I, np.kron(I, D)), np.kron(I, np.kron(D, I)), np.kron(D, np.kron(I, I))]
+    x = g.xigll
+    Gz, Gy, Gx = np.meshgrid(x, x, x, indexing="ij")
+    _, dN = hex8_shape(Gx.ravel(), Gy.ravel(), Gz.ravel())  # (125, 3, 8)
+    w = g.wgll
+    W = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel()
+    u = displ.reshape(-1, 3)
+    f = np.zeros_like(u)
+    for e in range(db.nspec):
+        ib = db.ibool[e].ravel()
+        U = u[ib]
+        J = np.einsum("pda,ac->pdc", dN, db.anchors[e])       # d x_c / d xi_d
+        G = np.linalg.inv(J).transpose(0, 2, 1)               # G[p, r, d] = d xi_r / d x_d
+        det = np.linalg.det(J)
+        dU = np.stack([Dr[r] @ U for r in range(3)], 1)        # (125, r, c)
+        gradU = np.einsum("prc,prd->pcd", dU, G)
+        eps = 0.5 * (gradU + gradU.transpose(0, 2, 1))
+        kap = db.kappastore[e].astype(np.float64)
+        mu = db.mustore[e].astype(np.float64)
+        lam = kap - 2.0 / 3.0 * mu
+        tr = np.trace(eps, axis1=1, axis2=2)
+        sig = lam[:, None, None] * tr[:, None, None] * np.eye(3) + 2.0 * mu[:, None, None] * eps
+        B = np.einsum("rpa,prd->pad", np.stack(Dr), G)          # d h_a / d x_d at p
+        fe = np.einsum("p,pcd,pad->ac", W * det, sig, B)
+        np.add.at(f, ib, fe)
+    return f.ravel()
+
+
+def test_stiffness_matches_independent_weak_form(oracle_mod):
+    db, _, _ = small_elastic_box(n=2, warp=smooth_warp(1000.0, 0.08), stacey=False)
+    assert db.nspec_irregular == db.nspec
+    rng = np.random.default_rng(1)
+    u = rng.normal(size=3 * db.nglob) * 1e-3
+    fo = oracle_force(oracle_mod, db, u)
+    fn = numpy_force(db, u)
+    # the oracle uses the float32 database (Jacobians, moduli): agreement to float32 level
+    assert rel_l2(-fo, fn) < 2e-6
+
+
+def test_stiffness_regular_matches_weak_form(oracle_mod):
+    db, _, _ = small_elastic_box(n=2, stacey=False)
+    assert db.nspec_irregular == 0
+    u = np.random.default_rng(2).normal(size=3 * db.nglob)
+    assert rel_l2(-oracle_force(oracle_mod, db, u), numpy_force(db, u)) < 2e-6
+
+
+def test_rigid_translation_no_force_and_symmetry(oracle_mod):
+    db, _, _ = small_elastic_box(n=3, warp=smooth_warp(1500.0), stacey=False)
+    n = db.nglob
+    t = np.tile([1.0, -2.0, 0.5], n)
+    f = oracle_force(oracle_mod, db, t)
+    scale = np.abs(oracle_force(oracle_mod, db, np.random.default_rng(0).normal(size=3 * n))).max()
+    # rows of the float32 hprime table sum to zero only to float32 rounding
+    assert np.abs(f).max() < 1e-6 * scale
+    rng = np.random.default_rng(3)
+    u, v = rng.normal(size=3 * n), rng.normal(size=3 * n)
+    a, b = v @ oracle_force(oracle_mod, db, u), u @ oracle_force(oracle_mod, db, v)
+    # hprimewgll = fl32(D w) is not exactly fl32(D) w: symmetry holds to float32 rounding; a
+    # transposed contraction would break it at O(1)
+    assert abs(a - b) < 1e-6 * abs(a)
+
+
+def _energy_run(oracle_mod, db, nstep, u0):
+    o = oracle_mod.OracleSolver(db, None, None, "f64")
+    o.displ[:] = u0
+    M = 1.0 / np.repeat(db.rmassx.astype(np.float64), 3)  # rmassx holds the inverse mass
+    M = 1.0 / np.stack([db.rmassx, db.rmassy, db.rmassz], 1).astype(np.float64).ravel()
+    E = []
+    u_prev = o.displ.copy()
+    for _ in range(nstep):
+        o.step()
+        vh = o.veloc - o.deltatover2 * o.accel   # v at the half step
+        E.append(0.5 * vh @ (M * vh) - 0.5 * u_prev @ (M * o.accel))
+        u_prev = o.displ.copy()
+    return np.array(E)
+
+
+def test_newmark_conserves_energy_closed_box(oracle_mod):
+    db, _, _ = small_elastic_box(n=3, warp=smooth_warp(1500.0), stacey=False, dt=0.002)
+    x, y, z = db.xstore.astype(np.float64), db.ystore.astype(np.float64), db.zstore.astype(np.float64)
+    u0 = np.stack([np.sin(x / 400.0), np.cos(y / 500.0) * 0.5, np.sin(z / 300.0 + x / 700.0)], 1).ravel() * 1e-3
+    E = _energy_run(oracle_mod, db, 300, u0)
+    # exact conservation for a symmetric K; the float32 tables make K symmetric to ~1e-7
+    assert np.abs(E - E[0]).max() < 1e-6 * abs(E[0])
+
+
+def test_stacey_only_removes_energy(oracle_mod):
+    db, _, _ = small_elastic_box(n=3, stacey=True, dt=0.002)
+    x, y, z = db.xstore.astype(np.float64), db.ystore.astype(np.float64), db.zstore.astype(np.float64)
+    r2 = (x - 750) ** 2 + (y - 750) ** 2 + (z + 750) ** 2
+    u0 = np.stack([np.exp(-r2 / 300.0 ** 2)] * 3, 1).ravel() * 1e-3
+    o = oracle_mod.OracleSolver(db, None, None, "f64")
+    o.displ[:] = u0
+    M = 1.0 / np.stack([db.rmassx, db.rmassy, db.rmassz], 1).astype(np.float64).ravel()
+    kin = []
+    for _ in range(400):
+        o.step()
+        kin.append(o.veloc @ (M * o.veloc))
+    # energy leaves through the absorbing faces: late kinetic energy well below the peak
+    assert max(kin[-50:]) < 0.2 * max(kin)
+
+
+def test_config1_matches_reference_fp32_and_fp64(oracle_mod):
+    """The first 400 steps of config 1 against the reference's own FP32 / FP64 CPU seismograms."""
+    from specfem3d_amd import workloads as W
+    ref = np.load(GOLDEN + "/cfg1_reference_seismograms.npz")
+    w = W.config1(nstep=400)
+    o = oracle_mod.OracleSolver(w.db, w.sources, w.receivers, "f32")
+    s = o.run(400)
+    e32 = rel_l2(s, ref["fp32"][:400])
+    e64 = rel_l2(s, ref["fp64"][:400])
+    assert e32 < 1e-5, e32
+    assert e64 < 1e-5, e64
+    np.testing.assert_allclose(ref["t"][:3], np.float32([-1.2, -1.19, -1.18]), atol=1e-6)

@@ -1,0 +1,116 @@
+"""Host-side set-up pinned by closed forms: box databases, source arrays, receivers, STFs, SLS."""
+import math
+
+import numpy as np
+
+from conftest import small_elastic_box, smooth_warp
+from specfem3d_amd import sources as S
+from specfem3d_amd.attenuation import memory_values, setup_attenuation, tau_sigma, _maxwell
+from specfem3d_amd.mesh.box import build_box, homogeneous
+from specfem3d_amd.workloads import nglob_box
+
+
+def test_box_database_geometry():
+    db = build_box(3, 2, 4, (0, 1500, 0, 1000, -2000, 0), material=homogeneous(2000, 3000, 1700))
+    db.finalize(0.005)
+    assert db.nglob == nglob_box(3, 2, 4)
+    assert db.nspec_irregular == 0
+    assert db.xix_regular == np.float32(2 / 500.0)
+    assert db.jacobian_regular == np.float32(250.0 ** 3)
+    # total mass: sum of the (uninverted) lumped mass = rho V
+    m = (1.0 / db.rmassx.astype(np.float64))
+    db2 = build_box(3, 2, 4, (0, 1500, 0, 1000, -2000, 0), material=homogeneous(2000, 3000, 1700))
+    db2.finalize(0.005, stacey=False)
+    np.testing.assert_allclose((1.0 / db2.rmassx.astype(np.float64)).sum(), 2000 * 1500 * 1000 * 2000, rtol=1e-6)
+    assert m.sum() > (1.0 / db2.rmassx.astype(np.float64)).sum()  # the implicit Stacey term adds mass
+    # faces: 5 absorbing sides, outward unit normals, weights summing to the face areas
+    f = db.abs_faces
+    assert f.n == 2 * 4 * 2 + 3 * 4 * 2 + 3 * 2
+    np.testing.assert_allclose(np.linalg.norm(f.normal, axis=-1), 1.0, atol=1e-6)
+    area = f.jacobian2Dw.astype(np.float64).sum()
+    np.testing.assert_allclose(area, 2 * 1000 * 2000 + 2 * 1500 * 2000 + 1500 * 1000, rtol=1e-6)
+    top = db.free_faces
+    np.testing.assert_allclose(top.normal[..., 2], 1.0)
+    np.testing.assert_allclose(top.jacobian2Dw.astype(np.float64).sum(), 1500 * 1000, rtol=1e-6)
+
+
+def test_warped_box_jacobians_consistent():
+    db = build_box(2, 2, 2, (0, 1000, 0, 1000, -1000, 0), warp=smooth_warp(1000.0, 0.08))
+    assert db.nspec_irregular == db.nspec
+    db.finalize(0.004, stacey=False)
+    # volume from the Jacobian determinants = volume of the warped box (= 1e9: the warp keeps faces)
+    from specfem3d_amd.gll import get_gll
+    w = get_gll().wgll
+    W = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel()
+    vol = (db.jacobian.astype(np.float64) * W).sum()
+    np.testing.assert_allclose(vol, 1e9, rtol=1e-6)
+
+
+def test_cmt_source_array_moments():
+    """sum_a F_a = 0 and sum_a F_a (x) x_a = M for a moment-tensor source (exact on a regular box)."""
+    db, src, _ = small_elastic_box(n=4)
+    L = src.located[0]
+    M = np.array([[1e13, 0.3e13, -0.5e13], [0.3e13, -2e13, 0.2e13], [-0.5e13, 0.2e13, 0.7e13]])
+    F = src.sourcearrays[0].astype(np.float64)  # (125, 3)
+    xyz = db.gll_coords(L.ispec)
+    np.testing.assert_allclose(F.sum(0), 0.0, atol=1e-6 * np.abs(F).max())
+    mom = np.einsum("ai,ak->ik", F, xyz)
+    np.testing.assert_allclose(mom, M, rtol=2e-6, atol=2e-6 * np.abs(M).max())
+
+
+def test_receiver_interpolation_exact_for_linear_fields():
+    db, _, rec = small_elastic_box(n=4)
+    x = db.xstore.astype(np.float64); y = db.ystore.astype(np.float64); z = db.zstore.astype(np.float64)
+    field = np.stack([2 * x + 3 * y - z, x - y, 0.5 * z + 7], 1)
+    for r, L in enumerate(rec.located):
+        ib = db.ibool[L.ispec].ravel()
+        h = (rec.hxir[r][None, None, :] * rec.hetar[r][None, :, None] * rec.hgammar[r][:, None, None]).ravel()
+        val = h @ field[ib]
+        exact = [2 * L.x + 3 * L.y - L.z, L.x - L.y, 0.5 * L.z + 7]
+        np.testing.assert_allclose(val, exact, rtol=1e-5, atol=1e-3)
+
+
+def test_locate_inverse_of_map():
+    db = build_box(2, 2, 2, (0, 1000, 0, 1000, -1000, 0), warp=smooth_warp(1000.0, 0.08))
+    loc = S.Locator(db)
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        p = rng.uniform([100, 100, -900], [900, 900, -100])
+        L = loc.locate(*p)
+        assert L.distance < 1e-6 and max(abs(L.xi), abs(L.eta), abs(L.gamma)) <= 1.0 + 1e-9
+
+
+def test_stf_closed_forms_and_t0():
+    f0 = 1.3
+    t = np.linspace(-2, 2, 101)
+    a = math.pi ** 2 * f0 ** 2
+    np.testing.assert_allclose(S.stf_ricker(t, f0), (1 - 2 * a * t ** 2) * np.exp(-a * t ** 2))
+    assert S.stf_heaviside(0.0, 0.7) == 0.5
+    assert S.stf_heaviside(10.0, 0.7) == 1.0
+    db, src, _ = small_elastic_box(n=2)
+    assert abs(src.t0 - 1.2 / 4.0) < 1e-15  # ricker: t0 = -1.2 min(tshift - 1/f0), f0 = 4 Hz
+    tab = src.stf_table(10)
+    np.testing.assert_allclose(tab[:, 0], S.stf_ricker(np.arange(10) * src.DT - src.t0, 4.0))
+
+
+def test_sls_memory_coefficients_taylor():
+    ts = np.float32([0.1, 0.5, 2.0])
+    a, b, g = memory_values(ts, np.float32(0.01))
+    np.testing.assert_allclose(a, np.exp(-0.01 / ts.astype(np.float64)), rtol=1e-6)
+    # beta + gamma integrate exp(-(dt - s)/tau) over the step (trapezoid weights of the RK2 scheme)
+    np.testing.assert_allclose(b + g, ts * (1 - np.exp(-0.01 / ts.astype(np.float64))), rtol=1e-5)
+
+
+def test_sls_fit_gives_constant_q():
+    db = build_box(2, 2, 2, (0, 1000, 0, 1000, -1000, 0), material=homogeneous(2500, 4000, 2300))
+    db.finalize(0.002)
+    att = setup_attenuation(db, qmu=80.0, qkappa=9000.0, min_resolved_period=0.2)
+    ts = tau_sigma(att.min_period, att.max_period)
+    f = np.linspace(np.log10(1 / att.max_period), np.log10(1 / att.min_period), 60)
+    fc = att.factor_common[0, 0].astype(np.float64)
+    from specfem3d_amd.attenuation import tau_eps_for_q
+    te = tau_eps_for_q(80.0, ts, att.min_period, att.max_period)
+    A, B = _maxwell(f, ts, te)
+    q = A / B
+    assert np.all(np.abs(q / 80.0 - 1.0) < 0.06)
+    np.testing.assert_allclose(fc, 2.0 * (te / ts - 1.0) / ts / np.sum(te / ts), rtol=1e-6)
