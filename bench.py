@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,6 +26,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic(assembly: int, exact: int):
+    """HBM bytes per stiffness launch from the committed rocprofv3 PMC summary of this mode.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE of the kernel, each scaled by what the same
+    counter reads for a plain 1 GiB device copy; returns (bytes, summary path) or (None, None).
+    """
+    if exact:
+        return None, None
+    name = os.path.join("profiles", f"k_elastic_config2_{'ordered' if assembly else 'atomic'}.md")
+    try:
+        txt = open(os.path.join(HERE, name)).read()
+    except OSError:
+        return None, None
+    m = re.search(r"traffic \(read \+ write\) \| ([0-9.]+) GB", txt)
+    return (float(m.group(1)) * 1e9, name) if m else (None, None)
 
 
 def log(*a):
@@ -60,7 +78,20 @@ def main():
     ap.add_argument("--assembly", type=int, default=0, help="1: ordered (deterministic) assembly")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        n = 256 * 1024 * 1024
+        x = torch.ones(n, device="cuda:0", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: 3 copies of {4 * n} bytes")
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path. "
@@ -106,10 +137,11 @@ def main():
     kbytes = stiffness_bytes_per_launch(db, w.attenuation is not None)
     kavg_s = tot.value / max(cnt.value, 1) / 1e3
     achieved = kbytes / kavg_s / 1e9
+    traffic, tsrc = measured_traffic(args.assembly, args.exact) if args.scale == 1.0 else (None, None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_elastic (viscoelastic 3-SLS stiffness)", "bytes_per_launch": kbytes,
-                "avg_launch_ms": round(kavg_s * 1e3, 4)}
+                "avg_launch_ms": round(kavg_s * 1e3, 4), "traffic_source": tsrc}
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:
