@@ -36,7 +36,7 @@ def measured_traffic(assembly: int, exact: int):
     """
     if exact:
         return None, None
-    name = os.path.join("profiles", f"k_elastic_config2_{'ordered' if assembly else 'atomic'}.md")
+    name = os.path.join("profiles", f"k_elastic_config2_{['atomic', 'ordered', 'coloured'][assembly]}.md")
     try:
         txt = open(os.path.join(HERE, name)).read()
     except OSError:
@@ -75,7 +75,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the mesh (tests only)")
     ap.add_argument("--exact", type=int, default=0, help="1: kernels without FMA (CPU rounding)")
-    ap.add_argument("--assembly", type=int, default=0, help="1: ordered (deterministic) assembly")
+    ap.add_argument("--assembly", type=int, default=0,
+                    help="0 atomic scatter, 1 ordered (CPU element order), 2 coloured (no atomics)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -171,7 +172,7 @@ def main():
         "data": "synthetic (seeded gradient + Gaussian random field model, random CMT source)",
         "config": {"workload": w.name, "nglob": nglob, "nspec": db.nspec, "nspec_acoustic": db.nspec_acoustic,
                    "attenuation": "3 SLS, Q_mu=100", "receivers": w.receivers.n, "exact": args.exact,
-                   "assembly": "ordered" if args.assembly else "atomic"},
+                   "assembly": ["atomic", "ordered", "coloured"][args.assembly]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -241,8 +241,28 @@ void unregister_host_array_(realw* h_array);
 
 /* ---- library extensions (no counterpart in the reference interface) ---------------------- */
 /* exact = 1: element kernels without fused multiply-add (the CPU solver's rounding);
-   assembly = 1: element contributions summed per point in element order (deterministic). */
+   assembly = 0: float atomic scatter-add; 1: element contributions summed per point in element
+   order (deterministic, the CPU solver's order); 2: one launch per colour of a greedy element
+   colouring with a plain read-modify-write scatter (deterministic, no atomics). */
 void sf_set_options(long* Mesh_pointer, int exact, int assembly);
+/* tuning switch by name: "elastic_shape" 0 = column-per-lane stiffness kernel (five elements per
+   workgroup), 1 = point-per-lane (one element per workgroup, the default); unknown names are fatal */
+void sf_set_tuning(long* Mesh_pointer, const char* name, int value);
+/* device-resident checkpoints for checkpointed kernel runs (the files of
+   save_forward_arrays_undoatt / read_forward_arrays_undoatt, save_forward_arrays.f90:102-179 and
+   read_forward_arrays.f90:133-229, and the b_displ buffer of iterate_time_undoatt.F90:45-47, kept in
+   HBM): FORWARD_OR_ADJOINT 1 = forward/adjoint field, 3 = backward/reconstructed field */
+void sf_snapshot_save(long* Mesh_pointer, int FORWARD_OR_ADJOINT, int slot);
+void sf_snapshot_load(long* Mesh_pointer, int FORWARD_OR_ADJOINT, int slot);
+void sf_history_reserve(long* Mesh_pointer, int nframes);
+void sf_history_store(long* Mesh_pointer, int FORWARD_OR_ADJOINT, int frame);
+void sf_history_load(long* Mesh_pointer, int FORWARD_OR_ADJOINT, int frame);
+void sf_checkpoints_free(long* Mesh_pointer);
+/* zero the sensitivity-kernel accumulators (prepare_fields_*_adj_dev zero them once) */
+void sf_zero_kernels(long* Mesh_pointer);
+/* greedy first-fit element colouring of assembly = 2 (host only): color[q] for elems[q], elements
+   sharing a global point differ; returns the number of colours or -1 (more than 32) */
+int sf_color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* color);
 /* jacobianstore (5,5,5,NSPEC_IRREGULAR) of the CPU solver for irregular elements (otherwise the
    inverse of det(d xi/d x) is recomputed from the nine derivatives) */
 void sf_set_jacobian(long* Mesh_pointer, const realw* jacobianstore);

@@ -296,3 +296,102 @@ def element_strain(db, displ, precision="f32"):
        _p(np.ascontiguousarray(db.jac9.reshape(-1), R)), _p(np.ascontiguousarray(g.hprime, R)),
        _p(np.ascontiguousarray(displ, R)), *[_p(o) for o in out])
     return out  # xx, yy, xy, xz, yz, trace/3
+
+
+def _snapshot(s: OracleSolver) -> dict:
+    """State saved per time subset (save_forward_arrays_undoatt, save_forward_arrays.f90:102-179)."""
+    snap = {"it": s.it, "displ": s.displ.copy(), "veloc": s.veloc.copy(), "accel": s.accel.copy()}
+    if s.att is not None:
+        snap["R"] = [r.copy() for r in s.R]
+    return snap
+
+
+def _restore(s: OracleSolver, snap: dict):
+    """read_forward_arrays_undoatt (read_forward_arrays.f90:133-229) followed by the strain
+    recomputation from the displacement (compute_strain_att_backward,
+    compute_element_strain.F90:228-254)."""
+    s.it = snap["it"]
+    s.displ[:] = snap["displ"]
+    s.veloc[:] = snap["veloc"]
+    s.accel[:] = snap["accel"]
+    if s.att is not None:
+        for r, v in zip(s.R, snap["R"]):
+            r[:] = v
+    _recompute_strain(s)
+
+
+def _recompute_strain(s: OracleSolver):
+    if not s.compute_strain:
+        return
+    e = element_strain(s.db, s.displ, s.prec)
+    for c in range(5):
+        s.eps[c][:] = e[c]
+    if s.att is not None:
+        s.eps[5][:] = s.dt_np(3) * e[5]
+    if s.eps_tr3 is not None:
+        s.eps_tr3[:] = e[5]
+
+
+def kernel_run(db, sources, receivers, nstep: int, adj: np.ndarray, nt_dump: int, precision: str = "f64",
+               attenuation=None):
+    """Checkpointed sensitivity-kernel run at NPROC=1 (elastic domains).
+
+    Follows iterate_time_undoatt.F90:29-632 for SIMULATION_TYPE=1 with SAVE_FORWARD (snapshot at the
+    start of every subset of ``nt_dump`` steps) and SIMULATION_TYPE=3 (subsets in reverse: restore a
+    snapshot, recompute the subset storing b_displ per step, then step the adjoint field through the
+    subset with b_displ read back time-reversed, :420-560). The adjoint source of step ``it`` is
+    ``adj[NSTEP - it]`` (forward-time index NSTEP-it+1 of the .adj record, added while it < NSTEP;
+    compute_add_sources_viscoelastic.F90:181-262 and compute_arrays_source.f90:203-291). Kernels
+    accumulate after every adjoint step (compute_kernels.f90:67-222) with the reconstructed strain
+    recomputed from b_displ.
+
+    Returns (forward seismograms (nstep, nrec, 3), rho_kl, mu_kl, kappa_kl) -- the raw time
+    integrals before save_adjoint_kernels' scaling.
+    """
+    nt_dump = min(nt_dump, nstep)
+    fwd = OracleSolver(db, sources, receivers, precision, attenuation=attenuation)
+    fwd.stf_table = sources.stf_table(nstep)
+    nsub = -(-nstep // nt_dump)
+    snaps = []
+    seis = np.zeros((nstep, receivers.n, 3), fwd.dt_np)
+    for s in range(nsub):
+        snaps.append(_snapshot(fwd))
+        n_s = nt_dump if s < nsub - 1 else nstep - (nsub - 1) * nt_dump
+        for _ in range(n_s):
+            fwd.step()
+            seis[fwd.it - 1] = fwd.seismogram()
+    del fwd
+
+    R = np.float32 if precision == "f32" else np.float64
+    b = OracleSolver(db, sources, receivers, precision, attenuation=attenuation)
+    b.stf_table = sources.stf_table(nstep)
+    adjs = OracleSolver(db, None, receivers, precision, attenuation=attenuation, compute_strain=True,
+                        store_eps_trace3=True)
+    adjs.adjoint_sign = 1
+    ne = db.nspec * 125
+    rho_kl, mu_kl, kappa_kl = np.zeros(ne, R), np.zeros(ne, R), np.zeros(ne, R)
+    L = lib()
+    fn = getattr(L, "orc_compute_kernels_el" + b.sfx)
+    elems = db.elastic_elements()
+    ib = np.ascontiguousarray(db.ibool.reshape(-1), np.int32)
+    adj = np.ascontiguousarray(adj, R)
+    it = 0
+    for i in range(1, nsub + 1):
+        slot = nsub - i
+        _restore(b, snaps[slot])
+        n_i = nstep - (nsub - 1) * nt_dump if i == 1 else nt_dump
+        buf = []
+        for _ in range(n_i):
+            b.step()
+            buf.append(b.displ.copy())
+        _recompute_strain(adjs)          # compute_strain_att before the adjoint subset
+        for m in range(n_i):
+            it += 1
+            bd = buf[n_i - 1 - m]
+            adjs.step(adjoint_source=adj[nstep - it] if it < nstep else None)
+            be = element_strain(db, bd, precision)
+            e = adjs.eps
+            fn(C.c_int(len(elems)), _p(elems), _p(ib), _p(adjs.accel), _p(bd), _p(e[0]), _p(e[1]), _p(e[2]),
+               _p(e[3]), _p(e[4]), _p(adjs.eps_tr3), _p(be[0]), _p(be[1]), _p(be[2]), _p(be[3]), _p(be[4]),
+               _p(be[5]), adjs._r(adjs.deltat), _p(rho_kl), _p(mu_kl), _p(kappa_kl))
+    return seis, rho_kl, mu_kl, kappa_kl

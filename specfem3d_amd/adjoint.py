@@ -1,0 +1,242 @@
+"""Checkpointed adjoint (sensitivity-kernel) runs on the GPU, and the host-side pieces around them.
+
+The reference computes kernels with SIMULATION_TYPE=3 either by reconstructing the forward field
+backward in time from its last frame plus stored boundary records (iterate_time.F90:233-290), or,
+with UNDO_ATTENUATION_AND_OR_PML, by forward checkpointing and recomputation
+(iterate_time_undoatt.F90:29-632). ``KernelSolver`` follows the second scheme through the C-ABI:
+
+* ``forward()``: the SIMULATION_TYPE=1 + SAVE_FORWARD pass. The forward field is stepped from rest
+  and its state is saved at the start of every subset of ``nt_dump`` steps
+  (save_forward_arrays_undoatt) -- here into HBM (``sf_snapshot_save``) instead of files;
+* ``adjoint(adj)``: subsets in reverse; each restores a snapshot (``sf_snapshot_load`` +
+  ``compute_strain_cuda``), recomputes the subset keeping every step's displacement in an HBM
+  history (``sf_history_store``), then steps the adjoint field through the subset with the
+  reconstructed displacement read back time-reversed (``sf_history_load``) and accumulates the
+  kernels (``compute_kernels_elastic_cuda``) after each adjoint step.
+
+``finalize_kernels`` applies save_adjoint_kernels.f90:276-412 (rho, mu, kappa, rhop, alpha, beta);
+``write_kernel_bin`` writes them as Fortran unformatted records. The torch.autograd wrapper used by
+the gradient tests lives in ``specfem3d_amd.autograd``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .gll import get_gll
+from .solver import GPUSolver
+
+
+# ------------------------------------------------------------------------------ host-side pieces
+def waveform_misfit(syn: np.ndarray, obs: np.ndarray, dt: float) -> float:
+    """L2 waveform misfit chi = 1/2 sum_r sum_t |syn - obs|^2 dt (FP64 accumulation)."""
+    r = syn.astype(np.float64) - obs.astype(np.float64)
+    return 0.5 * float((r * r).sum()) * float(dt)
+
+
+def waveform_adjoint_source(syn: np.ndarray, obs: np.ndarray) -> np.ndarray:
+    """Adjoint source of the L2 waveform misfit in forward time, (nstep, nrec, 3): syn - obs.
+
+    This is what the ``SEM/NET.STA.BX?.adj`` files hold; the solver reads it time-reversed
+    (compute_arrays_source.f90:203-291)."""
+    return (syn.astype(np.float64) - obs.astype(np.float64)).astype(np.float32)
+
+
+def quadrature_weights(db) -> np.ndarray:
+    """w_i w_j w_k J at every local point, (nspec, 5, 5, 5) [e, k, j, i]: the volume weights that turn
+    a kernel into a directional derivative, d chi = sum K * w J * dln m."""
+    g = get_gll()
+    w = np.asarray(g.wgll, np.float64)
+    w3 = w[None, None, :] * w[None, :, None] * w[:, None, None]
+    out = np.empty((db.nspec, 5, 5, 5), np.float64)
+    irr = db.irregular_element_number
+    reg = irr == 0
+    out[reg] = w3[None] * float(db.jacobian_regular)
+    if (~reg).any():
+        jac = np.asarray(db.jacobian, np.float64).reshape(-1, 5, 5, 5)
+        out[~reg] = w3[None] * jac[irr[~reg] - 1]
+    return out
+
+
+def finalize_kernels(db, rho_kl, mu_kl, kappa_kl, mustore=None, kappastore=None) -> dict:
+    """save_adjoint_kernels.f90:276-412 (isotropic, elastic elements): rho = -rho K_rho,
+    mu = -2 mu K_mu, kappa = -kappa K_kappa, rhop = rho + kappa + mu, beta and alpha kernels.
+    rho is recovered as rho_vs^2 / mu as the reference does."""
+    shp = (db.nspec, 5, 5, 5)
+    mu = np.asarray(db.mustore if mustore is None else mustore, np.float32).reshape(shp)
+    kap = np.asarray(db.kappastore if kappastore is None else kappastore, np.float32).reshape(shp)
+    rvs = np.asarray(db.rho_vs, np.float32).reshape(shp)
+    rk, mk, kk = (np.asarray(a, np.float32).reshape(shp) for a in (rho_kl, mu_kl, kappa_kl))
+    el = db.ispec_is_elastic
+    out = {k: np.zeros(shp, np.float32) for k in ("rho", "mu", "kappa", "rhop", "alpha", "beta")}
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rhol = rvs * rvs / mu
+        rho = -rhol * rk
+        mu_k = np.float32(-2.0) * mu * mk
+        kap_k = -kap * kk
+        out["rho"][el] = rho[el]
+        out["mu"][el] = mu_k[el]
+        out["kappa"][el] = kap_k[el]
+        out["rhop"][el] = (rho + kap_k + mu_k)[el]
+        out["beta"][el] = (np.float32(2.0) * (mu_k - np.float32(4.0 / 3.0) * mu / kap * kap_k))[el]
+        out["alpha"][el] = (np.float32(2.0) * (np.float32(1.0) + np.float32(4.0 / 3.0) * mu / kap) * kap_k)[el]
+    return out
+
+
+def write_kernel_bin(path: str, arr: np.ndarray) -> None:
+    """One Fortran unformatted sequential record (4-byte length markers), as save_adjoint_kernels
+    writes ``proc000000_<name>_kernel.bin`` (the array in (i,j,k,ispec) order)."""
+    a = np.ascontiguousarray(arr, np.float32).reshape(-1)
+    n = np.array([a.nbytes], np.int32)
+    with open(path, "wb") as f:
+        f.write(n.tobytes())
+        f.write(a.tobytes())
+        f.write(n.tobytes())
+
+
+def read_kernel_bin(path: str) -> np.ndarray:
+    raw = np.fromfile(path, np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    if n + 8 != raw.size or int(raw[-4:].view(np.int32)[0]) != n:
+        raise ValueError(f"{path}: not a single Fortran unformatted record")
+    return raw[4:4 + n].view(np.float32).copy()
+
+
+# ------------------------------------------------------------------------------ GPU driver
+class KernelSolver(GPUSolver):
+    """Checkpointed sensitivity-kernel run on the GPU (SIMULATION_TYPE=3, undo-attenuation loop).
+
+    Elastic (and viscoelastic) domains; the forward/adjoint field is ``FORWARD_OR_ADJOINT`` 1 and
+    the reconstructed forward field 3, as in the reference."""
+
+    def __init__(self, db, sources, receivers, nstep: int, nt_dump: int = 64, attenuation=None,
+                 exact: bool = False, assembly: int = 0, device: int = 0, tuning: dict | None = None):
+        if db.nspec_acoustic:
+            raise NotImplementedError("checkpointed kernel runs cover elastic domains only")
+        if receivers is None or receivers.n == 0:
+            raise ValueError("a kernel run needs receivers (the adjoint sources)")
+        super().__init__(db, sources, receivers, nstep=nstep, attenuation=attenuation, exact=exact,
+                         assembly=assembly, simulation_type=3, nlength=nstep, device=device, tuning=tuning,
+                         undo_attenuation=True)
+        self.nt_dump = max(1, min(int(nt_dump), self.nstep))
+        self.nsub = math.ceil(self.nstep / self.nt_dump)
+        self.forward_done = False
+
+    def _subset_len(self, slot: int) -> int:
+        return self.nt_dump if slot < self.nsub - 1 else self.nstep - (self.nsub - 1) * self.nt_dump
+
+    def _b_step(self, fwd_it: int):
+        """One step of the reconstructed field (compute_forces_viscoelastic_backward_calling with
+        b_deltat = +DT, prepare_timerun.F90:418-422), forward-time step index ``fwd_it``."""
+        L, MP = self.L, self.MP
+        three, zero = self._ref(3), self._ref(0)
+        dt, dt2, dtsq2 = (C.byref(self._c[k]) for k in ("dt", "dt2", "dtsq2"))
+        L.update_displacement_cuda_(MP, dt, dt2, dtsq2, three)
+        strain = C.c_int(1)
+        att = C.c_int(int(self.att is not None))
+        for iphase in (1, 2):
+            ip = C.c_int(iphase)
+            L.compute_forces_viscoelastic_cuda_(MP, C.byref(ip), dt, zero, zero, C.byref(strain), C.byref(att), three)
+            if iphase == 1:
+                L.compute_stacey_viscoelastic_undoatt_cuda_(MP, C.byref(ip), three)
+                if self.stf_table is not None:
+                    stf = np.ascontiguousarray(self.stf_table[fwd_it - 1], np.float64)
+                    L.compute_add_sources_el_s3_cuda_(MP, stf.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(stf))))
+        L.kernel_3_a_cuda_(MP, dt2, dt2, zero, three)
+
+    def _adj_step(self, it: int, adj: np.ndarray):
+        """One step of the adjoint field (compute_forces_viscoelastic_calling at SIMULATION_TYPE=3)."""
+        L, MP = self.L, self.MP
+        one, zero = self._ref(1), self._ref(0)
+        dt, dt2, dtsq2 = (C.byref(self._c[k]) for k in ("dt", "dt2", "dtsq2"))
+        L.update_displacement_cuda_(MP, dt, dt2, dtsq2, one)
+        strain = C.c_int(1)
+        att = C.c_int(int(self.att is not None))
+        nrec = C.c_int(self.receivers.n)
+        ntr = C.c_int(self.nstep)
+        for iphase in (1, 2):
+            ip = C.c_int(iphase)
+            L.compute_forces_viscoelastic_cuda_(MP, C.byref(ip), dt, zero, zero, C.byref(strain), C.byref(att), one)
+            if iphase == 1:
+                L.compute_stacey_viscoelastic_undoatt_cuda_(MP, C.byref(ip), one)
+                if it < self.nstep:
+                    L.add_sources_el_sim_type_2_or_3_(MP, abi.ptr(adj), C.byref(nrec), C.byref(nrec), C.byref(ntr),
+                                                      C.byref(C.c_int(it)))
+        L.kernel_3_a_cuda_(MP, dt2, dt2, zero, one)
+
+    def forward(self) -> np.ndarray:
+        """Forward pass with a snapshot per subset; returns the forward seismograms (nstep, nrec, 3)."""
+        L, MP = self.L, self.MP
+        self._zero_field(3)
+        self.seismo_current = 0
+        it = 0
+        for slot in range(self.nsub):
+            L.sf_snapshot_save(MP, C.c_int(3), C.c_int(slot))
+            for _ in range(self._subset_len(slot)):
+                it += 1
+                self._b_step(it)
+                self._record(it)
+        self.forward_done = True
+        return self.seis_d[: self.seismo_current].copy()
+
+    def adjoint(self, adj: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Adjoint pass driven by ``adj`` (nstep, nrec, 3) in forward time; returns the raw kernel
+        time integrals (rho_kl, mu_kl, kappa_kl) over (nspec, 5, 5, 5)."""
+        if not self.forward_done:
+            raise RuntimeError("KernelSolver.adjoint() needs forward() first (no snapshots saved)")
+        adj = np.ascontiguousarray(adj, np.float32)
+        if adj.shape != (self.nstep, self.receivers.n, 3):
+            raise ValueError(f"adjoint source must be (nstep, nrec, 3) = {(self.nstep, self.receivers.n, 3)}")
+        L, MP = self.L, self.MP
+        self._keep_adj = adj
+        self._zero_field(1)
+        self._zero_kernels()
+        L.sf_history_reserve(MP, C.c_int(self.nt_dump))
+        one, three = self._ref(1), self._ref(3)
+        it = 0
+        for i in range(1, self.nsub + 1):
+            slot = self.nsub - i
+            n_i = self._subset_len(slot)
+            L.sf_snapshot_load(MP, C.c_int(3), C.c_int(slot))
+            L.compute_strain_cuda_(MP, three)
+            base = slot * self.nt_dump
+            for m in range(n_i):
+                self._b_step(base + m + 1)
+                L.sf_history_store(MP, C.c_int(3), C.c_int(m))
+            L.compute_strain_cuda_(MP, one)
+            dtk = C.c_float(self.deltat)
+            for m in range(n_i):
+                it += 1
+                L.sf_history_load(MP, C.c_int(3), C.c_int(n_i - 1 - m))
+                self._adj_step(it, adj)
+                L.compute_kernels_elastic_cuda_(MP, C.byref(dtk))
+        L.sf_history_reserve(MP, C.c_int(0))
+        nloc = self.db.nspec * 125
+        rk, mk, kk = (np.zeros(nloc, np.float32) for _ in range(3))
+        L.transfer_kernels_el_to_host_(MP, abi.ptr(rk), abi.ptr(mk), abi.ptr(kk), None, C.byref(C.c_int(self.db.nspec)))
+        shp = (self.db.nspec, 5, 5, 5)
+        return rk.reshape(shp), mk.reshape(shp), kk.reshape(shp)
+
+    # -- helpers
+    def _record(self, it: int):
+        zero = self._ref(0)
+        self.seismo_current += 1
+        self.L.compute_seismograms_cuda_(
+            self.MP, abi.ptr(self.seis_d), abi.ptr(self.seis_v), abi.ptr(self.seis_a), abi.ptr(self.seis_p),
+            C.byref(C.c_int(self.seismo_current)), C.byref(C.c_int(self.nlength)), C.byref(C.c_int(it)),
+            C.byref(C.c_int(self.nstep)), C.byref(C.c_int(0)), C.byref(C.c_int(1)), zero)
+
+    def _zero_field(self, foa: int):
+        n3 = 3 * self.db.nglob
+        z = np.zeros(n3, np.float32)
+        A = abi.Args()
+        if foa == 3:
+            self.L.transfer_b_fields_to_device_(A.i(n3), abi.ptr(z), abi.ptr(z), abi.ptr(z), self.MP)
+        else:
+            self.L.transfer_fields_el_to_device_(A.i(n3), abi.ptr(z), abi.ptr(z), abi.ptr(z), self.MP)
+
+    def _zero_kernels(self):
+        self.L.sf_zero_kernels(self.MP)

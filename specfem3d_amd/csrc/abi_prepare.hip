@@ -13,6 +13,7 @@
 #include "kernels.h"
 
 namespace sf {
+void free_checkpoints(Mesh* m);
 
 Mesh* from_handle(long* p) {
   if (!p || !*p) fatal("Mesh_pointer", "null handle (prepare_constants_device not called?)");
@@ -283,6 +284,7 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
   H->nspec = m->nspec;
   if (const char* e = std::getenv("SPECFEM_AMD_EXACT")) m->exact = std::atoi(e) != 0;
   if (const char* e = std::getenv("SPECFEM_AMD_ASSEMBLY")) m->assembly = std::atoi(e);
+  if (const char* e = std::getenv("SPECFEM_AMD_ELASTIC_SHAPE")) m->elastic_shape = std::atoi(e) != 0;
 
   // GLL constants: hp[i*5+l] = hprime_xx(i+1,l+1) (column-major offset i + 5 l)
   realw hp[25], hpw[25], wxy[25];
@@ -412,13 +414,14 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
     if (m->save_a) m->seismo_a = dmalloc<realw>(sz);
     if (m->save_p) m->seismo_p = dmalloc<realw>(sz);
   }
-  (void)UNDO_ATTENUATION_AND_OR_PML;
+  m->undo_attenuation = *UNDO_ATTENUATION_AND_OR_PML;
   m->d_norm = dmalloc<realw>(4);
   SF_CHECK(hipHostMalloc((void**)&m->h_norm, sizeof(realw) * 4, hipHostMallocDefault));
 }
 
 namespace sf {
 static void build_elastic_assembly(Mesh* m);
+static void build_colors(Mesh* m, const std::vector<int>& elems, int*& d_list, int* off, int& ncolor);
 }  // namespace sf
 
 // prepare_mesh_constants_cuda.cu:657
@@ -531,9 +534,49 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
     }
   }
   if (m->assembly == 1) build_elastic_assembly(m);
+  if (m->assembly == 2) {
+    std::vector<int> lst(m->nspec_elastic);
+    SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+    build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
+  }
 }
 
 namespace sf {
+// Greedy first-fit colouring of an element list (in list order): an element takes the lowest
+// colour not yet used by any element sharing one of its global points. Each colour is then one
+// launch whose element-to-global scatter needs no atomics. Elements keep their list order
+// inside a colour. At most 32 colours (a bit mask per global point); a structured hexahedral
+// mesh needs 8.
+static int color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* col) {
+  std::vector<uint32_t> used((size_t)nglob, 0u);
+  int ncolor = 0;
+  for (int q = 0; q < nelem; q++) {
+    const int* ib = ibool0 + (int64_t)elems[q] * 125;
+    uint32_t u = 0;
+    for (int p = 0; p < 125; p++) u |= used[ib[p]];
+    if (u == 0xffffffffu) return -1;
+    const int c = __builtin_ctz(~u);
+    col[q] = c;
+    for (int p = 0; p < 125; p++) used[ib[p]] |= 1u << c;
+    if (c + 1 > ncolor) ncolor = c + 1;
+  }
+  return ncolor;
+}
+
+static void build_colors(Mesh* m, const std::vector<int>& elems, int*& d_list, int* off, int& ncolor) {
+  HostSide* H = host_of(m);
+  std::vector<int> col(elems.size());
+  ncolor = color_elements(m->nglob, (int)elems.size(), H->ibool.data(), elems.data(), col.data());
+  if (ncolor < 0) fatal("build_colors", "element colouring needs more than 32 colours");
+  for (int c = 0; c <= 32; c++) off[c] = 0;
+  for (size_t q = 0; q < elems.size(); q++) off[col[q] + 1]++;
+  for (int c = 0; c < 32; c++) off[c + 1] += off[c];
+  std::vector<int> pos(off, off + 32), lst(elems.size());
+  for (size_t q = 0; q < elems.size(); q++) lst[pos[col[q]]++] = elems[q];
+  if (d_list) dfree(d_list);
+  d_list = lst.empty() ? nullptr : dupload(lst.data(), (int64_t)lst.size());
+}
+
 // CSR of local slots per global point sorted by the position of their element in the
 // processing list (ordered assembly); shared by the elastic and acoustic gathers.
 static void build_csr(Mesh* m, const std::vector<int>& elems, int*& ptr, int*& idx) {
@@ -595,6 +638,7 @@ extern "C" void FC_FUNC_(prepare_fields_acoustic_device, PREPARE_FIELDS_ACOUSTIC
       build_csr(m, lst, m->asm_ac_ptr, m->asm_ac_idx);
       m->asm_ac_buf = dmalloc<realw>(nloc);
     }
+    if (m->assembly == 2) build_colors(m, lst, m->ac_color_elems, m->ac_color_off, m->ac_ncolor);
   }
   m->ac[0].pot = dmalloc<realw>(n);
   m->ac[0].potd = dmalloc<realw>(n);
@@ -748,7 +792,9 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   (void)ACOUSTIC_SIMULATION; (void)ELASTIC_SIMULATION; (void)NOISE_TOMOGRAPHY;
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
+  free_checkpoints(m);
   dfree(m->ibool); dfree(m->irregular); dfree(m->jac); dfree(m->elastic_elems); dfree(m->acoustic_elems);
+  dfree(m->el_color_elems); dfree(m->ac_color_elems);
   dfree(m->kappamu); dfree(m->rho); dfree(m->kappa_ac); dfree(m->rmass); dfree(m->rmass_ac);
   for (int f = 0; f < 2; f++) {
     dfree(m->el[f].displ); dfree(m->el[f].veloc); dfree(m->el[f].accel);
@@ -786,7 +832,8 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
 
 // exact = 1: element kernels without fused multiply-add (the CPU reference's rounding);
 // assembly = 1: element contributions summed per global point in element order (deterministic,
-// the reference's serial order) instead of float atomics.
+// the reference's serial order) instead of float atomics; assembly = 2: one launch per colour of
+// a greedy element colouring, plain read-modify-write scatter (deterministic, no atomics).
 extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
@@ -801,6 +848,38 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
       m->asm_ac_buf = dmalloc<realw>((int64_t)m->nspec * 125);
     }
   }
+  if (assembly == 2) {
+    if (!m->el_color_elems && m->nspec_elastic > 0 && m->elastic_elems) {
+      std::vector<int> lst(m->nspec_elastic);
+      SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+      build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
+    }
+    if (!m->ac_color_elems && m->nspec_acoustic > 0 && m->acoustic_elems) {
+      std::vector<int> lst(m->nspec_acoustic);
+      SF_CHECK(hipMemcpy(lst.data(), m->acoustic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+      build_colors(m, lst, m->ac_color_elems, m->ac_color_off, m->ac_ncolor);
+    }
+  }
+}
+
+// Tuning switches read by the step calls: "elastic_shape" (0 column-per-lane stiffness kernel,
+// 1 point-per-lane, the default). Unknown names are a fatal error.
+extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
+  Mesh* m = from_handle(Mesh_pointer);
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  if (std::strcmp(name, "elastic_shape") == 0) {
+    if (value < 0 || value > 1) fatal("sf_set_tuning", "elastic_shape must be 0 or 1");
+    m->elastic_shape = value;
+  } else {
+    fatal("sf_set_tuning", "unknown tuning switch");
+  }
+}
+
+// Greedy element colouring used by assembly = 2 (host only, no device needed): colour[q] of
+// elems[q] (0-based element ids into ibool0, 0-based, 125 per element); returns the number of
+// colours, or -1 if more than 32 would be needed.
+extern "C" int sf_color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* color) {
+  return color_elements(nglob, nelem, ibool0, elems, color);
 }
 
 // Replace the recomputed inverse determinant of irregular elements by the host's jacobianstore

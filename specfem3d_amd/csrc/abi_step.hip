@@ -74,9 +74,18 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
 void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
   ElasticArgs a = elastic_args(m, f, deltat);
   const bool ordered = m->assembly == 1 && m->asm_ready;
+  const bool colored = m->assembly == 2 && m->el_color_elems;
   const bool timed = m->timing_count < m->timing_max;
   if (timed) SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count], m->stream));
-  launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered);
+  if (colored) {
+    for (int c = 0; c < m->el_ncolor; c++) {
+      a.elems = m->el_color_elems + m->el_color_off[c];
+      a.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
+      launch_elastic(m->stream, a, strain, att, m->exact != 0, 2, m->elastic_shape);
+    }
+  } else {
+    launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->elastic_shape);
+  }
   if (timed) {
     SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count + 1], m->stream));
     m->timing_count++;
@@ -220,7 +229,15 @@ extern "C" void FC_FUNC_(compute_forces_acoustic_cuda, COMPUTE_FORCES_ACOUSTIC_C
     a.potdd = m->ac[f].potdd;
     a.local_out = m->asm_ac_buf;
     const bool ordered = m->assembly == 1 && m->asm_ac_buf;
-    launch_acoustic(m->stream, a, m->exact != 0, ordered);
+    if (m->assembly == 2 && m->ac_color_elems) {
+      for (int c = 0; c < m->ac_ncolor; c++) {
+        a.elems = m->ac_color_elems + m->ac_color_off[c];
+        a.nelem = m->ac_color_off[c + 1] - m->ac_color_off[c];
+        launch_acoustic(m->stream, a, m->exact != 0, 2);
+      }
+    } else {
+      launch_acoustic(m->stream, a, m->exact != 0, ordered ? 1 : 0);
+    }
     if (ordered)
       launch_ordered_gather(m->stream, m->nglob, 1, m->asm_ac_ptr, m->asm_ac_idx, m->asm_ac_buf,
                             (int64_t)m->nspec * 125, m->ac[f].potdd);
@@ -477,6 +494,11 @@ extern "C" void FC_FUNC_(compute_kernels_elastic_cuda, COMPUTE_KERNELS_ELASTIC_C
   flush_el(m, 0);
   flush_el(m, 1);
   const int64_t nl = (int64_t)m->nspec * 125;
+  if (m->undo_attenuation) {
+    // the reconstructed strain is recomputed from b_displ (compute_kernels_cuda.cu:57-100)
+    ElasticArgs a = elastic_args(m, 1, 0.f);
+    launch_element_strain(m->stream, a, m->att[1].eps, m->eps_trace_over_3[1], m->attenuation != 0, m->exact != 0);
+  }
   launch_compute_kernels_el(m->stream, m->nspec_elastic, m->elastic_elems, m->ibool, m->el[0].accel, m->el[1].displ,
                             m->att[0].eps, m->eps_trace_over_3[0], m->att[1].eps, m->eps_trace_over_3[1], nl,
                             *deltat_f, m->rho_kl, m->mu_kl, m->kappa_kl);
@@ -490,7 +512,7 @@ extern "C" void FC_FUNC_(compute_strain_cuda, COMPUTE_strain_CUDA)(long* Mesh_po
   if (!m->att[f].eps || !m->eps_trace_over_3[f]) fatal("compute_strain_cuda", "strain arrays not prepared");
   flush_el(m, f);
   ElasticArgs a = elastic_args(m, f, 0.f);
-  launch_element_strain(m->stream, a, m->att[f].eps, m->eps_trace_over_3[f], m->exact != 0);
+  launch_element_strain(m->stream, a, m->att[f].eps, m->eps_trace_over_3[f], m->attenuation != 0, m->exact != 0);
 }
 
 extern "C" void FC_FUNC_(compute_kernels_acoustic_cuda, COMPUTE_KERNELS_ACOUSTIC_CUDA)(long* Mesh_pointer,
@@ -683,7 +705,7 @@ extern "C" void FC_FUNC_(transfer_kernels_ac_to_host, TRANSFER_KERNELS_AC_TO_HOS
 
 // ---------------------------------------------------------------- library extensions (no reference counterpart)
 // sf_set_options: exact = 1 runs the element kernels without fused multiply-add (the CPU
-// reference's rounding); assembly = 1 sums element contributions in element order.
+// reference's rounding); assembly = 1 sums element contributions in element order, 2 colours.
 extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly);
 extern "C" void sf_synchronize(long* Mesh_pointer) {
   Mesh* m = from_handle(Mesh_pointer);

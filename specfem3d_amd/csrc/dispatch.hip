@@ -15,32 +15,33 @@ void set_gll_constants(const realw* hprime, const realw* hprimewgll, const realw
 }
 
 // variants compiled from k_elastic.hip / k_acoustic.hip with and without FMA contraction
-void launch_elastic_exact(hipStream_t, const ElasticArgs&, bool, bool, bool);
-void launch_elastic_fast(hipStream_t, const ElasticArgs&, bool, bool, bool);
-void launch_element_strain_exact(hipStream_t, const ElasticArgs&, realw*, realw*);
-void launch_element_strain_fast(hipStream_t, const ElasticArgs&, realw*, realw*);
-void launch_acoustic_exact(hipStream_t, const AcousticArgs&, bool);
-void launch_acoustic_fast(hipStream_t, const AcousticArgs&, bool);
+void launch_elastic_exact(hipStream_t, const ElasticArgs&, bool, bool, int, int);
+void launch_elastic_fast(hipStream_t, const ElasticArgs&, bool, bool, int, int);
+void launch_element_strain_exact(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
+void launch_element_strain_fast(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
+void launch_acoustic_exact(hipStream_t, const AcousticArgs&, int);
+void launch_acoustic_fast(hipStream_t, const AcousticArgs&, int);
 
-void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, bool ordered) {
+void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode, int shape) {
   if (exact)
-    launch_elastic_exact(s, a, strain, att, ordered);
+    launch_elastic_exact(s, a, strain, att, mode, shape);
   else
-    launch_elastic_fast(s, a, strain, att, ordered);
+    launch_elastic_fast(s, a, strain, att, mode, shape);
 }
 
-void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool exact) {
+void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,
+                           bool exact) {
   if (exact)
-    launch_element_strain_exact(s, a, eps, eps_tr3);
+    launch_element_strain_exact(s, a, eps, eps_tr3, with_trace);
   else
-    launch_element_strain_fast(s, a, eps, eps_tr3);
+    launch_element_strain_fast(s, a, eps, eps_tr3, with_trace);
 }
 
-void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, bool ordered) {
+void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, int mode) {
   if (exact)
-    launch_acoustic_exact(s, a, ordered);
+    launch_acoustic_exact(s, a, mode);
   else
-    launch_acoustic_fast(s, a, ordered);
+    launch_acoustic_fast(s, a, mode);
 }
 
 static std::string g_last_error;

@@ -20,7 +20,7 @@
 namespace sf {
 namespace SF_VNS {
 
-template <bool ORDERED>
+template <int MODE>
 __global__ void __launch_bounds__(128) k_acoustic(AcousticArgs A, GLLk G) {
   __shared__ realw s_buf[5][2][128];
   const int tid = threadIdx.x;
@@ -107,8 +107,10 @@ __global__ void __launch_bounds__(128) k_acoustic(AcousticArgs A, GLLk G) {
       n3 = n3 + G.hpw[l * 5 + k] * t3[l];
     }
     const realw f = G.wxy[j * 5 + k] * n1 + G.wxy[i * 5 + k] * n2 + G.wxy[i * 5 + j] * n3;
-    if (ORDERED) {
+    if (MODE == 1) {
       A.local_out[(int64_t)e * 125 + c + 25 * k] = f;
+    } else if (MODE == 2) {
+      A.potdd[g[k]] = A.potdd[g[k]] + (-f);
     } else {
       atomicAdd(A.potdd + g[k], -f);
     }
@@ -118,13 +120,15 @@ __global__ void __launch_bounds__(128) k_acoustic(AcousticArgs A, GLLk G) {
 }  // namespace SF_VNS
 using namespace SF_VNS;
 
-void SFX(launch_acoustic)(hipStream_t s, const AcousticArgs& a, bool ordered) {
+void SFX(launch_acoustic)(hipStream_t s, const AcousticArgs& a, int mode) {
   if (a.nelem <= 0) return;
   dim3 grid((a.nelem + 4) / 5), block(128);
-  if (ordered)
-    hipLaunchKernelGGL(k_acoustic<true>, grid, block, 0, s, a, g_gll_host);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_acoustic<1>, grid, block, 0, s, a, g_gll_host);
+  else if (mode == 2)
+    hipLaunchKernelGGL(k_acoustic<2>, grid, block, 0, s, a, g_gll_host);
   else
-    hipLaunchKernelGGL(k_acoustic<false>, grid, block, 0, s, a, g_gll_host);
+    hipLaunchKernelGGL(k_acoustic<0>, grid, block, 0, s, a, g_gll_host);
   SF_LAUNCH_CHECK("k_acoustic");
 }
 

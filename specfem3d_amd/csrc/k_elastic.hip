@@ -8,9 +8,10 @@
 //   * the gamma-direction contractions run in registers; the xi/eta ones read the element
 //     tile staged once in LDS (6 planes x 128 floats per element: u and dt*v, later the
 //     stress terms t1/t2);
-//   * the element-to-global scatter is a float atomic add (global_atomic_add_f32) or, in
-//     ordered-assembly mode, a store of the local contributions that a gather kernel later
-//     sums in the CPU's element order;
+//   * the element-to-global scatter is selected by MODE: 0 a float atomic add, 1 a store of
+//     the local contributions that a gather kernel later sums in the CPU's element order,
+//     2 a plain read-modify-write (the launch holds one colour of a greedy element colouring,
+//     so no two of its elements share a global point);
 //   * workgroup ids are remapped so each XCD (8 per chip, dispatched round-robin) walks one
 //     contiguous range of elements and keeps the shared points of neighbours in its own L2.
 // The arithmetic order of every sum follows the reference (the five terms of each
@@ -40,7 +41,7 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-template <bool STRAIN, bool ATT, bool ORDERED>
+template <bool STRAIN, bool ATT, int MODE>
 __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
   __shared__ realw s_buf[5][6][128];
   const int tid = threadIdx.x;
@@ -293,11 +294,16 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
       const realw fac1 = G.wxy[j * 5 + k], fac2 = G.wxy[i * 5 + k], fac3 = G.wxy[i * 5 + j];
       f[cc] = fac1 * n1 + fac2 * n2 + fac3 * n3;
     }
-    if (ORDERED) {
+    if (MODE == 1) {
       const int64_t ep = (int64_t)e * 125 + c + 25 * k;
       A.local_out[ep] = f[0];
       A.local_out[A.nloc + ep] = f[1];
       A.local_out[2 * A.nloc + ep] = f[2];
+    } else if (MODE == 2) {
+      realw* a = A.accel + 3 * (int64_t)g[k];
+      a[0] = a[0] + (-f[0]);
+      a[1] = a[1] + (-f[1]);
+      a[2] = a[2] + (-f[2]);
     } else {
       realw* a = A.accel + 3 * (int64_t)g[k];
       atomicAdd(a + 0, -f[0]);
@@ -307,8 +313,253 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
   }
 }
 
+// Point-per-lane variant: a 128-thread workgroup (two wave64s) processes one element and lane p
+// holds the GLL point p = i + 5 j + 25 k (lanes 125..127 shadow point 124 and store nothing).
+// All three contractions read the element tile from LDS, so no lane carries a column of state
+// between phases; the per-point state (memory variables, stored strain, SLS factors, moduli,
+// Jacobian terms) is loaded before the first barrier and is in flight while the tile is
+// gathered. Same arithmetic, in the same order, as k_elastic.
+template <bool STRAIN, bool ATT, int MODE>
+__global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
+  __shared__ realw sU[6][128];  // u (0..2) and dt*v (3..5)
+  __shared__ realw sT[9][128];  // stress products t1 (0..2), t2 (3..5), t3 (6..8)
+  const int p = threadIdx.x;
+  const bool active = p < 125;
+  const int ie = xcd_remap(blockIdx.x, gridDim.x);  // grid = one workgroup per listed element
+  const int e = A.elems[ie];
+  const int q = active ? p : 124;
+  const int k = q / 25, j = (q / 5) % 5, i = q % 5;
+  const int64_t ep = (int64_t)e * 125 + q;
+  const int64_t nl = A.nloc;
+  const int g = A.ibool[ep];
+  {
+    const realw* d = A.displ + 3 * (int64_t)g;
+    const realw u0 = d[0], u1 = d[1], u2 = d[2];
+    sU[0][p] = u0; sU[1][p] = u1; sU[2][p] = u2;
+    if (ATT) {
+      const realw* v = A.veloc + 3 * (int64_t)g;
+      sU[3][p] = A.deltat * v[0]; sU[4][p] = A.deltat * v[1]; sU[5][p] = A.deltat * v[2];
+    }
+  }
+  const int irr = A.irregular[e];
+  realw xixl = 0, xiyl = 0, xizl = 0, etaxl = 0, etayl = 0, etazl = 0, gaxl = 0, gayl = 0, gazl = 0, jacl;
+  if (irr) {
+    const int64_t qq = (int64_t)(irr - 1) * 125 + q;
+    const realw* J = A.jac;
+    const int64_t P = A.jac_plane;
+    xixl = J[qq]; xiyl = J[P + qq]; xizl = J[2 * P + qq];
+    etaxl = J[3 * P + qq]; etayl = J[4 * P + qq]; etazl = J[5 * P + qq];
+    gaxl = J[6 * P + qq]; gayl = J[7 * P + qq]; gazl = J[8 * P + qq];
+    jacl = J[9 * P + qq];
+  } else {
+    jacl = A.jacobian_regular;
+  }
+  const float2 km = A.kappamu[ep];
+  realw Rv[6][3], Sn[6], fcv[3], fckv[3];
+  if (ATT) {
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+      for (int s = 0; s < 3; s++) Rv[cc][s] = A.R[(cc * 3 + s) * nl + ep];
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++) Sn[cc] = A.eps[cc * nl + ep];
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = A.fc[s * nl + ep];
+      fckv[s] = A.fck[s * nl + ep];
+    }
+  }
+  __syncthreads();
+
+  const realw ONE_THIRD = (realw)1 / (realw)3;
+  const realw FOUR_THIRDS = (realw)4 / (realw)3;
+  const int b1 = 5 * j + 25 * k;  // f(l, j, k) = s[b1 + l]
+  const int b2 = i + 25 * k;      // f(i, l, k) = s[b2 + 5 l]
+  const int b3 = i + 5 * j;       // f(i, j, l) = s[b3 + 25 l]
+  realw tx1, ty1, tz1, tx2, ty2, tz2, tx3, ty3, tz3;
+  {
+    const realw *sx = sU[0], *sy = sU[1], *sz = sU[2];
+    const realw h1 = G.hp[i * 5], h2 = G.hp[j * 5], h3 = G.hp[k * 5];
+    tx1 = h1 * sx[b1]; ty1 = h1 * sy[b1]; tz1 = h1 * sz[b1];
+    tx2 = h2 * sx[b2]; ty2 = h2 * sy[b2]; tz2 = h2 * sz[b2];
+    tx3 = h3 * sx[b3]; ty3 = h3 * sy[b3]; tz3 = h3 * sz[b3];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      const realw a1 = G.hp[i * 5 + l], a2 = G.hp[j * 5 + l], a3 = G.hp[k * 5 + l];
+      tx1 = tx1 + a1 * sx[b1 + l]; ty1 = ty1 + a1 * sy[b1 + l]; tz1 = tz1 + a1 * sz[b1 + l];
+      tx2 = tx2 + a2 * sx[b2 + 5 * l]; ty2 = ty2 + a2 * sy[b2 + 5 * l]; tz2 = tz2 + a2 * sz[b2 + 5 * l];
+      tx3 = tx3 + a3 * sx[b3 + 25 * l]; ty3 = ty3 + a3 * sy[b3 + 25 * l]; tz3 = tz3 + a3 * sz[b3 + 25 * l];
+    }
+  }
+  realw duxdx, duxdy, duxdz, duydx, duydy, duydz, duzdx, duzdy, duzdz;
+  if (irr) {
+    duxdx = xixl * tx1 + etaxl * tx2 + gaxl * tx3;
+    duxdy = xiyl * tx1 + etayl * tx2 + gayl * tx3;
+    duxdz = xizl * tx1 + etazl * tx2 + gazl * tx3;
+    duydx = xixl * ty1 + etaxl * ty2 + gaxl * ty3;
+    duydy = xiyl * ty1 + etayl * ty2 + gayl * ty3;
+    duydz = xizl * ty1 + etazl * ty2 + gazl * ty3;
+    duzdx = xixl * tz1 + etaxl * tz2 + gaxl * tz3;
+    duzdy = xiyl * tz1 + etayl * tz2 + gayl * tz3;
+    duzdz = xizl * tz1 + etazl * tz2 + gazl * tz3;
+  } else {
+    const realw xr = A.xix_regular;
+    duxdx = xr * tx1; duxdy = xr * tx2; duxdz = xr * tx3;
+    duydx = xr * ty1; duydy = xr * ty2; duydz = xr * ty3;
+    duzdx = xr * tz1; duzdy = xr * tz2; duzdz = xr * tz3;
+  }
+
+  realw e_xx = 0, e_yy = 0, e_xy = 0, e_xz = 0, e_yz = 0, e_tr = 0;
+  if (STRAIN) {
+    realw sxx, sxy, sxz, syx, syy, syz, szx, szy, szz;
+    if (ATT) {
+      // strain at u + dt v: continue each contraction with the dt*v tile
+      const realw *wx = sU[3], *wy = sU[4], *wz = sU[5];
+      realw a1 = tx1, a2 = tx2, a3 = tx3, c1 = ty1, c2 = ty2, c3 = ty3, d1 = tz1, d2 = tz2, d3 = tz3;
+#pragma unroll
+      for (int l = 0; l < 5; l++) {
+        const realw h1 = G.hp[i * 5 + l], h2 = G.hp[j * 5 + l], h3 = G.hp[k * 5 + l];
+        a1 = a1 + wx[b1 + l] * h1; c1 = c1 + wy[b1 + l] * h1; d1 = d1 + wz[b1 + l] * h1;
+        a2 = a2 + wx[b2 + 5 * l] * h2; c2 = c2 + wy[b2 + 5 * l] * h2; d2 = d2 + wz[b2 + 5 * l] * h2;
+        a3 = a3 + wx[b3 + 25 * l] * h3; c3 = c3 + wy[b3 + 25 * l] * h3; d3 = d3 + wz[b3 + 25 * l] * h3;
+      }
+      if (irr) {
+        sxx = xixl * a1 + etaxl * a2 + gaxl * a3;
+        sxy = xiyl * a1 + etayl * a2 + gayl * a3;
+        sxz = xizl * a1 + etazl * a2 + gazl * a3;
+        syx = xixl * c1 + etaxl * c2 + gaxl * c3;
+        syy = xiyl * c1 + etayl * c2 + gayl * c3;
+        syz = xizl * c1 + etazl * c2 + gazl * c3;
+        szx = xixl * d1 + etaxl * d2 + gaxl * d3;
+        szy = xiyl * d1 + etayl * d2 + gayl * d3;
+        szz = xizl * d1 + etazl * d2 + gazl * d3;
+      } else {
+        const realw xr = A.xix_regular;
+        sxx = xr * a1; sxy = xr * a2; sxz = xr * a3;
+        syx = xr * c1; syy = xr * c2; syz = xr * c3;
+        szx = xr * d1; szy = xr * d2; szz = xr * d3;
+      }
+    } else {
+      sxx = duxdx; sxy = duxdy; sxz = duxdz; syx = duydx; syy = duydy; syz = duydz;
+      szx = duzdx; szy = duzdy; szz = duzdz;
+    }
+    e_tr = (sxx + syy + szz);
+    const realw templ = ONE_THIRD * (sxx + syy + szz);
+    if (A.eps_tr3 && active) A.eps_tr3[ep] = templ;
+    e_xx = sxx - templ;
+    e_yy = syy - templ;
+    e_xy = (realw)0.5 * (sxy + syx);
+    e_xz = (realw)0.5 * (szx + sxz);
+    e_yz = (realw)0.5 * (szy + syz);
+  }
+
+  const realw kappal = km.x, mul = km.y;
+  const realw lp2m = kappal + FOUR_THIRDS * mul;
+  const realw lambdal = lp2m - (realw)2 * mul;
+  realw sigma_xx = lp2m * duxdx + lambdal * (duydy + duzdz);
+  realw sigma_yy = lp2m * duydy + lambdal * (duxdx + duzdz);
+  realw sigma_zz = lp2m * duzdz + lambdal * (duxdx + duydy);
+  realw sigma_xy = mul * (duxdy + duydx);
+  realw sigma_xz = mul * (duzdx + duxdz);
+  realw sigma_yz = mul * (duzdy + duydz);
+
+  if (ATT) {
+    const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
+    const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
+    const realw rtr = (Rv[5][0] + Rv[5][1]) + Rv[5][2];
+    sigma_xx = sigma_xx - rxx - rtr;
+    sigma_yy = sigma_yy - ryy - rtr;
+    sigma_zz = sigma_zz + rxx + ryy - rtr;
+    sigma_xy = sigma_xy - ((Rv[2][0] + Rv[2][1]) + Rv[2][2]);
+    sigma_xz = sigma_xz - ((Rv[3][0] + Rv[3][1]) + Rv[3][2]);
+    sigma_yz = sigma_yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
+    if (active) {
+      // memory-variable update (second-order RK) with old strain Sn and new strain Snp1
+      const realw Snp1[6] = {e_xx, e_yy, e_xy, e_xz, e_yz, e_tr};
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        const realw flk = kappal * fckv[s];
+        A.R[(5 * 3 + s) * nl + ep] = A.alpha[s] * Rv[5][s] + flk * (A.beta[s] * Sn[5] + A.gamma[s] * Snp1[5]);
+        const realw flm = mul * fcv[s];
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++)
+          A.R[(cc * 3 + s) * nl + ep] = A.alpha[s] * Rv[cc][s] + flm * (A.beta[s] * Sn[cc] + A.gamma[s] * Snp1[cc]);
+      }
+    }
+  }
+  if (STRAIN && active) {
+    realw* Ep = A.eps;
+    Ep[0 * nl + ep] = e_xx; Ep[1 * nl + ep] = e_yy; Ep[2 * nl + ep] = e_xy;
+    Ep[3 * nl + ep] = e_xz; Ep[4 * nl + ep] = e_yz;
+    if (ATT) Ep[5 * nl + ep] = e_tr;
+  }
+
+  {
+    const realw syx = sigma_xy, szx = sigma_xz, szy = sigma_yz;
+    realw t[9];
+    if (irr) {
+      t[0] = jacl * (sigma_xx * xixl + syx * xiyl + szx * xizl);
+      t[1] = jacl * (sigma_xy * xixl + sigma_yy * xiyl + szy * xizl);
+      t[2] = jacl * (sigma_xz * xixl + sigma_yz * xiyl + sigma_zz * xizl);
+      t[3] = jacl * (sigma_xx * etaxl + syx * etayl + szx * etazl);
+      t[4] = jacl * (sigma_xy * etaxl + sigma_yy * etayl + szy * etazl);
+      t[5] = jacl * (sigma_xz * etaxl + sigma_yz * etayl + sigma_zz * etazl);
+      t[6] = jacl * (sigma_xx * gaxl + syx * gayl + szx * gazl);
+      t[7] = jacl * (sigma_xy * gaxl + sigma_yy * gayl + szy * gazl);
+      t[8] = jacl * (sigma_xz * gaxl + sigma_yz * gayl + sigma_zz * gazl);
+    } else {
+      const realw xr = A.xix_regular;
+      t[0] = jacl * sigma_xx * xr;
+      t[1] = jacl * sigma_xy * xr;
+      t[2] = jacl * sigma_xz * xr;
+      t[3] = jacl * syx * xr;
+      t[4] = jacl * sigma_yy * xr;
+      t[5] = jacl * sigma_yz * xr;
+      t[6] = jacl * szx * xr;
+      t[7] = jacl * szy * xr;
+      t[8] = jacl * sigma_zz * xr;
+    }
+#pragma unroll
+    for (int m = 0; m < 9; m++) sT[m][p] = t[m];
+  }
+  __syncthreads();
+  if (!active) return;
+
+  realw f[3];
+#pragma unroll
+  for (int cc = 0; cc < 3; cc++) {
+    const realw *s1 = sT[cc], *s2 = sT[3 + cc], *s3 = sT[6 + cc];
+    realw n1 = G.hpw[i] * s1[b1];
+    realw n2 = G.hpw[j] * s2[b2];
+    realw n3 = G.hpw[k] * s3[b3];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      n1 = n1 + G.hpw[l * 5 + i] * s1[b1 + l];
+      n2 = n2 + G.hpw[l * 5 + j] * s2[b2 + 5 * l];
+      n3 = n3 + G.hpw[l * 5 + k] * s3[b3 + 25 * l];
+    }
+    f[cc] = G.wxy[j * 5 + k] * n1 + G.wxy[i * 5 + k] * n2 + G.wxy[i * 5 + j] * n3;
+  }
+  if (MODE == 1) {
+    A.local_out[ep] = f[0];
+    A.local_out[nl + ep] = f[1];
+    A.local_out[2 * nl + ep] = f[2];
+  } else if (MODE == 2) {
+    realw* a = A.accel + 3 * (int64_t)g;
+    a[0] = a[0] + (-f[0]);
+    a[1] = a[1] + (-f[1]);
+    a[2] = a[2] + (-f[2]);
+  } else {
+    realw* a = A.accel + 3 * (int64_t)g;
+    atomicAdd(a + 0, -f[0]);
+    atomicAdd(a + 1, -f[1]);
+    atomicAdd(a + 2, -f[2]);
+  }
+}
+
 // element strain of a displacement field (compute_element_strain.F90:28-130); one lane per point
-__global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, realw* eps, realw* eps_tr3) {
+__global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, realw* eps, realw* eps_tr3,
+                                                        int with_trace) {
   __shared__ realw s_buf[5][3][128];
   const int tid = threadIdx.x;
   const int el = tid / 25;
@@ -379,34 +630,59 @@ __global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, r
     eps[2 * nl + ep] = (realw)0.5 * (duxdy + duydx);
     eps[3 * nl + ep] = (realw)0.5 * (duzdx + duxdz);
     eps[4 * nl + ep] = (realw)0.5 * (duzdy + duydz);
+    // with attenuation the trace plane feeds the bulk memory variables (compute_strain_att,
+    // compute_element_strain.F90:196: epsilondev_trace = 3 * epsilon_trace_over_3)
+    if (with_trace) eps[5 * nl + ep] = (realw)3 * templ;
   }
 }
 
 }  // namespace SF_VNS
 using namespace SF_VNS;
 
-void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool ordered) {
+void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool att, int mode, int shape) {
   if (a.nelem <= 0) return;
-  dim3 grid((a.nelem + 4) / 5), block(128);
   const GLLk G = g_gll_host;
-#define SF_K(S, T, O) hipLaunchKernelGGL((k_elastic<S, T, O>), grid, block, 0, s, a, G)
-  if (ordered) {
-    if (att) SF_K(true, true, true);
-    else if (strain) SF_K(true, false, true);
-    else SF_K(false, false, true);
-  } else {
-    if (att) SF_K(true, true, false);
-    else if (strain) SF_K(true, false, false);
-    else SF_K(false, false, false);
+  if (shape == 1) {
+    dim3 grid(a.nelem), block(128);
+#define SF_K(S, T, O) hipLaunchKernelGGL((k_elastic_pt<S, T, O>), grid, block, 0, s, a, G)
+#define SF_KM(O)                      \
+  if (att) SF_K(true, true, O);       \
+  else if (strain) SF_K(true, false, O); \
+  else SF_K(false, false, O);
+    if (mode == 1) {
+      SF_KM(1)
+    } else if (mode == 2) {
+      SF_KM(2)
+    } else {
+      SF_KM(0)
+    }
+#undef SF_KM
+#undef SF_K
+    SF_LAUNCH_CHECK("k_elastic_pt");
+    return;
   }
+  dim3 grid((a.nelem + 4) / 5), block(128);
+#define SF_K(S, T, O) hipLaunchKernelGGL((k_elastic<S, T, O>), grid, block, 0, s, a, G)
+#define SF_KM(O)                      \
+  if (att) SF_K(true, true, O);       \
+  else if (strain) SF_K(true, false, O); \
+  else SF_K(false, false, O);
+  if (mode == 1) {
+    SF_KM(1)
+  } else if (mode == 2) {
+    SF_KM(2)
+  } else {
+    SF_KM(0)
+  }
+#undef SF_KM
 #undef SF_K
   SF_LAUNCH_CHECK("k_elastic");
 }
 
-void SFX(launch_element_strain)(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3) {
+void SFX(launch_element_strain)(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace) {
   if (a.nelem <= 0) return;
   dim3 grid((a.nelem + 4) / 5), block(128);
-  hipLaunchKernelGGL(k_element_strain, grid, block, 0, s, a, g_gll_host, eps, eps_tr3);
+  hipLaunchKernelGGL(k_element_strain, grid, block, 0, s, a, g_gll_host, eps, eps_tr3, with_trace ? 1 : 0);
   SF_LAUNCH_CHECK("k_element_strain");
 }
 

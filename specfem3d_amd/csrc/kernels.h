@@ -44,8 +44,14 @@ struct ElasticArgs {
   realw alpha[3], beta[3], gamma[3];
   realw* local_out;        // ordered assembly: 3 planes of nloc
 };
-void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, bool ordered);
-void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool exact);
+// mode: 0 atomic scatter, 1 store local contributions (ordered gather), 2 plain read-modify-write
+// (the element list is one colour: its elements share no global point); shape: 0 five elements
+// per workgroup with a GLL column per lane (k_elastic), 1 one element per workgroup with a GLL
+// point per lane (k_elastic_pt)
+void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode, int shape);
+// deviatoric strain planes 0..4 and trace/3 of A.displ; with_trace also writes plane 5 = 3 * trace/3
+void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,
+                           bool exact);
 
 struct AcousticArgs {
   int nelem;
@@ -60,7 +66,7 @@ struct AcousticArgs {
   realw* potdd;
   realw* local_out;        // ordered assembly: 1 plane of nloc
 };
-void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, bool ordered);
+void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, int mode);
 
 // ordered assembly: out[g*ncomp + c] -= contributions in element order
 void launch_ordered_gather(hipStream_t s, int nglob, int ncomp, const int* ptr, const int* idx,

@@ -14,6 +14,8 @@
 //     the per-point impedances gathered once at prepare time.
 #pragma once
 
+#include <vector>
+
 #include "common.h"
 
 namespace sf {
@@ -52,10 +54,17 @@ struct FacePoints {          // flattened boundary/coupling face points
   int* slot = nullptr;       // original host slot (face*25 + igll) of each point, for records
 };
 
+struct Snapshot {            // one saved state of a wavefield (save_forward_arrays_undoatt)
+  realw* el = nullptr;       // displ, veloc, accel (3 x 3*nglob)
+  realw* ac = nullptr;       // potential, first and second derivatives (3 x nglob)
+  realw* R = nullptr;        // 18 memory-variable planes (attenuation)
+};
+
 struct Mesh {
   int nspec = 0, nglob = 0, nspec_irregular = 0;
   int simulation_type = 1;
   int save_forward = 0;
+  int undo_attenuation = 0;        // UNDO_ATTENUATION_AND_OR_PML: checkpointed kernel runs
   int myrank = 0;
 
   // geometry
@@ -142,7 +151,8 @@ struct Mesh {
   // execution
   hipStream_t stream = nullptr;
   int exact = 1;                   // 1: kernels built without fused multiply-add (oracle order)
-  int assembly = 0;                // 0 atomic scatter-add, 1 ordered gather (oracle order)
+  int assembly = 0;                // 0 atomic scatter-add, 1 ordered gather (oracle order), 2 coloured
+  int elastic_shape = 1;           // 0 column-per-lane kernel (5 elements/WG), 1 point-per-lane
   // deferred corrector (kernel_3) per field, fused into the next predictor when possible
   int pending_k3_el[2] = {0, 0};
   realw pending_dt2_el[2] = {0.f, 0.f};
@@ -156,6 +166,18 @@ struct Mesh {
   int* asm_ac_ptr = nullptr;       // same for the acoustic elements (1 component)
   int* asm_ac_idx = nullptr;
   realw* asm_ac_buf = nullptr;
+  // coloured assembly (assembly = 2): element lists regrouped by colour; elements of one
+  // colour share no global point, colour c is elems[off[c] .. off[c+1])
+  int el_ncolor = 0, ac_ncolor = 0;
+  int el_color_off[33] = {0}, ac_color_off[33] = {0};
+  int* el_color_elems = nullptr;
+  int* ac_color_elems = nullptr;
+
+  // device-resident checkpoints of checkpointed adjoint runs (abi_checkpoint.hip)
+  std::vector<Snapshot> snapshots;
+  int history_frames = 0;
+  realw* history_el = nullptr;     // history_frames x 3*nglob displacement frames
+  realw* history_ac = nullptr;     // history_frames x 2*nglob (potential, second derivative)
 
   // scratch for reductions
   realw* d_norm = nullptr;

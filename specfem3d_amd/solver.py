@@ -40,7 +40,7 @@ class GPUSolver:
     def __init__(self, db: Database, sources=None, receivers=None, nstep: int = 1000,
                  attenuation=None, exact: bool = True, assembly: int = 0, simulation_type: int = 1,
                  save_forward: bool = False, save_seismograms=("d",), nlength: int | None = None,
-                 device: int = 0):
+                 device: int = 0, tuning: dict | None = None, undo_attenuation: bool = False):
         self.L = abi.load()
         self.db = db
         self.sources = sources
@@ -56,6 +56,8 @@ class GPUSolver:
         self.seismo_current = 0
         self.nlength = int(nlength or nstep)
         self.save = save_seismograms
+        self.tuning = dict(tuning or {})
+        self.undo_att = bool(undo_attenuation)
         self._prepare(exact, assembly, save_forward, device)
 
     # ------------------------------------------------------------------------------ set-up
@@ -104,8 +106,11 @@ class GPUSolver:
             A.ia(db.ispec_is_acoustic.astype(np.int32)), A.ia(db.ispec_is_elastic.astype(np.int32)),
             A.i(0), A.i(int(save_forward)), A.fa(hx.ravel()), A.fa(hy.ravel()), A.fa(hz.ravel()),
             A.da(nu.ravel()), A.da(np.zeros(9 * max(nsrc, 1))), A.ia(np.zeros(max(nrec, 1), np.int32)),
-            A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(0), A.i(1), A.i(0), A.i(0), A.i(0))
+            A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(0), A.i(1), A.i(0),
+            A.i(int(self.undo_att)), A.i(0))
         L.sf_set_options(self.MP, C.c_int(int(exact)), C.c_int(int(assembly)))
+        for name, value in self.tuning.items():
+            L.sf_set_tuning(self.MP, name.encode(), C.c_int(int(value)))
         if nirr:
             # the CPU solver's jacobianstore (prepare_constants_device itself recomputes it)
             L.sf_set_jacobian(self.MP, A.fa(db.jacobian.ravel()))
@@ -147,6 +152,19 @@ class GPUSolver:
                 A.fa(Rz), A.fa(Rz), A.fa(Rz), A.fa(Rz), A.fa(Rz), A.fa(fc), A.fa(Rz), A.fa(z), A.fa(fck),
                 A.fa(al), A.fa(be), A.fa(ga), A.i(0), A.fa([0]), A.i(0), A.fa([0]), A.ia([0]), A.ia([0]), A.i(0),
                 A.i(int(self.acoustic)), A.i(0), A.i(0), A.ia([0]), A.i(0), *[A.fa([0])] * 21)
+            if self.sim == 3:
+                # backward/reconstructed field, its strain and memory variables, kernel arrays
+                zb = np.zeros(3 * nloc if has_att else 1, np.float32)
+                L.prepare_fields_elastic_adj_dev_(
+                    self.MP, A.i(3 * db.nglob), A.fa(z), A.fa(z), A.fa(z), A.fa(z), A.fa(z), A.fa(z), A.fa(z),
+                    A.i(3 * nloc if has_att else 1), A.fa(zb), A.fa(zb), A.fa(zb), A.fa(zb), A.fa(zb), A.fa(zb),
+                    A.fa(z), A.fa(al), A.fa(be), A.fa(ga), A.i(0), A.i(0))
+                if nrec:
+                    R = self.receivers
+                    L.prepare_sim2_or_3_const_device_(
+                        self.MP, A.i(nrec), A.i(self.nstep), A.fa(R.hxir.astype(np.float32).ravel()),
+                        A.fa(R.hetar.astype(np.float32).ravel()), A.fa(R.hgammar.astype(np.float32).ravel()),
+                        A.i(nrec), A.ia(np.zeros(nrec, np.int32)), A.ia(rec_ispec))
         n3 = 3 * db.nglob
         self.displ = np.zeros(n3, np.float32)
         self.veloc = np.zeros(n3, np.float32)

@@ -18,10 +18,11 @@ from conftest import GOLDEN, rel_l2, small_elastic_box, smooth_warp
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None):
+def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, shape=0):
     from specfem3d_amd.solver import GPUSolver
-    print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly}", flush=True)
-    g = GPUSolver(db, src, rec, nstep=nstep, exact=exact, assembly=assembly, attenuation=attenuation)
+    print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly} shape={shape}", flush=True)
+    g = GPUSolver(db, src, rec, nstep=nstep, exact=exact, assembly=assembly, attenuation=attenuation,
+                  tuning={"elastic_shape": shape})
     out = g.run().copy()
     g.close()
     return out
@@ -31,11 +32,12 @@ def run_oracle(oracle_mod, db, src, rec, nstep, attenuation=None):
     return oracle_mod.OracleSolver(db, src, rec, "f32", attenuation=attenuation).run(nstep)
 
 
+@pytest.mark.parametrize("shape", [0, 1])
 @pytest.mark.parametrize("warp", [False, True])
-def test_elastic_exact_ordered_bitwise(oracle_mod, warp):
+def test_elastic_exact_ordered_bitwise(oracle_mod, warp, shape):
     db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0) if warp else None)
     ref = run_oracle(oracle_mod, db, src, rec, 200)
-    out = run_gpu(db, src, rec, 200, True, 1)
+    out = run_gpu(db, src, rec, 200, True, 1, shape=shape)
     assert np.abs(ref).max() > 0
     assert np.array_equal(out, ref), rel_l2(out, ref)
 
@@ -48,15 +50,29 @@ def test_elastic_atomic_within_tolerance(oracle_mod, exact):
     assert rel_l2(out, ref) < 1e-5
 
 
+@pytest.mark.parametrize("shape", [0, 1])
 @pytest.mark.parametrize("att", [False, True])
-def test_coupled_acoustic_viscoelastic(oracle_mod, att):
+def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     from specfem3d_amd import workloads as W
     w = W.config2(scale=0.125, nstep=200, with_attenuation=att, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
-    out = run_gpu(w.db, w.sources, w.receivers, 200, True, 1, w.attenuation)
+    out = run_gpu(w.db, w.sources, w.receivers, 200, True, 1, w.attenuation, shape=shape)
     assert np.array_equal(out, ref), rel_l2(out, ref)
-    fast = run_gpu(w.db, w.sources, w.receivers, 200, False, 0, w.attenuation)
+    fast = run_gpu(w.db, w.sources, w.receivers, 200, False, 0, w.attenuation, shape=shape)
     assert rel_l2(fast, ref) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("exact", [True, False])
+def test_colored_assembly_deterministic_within_tolerance(oracle_mod, exact, shape):
+    """Coloured scatter (no atomics): same answer on every run, within 1e-5 of the oracle."""
+    from specfem3d_amd import workloads as W
+    w = W.config2(scale=0.125, nstep=200, with_attenuation=True, nrec_side=3)
+    ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
+    a = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation, shape=shape)
+    b = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation, shape=shape)
+    assert np.array_equal(a, b)
+    assert rel_l2(a, ref) < 1e-5
 
 
 def test_config1_against_reference_fp32_fp64():
