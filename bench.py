@@ -68,6 +68,102 @@ def stiffness_bytes_per_launch(db, attenuation: bool) -> float:
     return float(per_glob * nglob_el + per_loc * nloc_el + 40 * nirr_el)
 
 
+def batch_stiffness_bytes(db, nshots: int) -> float:
+    """Algorithmic HBM bytes of one batched stiffness pass with every shot live: per shot and global
+    point u read (12 B) and accel read + write (24 B); per local point, shared by the shots, ibool +
+    kappa + mu (12 B) and 40 B of Jacobian terms in irregular elements."""
+    import numpy as np
+    el = db.ispec_is_elastic
+    nloc = int(el.sum()) * 125
+    nglob_el = int(np.unique(db.ibool[el]).size)
+    nirr = int((db.irregular_element_number[el] > 0).sum()) * 125
+    return float(36 * nglob_el * nshots + 12 * nloc + 40 * nirr)
+
+
+def batched_measurement(args) -> dict:
+    """Many shots on one mesh: 64x64x32 elastic box (gradient + GRF model, no attenuation), 32 shots with
+    seeded random CMT sources (Ricker 0.5-2 Hz), 50 receivers each and record lengths drawn from
+    1000-4000 steps, run to the end of the longest record with autotuned shots-per-workgroup/assembly and
+    hipGraph-captured step groups. Also times the plain single-shot path on the same mesh."""
+    import numpy as np
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.batch import BatchSolver
+    from specfem3d_amd.solver import GPUSolver
+    w = W.config3(scale=args.batch_scale, nshots=args.batch_shots)
+    db = w.db
+    nsteps = [n for _, _, n in w.shots]
+    log(f"[bench] batched workload {w.name}: NGLOB {db.nglob}, record lengths {min(nsteps)}-{max(nsteps)}")
+    b = BatchSolver(db, w.shots, exact=False)
+    choice = b.autotune(trial_steps=5)
+    log(f"[bench] autotuner: {choice}")
+    kms, sms = b.time(20)            # all shots live (first records are >= 1000 steps)
+    b.reset()
+    b.synchronize()
+    t = time.perf_counter()
+    b.run(graph_steps=10)
+    b.synchronize()
+    wall = time.perf_counter() - t
+    finite = all(bool(np.isfinite(b.seismograms(i)).all()) for i in range(len(w.shots)))
+    b.close()
+    value = db.nglob * float(sum(nsteps)) / wall
+    kbytes = batch_stiffness_bytes(db, len(w.shots))
+    ach = kbytes / (kms / 1e3) / 1e9
+    # plain single-shot path on the same mesh (first shot), timed the same way
+    src, rec, n = w.shots[0]
+    ps = min(args.plain_steps, n)
+    g = GPUSolver(db, src, rec, nstep=ps, exact=False, assembly=0)
+    for _ in range(10):
+        g.step()
+    g.synchronize()
+    tp = time.perf_counter()
+    for _ in range(ps - 10):
+        g.step()
+    g.synchronize()
+    plain = db.nglob * (ps - 10) / (time.perf_counter() - tp)
+    g.close()
+    log(f"[bench] batched {value:.3e} GLL-pt updates/s in {wall:.2f}s (stiffness {kms:.3f} ms/step, step {sms:.3f} ms "
+        f"with all shots live), plain single shot {plain:.3e}, ratio {value / plain:.2f}, finite={finite}")
+    return {"workload": w.name, "value": value, "unit": "GLL-point updates/s", "wall_s": round(wall, 3),
+            "shot_steps": int(sum(nsteps)), "plain_single_shot": plain, "ratio_vs_plain": value / plain,
+            "autotuned": choice, "seismograms_finite": finite,
+            "roofline": {"bound": "hbm", "kernel": "k_elastic_batch (all shots live)", "achieved": round(ach, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "bytes_per_launch": kbytes, "avg_pass_ms": round(kms, 4), "step_ms": round(sms, 4)}}
+
+
+def adjoint_measurement(args) -> dict:
+    """Wall time of a checkpointed sensitivity-kernel run: forward with a snapshot every 64 steps +
+    adjoint with recomputation (kernels copied back to the host included), on the 48^3 elastic
+    workload with an L2 waveform misfit against synthetics of a 3% Gaussian mu perturbation."""
+    import numpy as np
+    from specfem3d_amd import adjoint as A
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.autograd import perturbed_database
+    from specfem3d_amd.solver import GPUSolver
+    w = W.config4(scale=args.adjoint_scale, nstep=args.adjoint_steps)
+    db = w.db
+    n = args.adjoint_steps
+    g = GPUSolver(perturbed_database(db, dlnmu=0.03 * W.gaussian_bump(db)), w.sources, w.receivers, nstep=n,
+                  exact=False, assembly=0)
+    obs = g.run().copy()
+    g.close()
+    log(f"[bench] adjoint workload {w.name}: NGLOB {db.nglob}, observed data computed")
+    ks = A.KernelSolver(db, w.sources, w.receivers, n, nt_dump=64)
+    ks.synchronize()
+    t = time.perf_counter()
+    syn = ks.forward()
+    tf = time.perf_counter() - t
+    rk, mk, kk = ks.adjoint(A.waveform_adjoint_source(syn, obs))
+    wall = time.perf_counter() - t
+    ks.close()
+    chi = A.waveform_misfit(syn, obs, db.dt)
+    finite = bool(np.isfinite(mk).all() and np.abs(mk).max() > 0)
+    log(f"[bench] kernel run {wall:.2f}s (forward with snapshots {tf:.2f}s), misfit {chi:.4e}, kernels finite={finite}")
+    return {"workload": w.name, "wall_s": round(wall, 3), "forward_s": round(tf, 3), "steps": n, "nt_dump": 64,
+            "nglob": db.nglob, "misfit": chi, "kernels_finite": finite,
+            "unit": "s (forward with checkpoints + adjoint with recompute)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +175,12 @@ def main():
                     help="0 atomic scatter, 1 ordered (CPU element order), 2 coloured (no atomics)")
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-shots", type=int, default=32, help="shots of the batched 64x64x32 workload (0: skip)")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="shrink the batched mesh (tests only)")
+    ap.add_argument("--plain-steps", type=int, default=300, help="steps of the single-shot run on the batched mesh")
+    ap.add_argument("--adjoint-steps", type=int, default=4000,
+                    help="steps of the checkpointed kernel run on the 48^3 adjoint workload (0: skip)")
+    ap.add_argument("--adjoint-scale", type=float, default=1.0, help="shrink the adjoint mesh (tests only)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -157,6 +259,9 @@ def main():
                "sample": f"{args.cpu_steps} time steps of the same {w.name} mesh (FP32 C oracle, 1 thread)"}
         log(f"[bench] CPU oracle {args.cpu_steps} steps in {ct:.1f}s")
 
+    adj = adjoint_measurement(args) if args.adjoint_steps > 0 else None
+    batched = batched_measurement(args) if args.batch_shots > 0 else None
+
     out = {
         "metric": "GLL-point updates/s/GPU (forward, single shot)",
         "value": value,
@@ -175,6 +280,8 @@ def main():
                    "assembly": ["atomic", "ordered", "coloured"][args.assembly]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -260,6 +260,25 @@ void sf_history_load(long* Mesh_pointer, int FORWARD_OR_ADJOINT, int frame);
 void sf_checkpoints_free(long* Mesh_pointer);
 /* zero the sensitivity-kernel accumulators (prepare_fields_*_adj_dev zero them once) */
 void sf_zero_kernels(long* Mesh_pointer);
+/* batched many-shot forward engine (no counterpart in the reference, which runs one shot per
+   process): shots with their own sources, receivers and record lengths stepped together on one
+   prepared elastic mesh in shot-interleaved wavefields; element ids 0-based */
+long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep, realw deltat, realw deltatover2,
+                     realw deltatsqover2);
+void sf_batch_add_source(long batch, int shot, int ispec, const realw* sourcearray, const realw* stf);
+void sf_batch_add_receiver(long batch, int shot, int ispec, const double* hxi, const double* heta,
+                           const double* hgamma, const double* nu);
+void sf_batch_finalize(long batch);
+void sf_batch_set_options(long batch, int shots_per_group, int assembly);
+void sf_batch_get_options(long batch, int* shots_per_group, int* assembly);
+double sf_batch_autotune(long batch, int trial_steps);
+void sf_batch_run(long batch, int nsteps, int graph_steps);
+int sf_batch_current_step(long batch);
+void sf_batch_seismograms(long batch, int shot, realw* out);
+void sf_batch_displ(long batch, int shot, realw* out);
+void sf_batch_reset(long batch);
+void sf_batch_time(long batch, int nsteps, double* stiffness_ms, double* step_ms);
+void sf_batch_destroy(long batch);
 /* greedy first-fit element colouring of assembly = 2 (host only): color[q] for elems[q], elements
    sharing a global point differ; returns the number of colours or -1 (more than 32) */
 int sf_color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* color);

@@ -542,6 +542,15 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
 }
 
 namespace sf {
+// elastic element colours for coloured launches that are not the mesh's own assembly mode
+// (the batched engine)
+void ensure_element_colors(Mesh* m) {
+  if (m->el_color_elems || m->nspec_elastic <= 0 || !m->elastic_elems) return;
+  std::vector<int> lst(m->nspec_elastic);
+  SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+  build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
+}
+
 // Greedy first-fit colouring of an element list (in list order): an element takes the lowest
 // colour not yet used by any element sharing one of its global points. Each colour is then one
 // launch whose element-to-global scatter needs no atomics. Elements keep their list order

@@ -1,0 +1,615 @@
+// batch.hip -- batched many-shot forward engine: many independent shots (own sources, receivers
+// and record lengths) on one mesh prepared through the reference C-ABI.
+//
+// The reference runs one shot per process (one Mesh_pointer, one source set, iterate_time.F90).
+// Here a Batch borrows a prepared elastic Mesh (ibool, moduli, Jacobians, inverse mass, Stacey
+// face points, element colours) and steps S interleaved wavefields together:
+//   * layout: value (g, c, s) at ((int64_t)g * 3 + c) * S + s (shot-fastest rows, see k_batch.hip);
+//   * shots are placed in decreasing order of record length, so the live shots at any step are
+//     a prefix of the row and finished shots cost no memory traffic;
+//   * one step = fused corrector/predictor, Stacey rounds, source rounds, stiffness (one launch per
+//     element colour with plain read-modify-writes, or one atomic launch), receivers, and a
+//     one-lane kernel that advances the device step counter. Every kernel reads the counter, so
+//     a group of steps is captured once into a hipGraph and replayed;
+//   * the per-point arithmetic and the order of every sum equal the single-shot path (same
+//     kernels' expressions, same colour order, same face rounds), so each shot of a batch is
+//     bit-identical to its single-shot run with the same options.
+// Shots-per-workgroup and the assembly mode are chosen by an autotuner that times each candidate
+// for a few steps from rest and then resets the batch.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "kernels.h"
+
+namespace sf {
+void ensure_element_colors(Mesh* m);
+
+struct Batch {
+  Mesh* m = nullptr;
+  int nshots = 0, S = 0;
+  int sc = 4, mode = 2;          // shots per workgroup, assembly (0 atomic, 2 coloured)
+  int max_nstep = 0;
+  realw dt = 0.f, dt2 = 0.f, dtsq2 = 0.f;  // deltat, deltatover2, deltatsqover2
+  std::vector<int> nstep_user;   // per user shot
+  std::vector<int> slot_of;      // user shot -> slot
+  int* d_nstep = nullptr;        // per slot (S entries, decreasing, padding 0)
+  int* d_it = nullptr;
+  int h_it = 1;                  // host mirror of the step counter
+  realw *u = nullptr, *v = nullptr, *a = nullptr;
+  // sources: (source, point) entries sorted into rounds (the k-th source of each shot)
+  struct Src { int shot, ispec; std::vector<realw> arr, stf; };
+  std::vector<Src> srcs;
+  int src_n = 0, src_nrounds = 0;
+  std::vector<int> src_round_off;
+  int *src_ig = nullptr, *src_slot = nullptr, *src_id = nullptr, *src_ploc = nullptr;
+  realw *src_arr = nullptr, *src_stf = nullptr;
+  // receivers
+  struct Rec { int shot, ispec; double h[125]; double nu[9]; };
+  std::vector<Rec> recs;
+  int nrec = 0;
+  std::vector<int> rec_order;    // device receiver index -> position in recs
+  int *rec_slot = nullptr, *rec_ig = nullptr;
+  double *rec_h = nullptr, *rec_nu = nullptr;
+  realw* seis = nullptr;         // [max_nstep][nrec][3]
+  bool ready = false;
+  // captured step groups
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+  // timing of the stiffness launches (sf_batch_time)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  int ev_used = 0;
+};
+
+static Batch* batch_of(long h) {
+  if (!h) fatal("sf_batch", "null batch handle");
+  return reinterpret_cast<Batch*>(h);
+}
+
+// ------------------------------------------------------------------------------ kernels
+static __device__ __forceinline__ int load_it(const int* it) {
+  return __hip_atomic_load(it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Newmark update of every live (row, shot): predictor only on the first step, otherwise the
+// deferred corrector of the previous step fused with the predictor (k_update.hip arithmetic).
+__global__ void k_batch_update(int64_t nrow, int S, const int* __restrict__ nstep, const int* __restrict__ itp,
+                               realw* __restrict__ u, realw* __restrict__ v, realw* __restrict__ a,
+                               const realw* __restrict__ rm, realw dt2p, realw dt, realw dt2, realw dtsq2, int first) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = t / S;
+  const int s = (int)(t - row * S);
+  if (row >= nrow) return;
+  if (nstep[s] < load_it(itp)) return;
+  realw U = u[t], V = v[t], A = a[t];
+  if (!first) {
+    A = A * rm[row];
+    V = V + dt2p * A;
+  }
+  U = U + dt * V + dtsq2 * A;
+  V = V + dt2 * A;
+  u[t] = U;
+  v[t] = V;
+  a[t] = 0.f;
+}
+
+// Stacey (k_stacey_el arithmetic) for every (face point of one round, live shot)
+__global__ void k_batch_stacey(int n, int S, const int* __restrict__ ig, const realw* __restrict__ nrm,
+                               const realw* __restrict__ jw, const realw* __restrict__ rvp,
+                               const realw* __restrict__ rvs, const int* __restrict__ nstep,
+                               const int* __restrict__ itp, const realw* __restrict__ veloc, realw* accel) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = (int)(t / S);
+  const int s = (int)(t - (int64_t)q * S);
+  if (q >= n) return;
+  if (nstep[s] < load_it(itp)) return;
+  const int64_t b = (int64_t)ig[q] * 3 * S + s;
+  const realw vx = veloc[b], vy = veloc[b + S], vz = veloc[b + 2 * S];
+  const realw nx = nrm[3 * q], ny = nrm[3 * q + 1], nz = nrm[3 * q + 2];
+  const realw vn = vx * nx + vy * ny + vz * nz;
+  const realw p = rvp[q], sv = rvs[q];
+  const realw tx = p * vn * nx + sv * (vx - vn * nx);
+  const realw ty = p * vn * ny + sv * (vy - vn * ny);
+  const realw tz = p * vn * nz + sv * (vz - vn * nz);
+  const realw w = jw[q];
+  const realw fx = tx * w, fy = ty * w, fz = tz * w;
+  accel[b] = accel[b] - fx;
+  accel[b + S] = accel[b + S] - fy;
+  accel[b + 2 * S] = accel[b + 2 * S] - fz;
+}
+
+// source entries of one round: accel += sourcearray * stf(it) (k_add_sources_el arithmetic)
+__global__ void k_batch_sources(int n, int S, int max_nstep, const int* __restrict__ ig, const int* __restrict__ slot,
+                                const int* __restrict__ sid, const int* __restrict__ ploc,
+                                const realw* __restrict__ arr, const realw* __restrict__ stf,
+                                const int* __restrict__ nstep, const int* __restrict__ itp, realw* accel) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int s = slot[t];
+  const int it = load_it(itp);
+  if (nstep[s] < it) return;
+  const realw* sa = arr + ((int64_t)sid[t] * 125 + ploc[t]) * 3;
+  const realw st = stf[(int64_t)sid[t] * max_nstep + it - 1];
+  const int64_t b = (int64_t)ig[t] * 3 * S + s;
+  accel[b] = accel[b] + sa[0] * st;
+  accel[b + S] = accel[b + S] + sa[1] * st;
+  accel[b + 2 * S] = accel[b + 2 * S] + sa[2] * st;
+}
+
+// receivers (k_seismo_el arithmetic): one wave64 per receiver, FP64 sums, rotation by nu
+__global__ void __launch_bounds__(64) k_batch_seismo(int nrec, int S, const int* __restrict__ slot,
+                                                     const int* __restrict__ ig, const double* __restrict__ h,
+                                                     const double* __restrict__ nu, const int* __restrict__ nstep,
+                                                     const int* __restrict__ itp, const realw* __restrict__ u,
+                                                     realw* __restrict__ seis) {
+  const int r = blockIdx.x;
+  if (r >= nrec) return;
+  const int s = slot[r];
+  const int it = load_it(itp);
+  if (nstep[s] < it) return;
+  const int lane = threadIdx.x;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int p = lane; p < 125; p += 64) {
+    const int64_t b = (int64_t)ig[(int64_t)r * 125 + p] * 3 * S + s;
+    const double hl = h[(int64_t)r * 125 + p];
+    sx += (double)u[b] * hl;
+    sy += (double)u[b + S] * hl;
+    sz += (double)u[b + 2 * S] * hl;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sx += __shfl_xor(sx, off, 64);
+    sy += __shfl_xor(sy, off, 64);
+    sz += __shfl_xor(sz, off, 64);
+  }
+  if (lane == 0) {
+    const double* n = nu + (int64_t)r * 9;
+    realw* o = seis + ((int64_t)(it - 1) * nrec + r) * 3;
+    o[0] = (realw)(n[0] * sx + n[1] * sy + n[2] * sz);
+    o[1] = (realw)(n[3] * sx + n[4] * sy + n[5] * sz);
+    o[2] = (realw)(n[6] * sx + n[7] * sy + n[8] * sz);
+  }
+}
+
+__global__ void k_batch_tick(int* itp) {
+  if (threadIdx.x == 0) itp[0] = itp[0] + 1;
+}
+
+__global__ void k_batch_fill(int64_t n, realw* p) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0.f;
+}
+
+// ------------------------------------------------------------------------------ host side
+template <typename T>
+static T* balloc(int64_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, sizeof(T) * (size_t)(n > 0 ? n : 1)) != hipSuccess)
+    fatal("sf_batch", "out of device memory for the batched wavefields");
+  return reinterpret_cast<T*>(p);
+}
+template <typename T>
+static void bfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+template <typename T>
+static T* bupload(const T* h, int64_t n) {
+  T* d = balloc<T>(n);
+  if (n > 0) SF_CHECK(hipMemcpy(d, h, sizeof(T) * (size_t)n, hipMemcpyHostToDevice));
+  return d;
+}
+
+static void drop_graph(Batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->graph_steps = 0;
+}
+
+static void zero_state(Batch* b) {
+  Mesh* m = b->m;
+  const int64_t n = 3 * (int64_t)m->nglob * b->S;
+  for (realw* p : {b->u, b->v, b->a})
+    hipLaunchKernelGGL(k_batch_fill, sf_grid(n, 256), dim3(256), 0, m->stream, n, p);
+  if (b->seis) {
+    const int64_t ns = (int64_t)b->max_nstep * b->nrec * 3;
+    hipLaunchKernelGGL(k_batch_fill, sf_grid(ns, 256), dim3(256), 0, m->stream, ns, b->seis);
+  }
+  SF_LAUNCH_CHECK("k_batch_fill");
+  b->h_it = 1;
+  SF_CHECK(hipMemcpyAsync(b->d_it, &b->h_it, sizeof(int), hipMemcpyHostToDevice, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+}
+
+static void enqueue_stiffness(Batch* b, hipStream_t st) {
+  Mesh* m = b->m;
+  BatchArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.ngroups = b->S / b->sc;
+  A.S = b->S;
+  A.ibool = m->ibool;
+  A.irregular = m->irregular;
+  A.xix_regular = m->xix_regular;
+  A.jacobian_regular = m->jacobian_regular;
+  A.jac = m->jac;
+  A.jac_plane = (int64_t)m->nspec_irregular * 125;
+  A.kappamu = m->kappamu;
+  A.u = b->u;
+  A.a = b->a;
+  A.nstep = b->d_nstep;
+  A.it = b->d_it;
+  if (b->mode == 2) {
+    for (int c = 0; c < m->el_ncolor; c++) {
+      A.elems = m->el_color_elems + m->el_color_off[c];
+      A.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
+      launch_elastic_batch(st, A, b->sc, 2, m->exact != 0);
+    }
+  } else {
+    A.elems = m->elastic_elems;
+    A.nelem = m->nspec_elastic;
+    launch_elastic_batch(st, A, b->sc, 0, m->exact != 0);
+  }
+}
+
+// one time step; `first` selects the plain predictor (no pending corrector)
+static void enqueue_step(Batch* b, hipStream_t st, bool first) {
+  Mesh* m = b->m;
+  const int64_t nrow = 3 * (int64_t)m->nglob;
+  const realw dt = b->dt, dt2 = b->dt2, dtsq2 = b->dtsq2;
+  hipLaunchKernelGGL(k_batch_update, sf_grid(nrow * b->S, 256), dim3(256), 0, st, nrow, b->S, b->d_nstep, b->d_it,
+                     b->u, b->v, b->a, m->rmass, dt2, dt, dt2, dtsq2, first ? 1 : 0);
+  SF_LAUNCH_CHECK("k_batch_update");
+  if (m->stacey) {
+    const FacePoints& f = m->abs_el;
+    for (int r = 0; r < f.nrounds; r++) {
+      const int o = f.round_off[r], cnt = f.round_off[r + 1] - o;
+      if (cnt <= 0) continue;
+      hipLaunchKernelGGL(k_batch_stacey, sf_grid((int64_t)cnt * b->S, 256), dim3(256), 0, st, cnt, b->S,
+                         f.iglob + o, f.normal + 3 * o, f.jacw + o, f.imp_p + o, f.imp_s + o, b->d_nstep, b->d_it,
+                         b->v, b->a);
+    }
+    SF_LAUNCH_CHECK("k_batch_stacey");
+  }
+  for (int r = 0; r < b->src_nrounds; r++) {
+    const int o = b->src_round_off[r], cnt = b->src_round_off[r + 1] - o;
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(k_batch_sources, sf_grid(cnt, 128), dim3(128), 0, st, cnt, b->S, b->max_nstep, b->src_ig + o,
+                       b->src_slot + o, b->src_id + o, b->src_ploc + o, b->src_arr, b->src_stf, b->d_nstep, b->d_it,
+                       b->a);
+    SF_LAUNCH_CHECK("k_batch_sources");
+  }
+  const bool timed = b->timing && b->ev_used + 2 <= (int)b->ev.size();
+  if (timed) SF_CHECK(hipEventRecord(b->ev[b->ev_used], st));
+  enqueue_stiffness(b, st);
+  if (timed) {
+    SF_CHECK(hipEventRecord(b->ev[b->ev_used + 1], st));
+    b->ev_used += 2;
+  }
+  if (b->nrec > 0) {
+    hipLaunchKernelGGL(k_batch_seismo, dim3(b->nrec), dim3(64), 0, st, b->nrec, b->S, b->rec_slot, b->rec_ig,
+                       b->rec_h, b->rec_nu, b->d_nstep, b->d_it, b->u, b->seis);
+    SF_LAUNCH_CHECK("k_batch_seismo");
+  }
+  hipLaunchKernelGGL(k_batch_tick, dim3(1), dim3(64), 0, st, b->d_it);
+  SF_LAUNCH_CHECK("k_batch_tick");
+}
+
+// advance n steps (eagerly for the first step and for a tail shorter than the captured group)
+static void run_steps(Batch* b, int n, int graph_steps) {
+  Mesh* m = b->m;
+  n = std::min(n, b->max_nstep - b->h_it + 1);
+  while (n > 0) {
+    if (b->h_it == 1 || graph_steps <= 1 || n < graph_steps) {
+      enqueue_step(b, m->stream, b->h_it == 1);
+      b->h_it++;
+      n--;
+      continue;
+    }
+    if (!b->gexec || b->graph_steps != graph_steps) {
+      drop_graph(b);
+      hipGraph_t g;
+      SF_CHECK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < graph_steps; k++) enqueue_step(b, m->stream, false);
+      SF_CHECK(hipStreamEndCapture(m->stream, &g));
+      SF_CHECK(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+      SF_CHECK(hipGraphDestroy(g));
+      b->graph_steps = graph_steps;
+    }
+    SF_CHECK(hipGraphLaunch(b->gexec, m->stream));
+    b->h_it += graph_steps;
+    n -= graph_steps;
+  }
+}
+
+}  // namespace sf
+
+using namespace sf;
+
+// Create a batch of `nshots` shots with record lengths nstep[i] on a prepared elastic mesh (after
+// prepare_constants_device_ and prepare_fields_elastic_device_; sources/receivers of the mesh
+// itself are not used), time step constants as the host computes them (prepare_timerun.F90:
+// deltat, deltatover2 = deltat/2, deltatsqover2 = deltat^2/2). Returns the handle; shots are
+// numbered 0..nshots-1 by the caller.
+extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep, realw deltat, realw deltatover2,
+                                realw deltatsqover2) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (nshots <= 0) fatal("sf_batch_create", "need at least one shot");
+  if (!m->rmass || !m->kappamu || m->nspec_elastic <= 0) fatal("sf_batch_create", "mesh has no prepared elastic domain");
+  if (m->nspec_acoustic > 0) fatal("sf_batch_create", "batched shots cover elastic meshes only");
+  if (m->attenuation) fatal("sf_batch_create", "batched shots do not support attenuation");
+  Batch* b = new Batch();
+  b->m = m;
+  b->dt = deltat;
+  b->dt2 = deltatover2;
+  b->dtsq2 = deltatsqover2;
+  b->nshots = nshots;
+  ensure_element_colors(m);
+  b->S = (nshots + 7) / 8 * 8;
+  b->nstep_user.assign(nstep, nstep + nshots);
+  for (int i = 0; i < nshots; i++)
+    if (nstep[i] <= 0) fatal("sf_batch_create", "record lengths must be positive");
+  b->max_nstep = *std::max_element(nstep, nstep + nshots);
+  std::vector<int> order(nshots);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nstep[x] > nstep[y]; });
+  b->slot_of.assign(nshots, 0);
+  std::vector<int> ns(b->S, 0);
+  for (int k = 0; k < nshots; k++) {
+    b->slot_of[order[k]] = k;
+    ns[k] = nstep[order[k]];
+  }
+  b->d_nstep = bupload(ns.data(), b->S);
+  b->d_it = balloc<int>(1);
+  const int64_t n = 3 * (int64_t)m->nglob * b->S;
+  b->u = balloc<realw>(n);
+  b->v = balloc<realw>(n);
+  b->a = balloc<realw>(n);
+  return reinterpret_cast<long>(b);
+}
+
+// Add a moment-tensor or force source to a shot: element ispec (0-based), sourcearray in the
+// reference's (NDIM,NGLLX,NGLLY,NGLLZ) order (component fastest), and the source-time-function
+// value used at every step of the shot (stf[it-1] = real(stf_pre_compute) of step it).
+extern "C" void sf_batch_add_source(long h, int shot, int ispec, const realw* sourcearray, const realw* stf) {
+  Batch* b = batch_of(h);
+  if (shot < 0 || shot >= b->nshots) fatal("sf_batch_add_source", "shot index out of range");
+  if (ispec < 0 || ispec >= b->m->nspec) fatal("sf_batch_add_source", "element index out of range");
+  Batch::Src s;
+  s.shot = shot;
+  s.ispec = ispec;
+  s.arr.assign(sourcearray, sourcearray + 375);
+  s.stf.assign(stf, stf + b->nstep_user[shot]);
+  b->srcs.push_back(std::move(s));
+  b->ready = false;
+}
+
+// Add a receiver to a shot: element ispec (0-based), Lagrange interpolants hxi/heta/hgamma (5 each,
+// double) and the rotation nu (3x3, nu_rec(3,3) column-major as the reference passes it).
+extern "C" void sf_batch_add_receiver(long h, int shot, int ispec, const double* hxi, const double* heta,
+                                      const double* hgamma, const double* nu) {
+  Batch* b = batch_of(h);
+  if (shot < 0 || shot >= b->nshots) fatal("sf_batch_add_receiver", "shot index out of range");
+  if (ispec < 0 || ispec >= b->m->nspec) fatal("sf_batch_add_receiver", "element index out of range");
+  Batch::Rec r;
+  r.shot = shot;
+  r.ispec = ispec;
+  for (int p = 0; p < 125; p++) r.h[p] = hxi[p % 5] * heta[(p / 5) % 5] * hgamma[p / 25];
+  // rotated component a = sum_b nu(a, b) u_b; nu_rec(3,3) column-major: nu(a,b) at a + 3 b
+  for (int x = 0; x < 3; x++)
+    for (int y = 0; y < 3; y++) r.nu[3 * x + y] = nu[x + 3 * y];
+  b->recs.push_back(r);
+  b->ready = false;
+}
+
+// Upload sources and receivers, zero the wavefields and seismograms, set the step counter to 1.
+extern "C" void sf_batch_finalize(long h) {
+  Batch* b = batch_of(h);
+  Mesh* m = b->m;
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  drop_graph(b);
+  std::vector<int> hib((size_t)m->nspec * 125);
+  SF_CHECK(hipMemcpy(hib.data(), m->ibool, sizeof(int) * hib.size(), hipMemcpyDeviceToHost));
+  // sources: round of an entry = index of its source within its shot
+  {
+    std::vector<int> k_in_shot(b->srcs.size()), count(b->nshots, 0);
+    for (size_t q = 0; q < b->srcs.size(); q++) k_in_shot[q] = count[b->srcs[q].shot]++;
+    const int nr = b->srcs.empty() ? 0 : *std::max_element(count.begin(), count.end());
+    std::vector<int> ig, slot, sid, ploc;
+    b->src_round_off.assign(nr + 1, 0);
+    for (int r = 0; r < nr; r++) {
+      b->src_round_off[r] = (int)ig.size();
+      for (size_t q = 0; q < b->srcs.size(); q++) {
+        if (k_in_shot[q] != r) continue;
+        for (int p = 0; p < 125; p++) {
+          ig.push_back(hib[(size_t)b->srcs[q].ispec * 125 + p]);
+          slot.push_back(b->slot_of[b->srcs[q].shot]);
+          sid.push_back((int)q);
+          ploc.push_back(p);
+        }
+      }
+    }
+    b->src_round_off[nr] = (int)ig.size();
+    b->src_nrounds = nr;
+    b->src_n = (int)ig.size();
+    std::vector<realw> arr(b->srcs.size() * 375), stf(b->srcs.size() * (size_t)b->max_nstep, 0.f);
+    for (size_t q = 0; q < b->srcs.size(); q++) {
+      // (c, i, j, k) with c fastest -> [p][c]
+      for (int p = 0; p < 125; p++)
+        for (int c = 0; c < 3; c++) arr[q * 375 + p * 3 + c] = b->srcs[q].arr[c + 3 * p];
+      std::copy(b->srcs[q].stf.begin(), b->srcs[q].stf.end(), stf.begin() + q * b->max_nstep);
+    }
+    bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
+    b->src_ig = bupload(ig.data(), (int64_t)ig.size());
+    b->src_slot = bupload(slot.data(), (int64_t)slot.size());
+    b->src_id = bupload(sid.data(), (int64_t)sid.size());
+    b->src_ploc = bupload(ploc.data(), (int64_t)ploc.size());
+    b->src_arr = bupload(arr.data(), (int64_t)arr.size());
+    b->src_stf = bupload(stf.data(), (int64_t)stf.size());
+  }
+  // receivers, device order = order of addition
+  {
+    b->nrec = (int)b->recs.size();
+    std::vector<int> slot(b->nrec), ig((size_t)b->nrec * 125);
+    std::vector<double> hh((size_t)b->nrec * 125), nu((size_t)b->nrec * 9);
+    for (int r = 0; r < b->nrec; r++) {
+      const Batch::Rec& R = b->recs[r];
+      slot[r] = b->slot_of[R.shot];
+      for (int p = 0; p < 125; p++) {
+        ig[(size_t)r * 125 + p] = hib[(size_t)R.ispec * 125 + p];
+        hh[(size_t)r * 125 + p] = R.h[p];
+      }
+      for (int q = 0; q < 9; q++) nu[(size_t)r * 9 + q] = R.nu[q];
+    }
+    bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);
+    b->rec_slot = bupload(slot.data(), (int64_t)slot.size());
+    b->rec_ig = bupload(ig.data(), (int64_t)ig.size());
+    b->rec_h = bupload(hh.data(), (int64_t)hh.size());
+    b->rec_nu = bupload(nu.data(), (int64_t)nu.size());
+    b->seis = balloc<realw>((int64_t)b->max_nstep * std::max(1, b->nrec) * 3);
+  }
+  b->ready = true;
+  zero_state(b);
+}
+
+// shots per workgroup (2, 4 or 8; must divide the padded shot count) and assembly (0 atomic, 2 coloured)
+extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) {
+  Batch* b = batch_of(h);
+  if (shots_per_group != 2 && shots_per_group != 4 && shots_per_group != 8)
+    fatal("sf_batch_set_options", "shots per workgroup must be 2, 4 or 8");
+  if (assembly != 0 && assembly != 2) fatal("sf_batch_set_options", "assembly must be 0 (atomic) or 2 (coloured)");
+  SF_CHECK(hipStreamSynchronize(b->m->stream));
+  drop_graph(b);
+  b->sc = shots_per_group;
+  b->mode = assembly;
+}
+
+extern "C" void sf_batch_get_options(long h, int* shots_per_group, int* assembly) {
+  Batch* b = batch_of(h);
+  *shots_per_group = b->sc;
+  *assembly = b->mode;
+}
+
+// advance `nsteps` steps (at most to the longest record); `graph_steps` > 1 captures that many
+// steps into one hipGraph and replays it
+extern "C" void sf_batch_run(long h, int nsteps, int graph_steps) {
+  Batch* b = batch_of(h);
+  if (!b->ready) fatal("sf_batch_run", "sf_batch_finalize not called after adding sources/receivers");
+  run_steps(b, nsteps, graph_steps);
+}
+
+extern "C" int sf_batch_current_step(long h) { return batch_of(h)->h_it; }
+
+// Seismograms of one shot: out[(it-1) * nrec_shot * 3 + r * 3 + c] for its receivers in the order
+// they were added, for it = 1..nstep of the shot. Synchronises the stream.
+extern "C" void sf_batch_seismograms(long h, int shot, realw* out) {
+  Batch* b = batch_of(h);
+  if (shot < 0 || shot >= b->nshots) fatal("sf_batch_seismograms", "shot index out of range");
+  SF_CHECK(hipStreamSynchronize(b->m->stream));
+  std::vector<int> mine;
+  for (int r = 0; r < b->nrec; r++)
+    if (b->recs[r].shot == shot) mine.push_back(r);
+  if (mine.empty()) return;
+  const int ns = b->nstep_user[shot];
+  std::vector<realw> all((size_t)ns * b->nrec * 3);
+  SF_CHECK(hipMemcpy(all.data(), b->seis, sizeof(realw) * all.size(), hipMemcpyDeviceToHost));
+  const size_t nr = mine.size();
+  for (int t = 0; t < ns; t++)
+    for (size_t q = 0; q < nr; q++)
+      for (int c = 0; c < 3; c++) out[((size_t)t * nr + q) * 3 + c] = all[((size_t)t * b->nrec + mine[q]) * 3 + c];
+}
+
+// Displacement of one shot (3*NGLOB, xyz interleaved) copied to the host; synchronises.
+extern "C" void sf_batch_displ(long h, int shot, realw* out) {
+  Batch* b = batch_of(h);
+  if (shot < 0 || shot >= b->nshots) fatal("sf_batch_displ", "shot index out of range");
+  SF_CHECK(hipStreamSynchronize(b->m->stream));
+  const int64_t n = 3 * (int64_t)b->m->nglob;
+  std::vector<realw> all((size_t)n * b->S);
+  SF_CHECK(hipMemcpy(all.data(), b->u, sizeof(realw) * all.size(), hipMemcpyDeviceToHost));
+  const int s = b->slot_of[shot];
+  for (int64_t q = 0; q < n; q++) out[q] = all[(size_t)q * b->S + s];
+}
+
+// Rewind to step 1 with zero wavefields and seismograms (keeps sources, receivers and options).
+extern "C" void sf_batch_reset(long h) {
+  Batch* b = batch_of(h);
+  SF_CHECK(hipStreamSynchronize(b->m->stream));
+  zero_state(b);
+}
+
+// Time `nsteps` eager steps from the current state: average milliseconds of the stiffness launches
+// per step (HIP events around them on the mesh stream) and of the whole step.
+extern "C" void sf_batch_time(long h, int nsteps, double* stiffness_ms, double* step_ms) {
+  Batch* b = batch_of(h);
+  Mesh* m = b->m;
+  nsteps = std::min(nsteps, b->max_nstep - b->h_it + 1);
+  if (nsteps <= 0) fatal("sf_batch_time", "no steps left to time");
+  b->ev.resize(2 * (size_t)nsteps + 2);
+  for (auto& e : b->ev) SF_CHECK(hipEventCreate(&e));
+  b->ev_used = 0;
+  b->timing = true;
+  hipEvent_t t0 = b->ev[2 * nsteps], t1 = b->ev[2 * nsteps + 1];
+  b->ev.resize(2 * (size_t)nsteps);
+  SF_CHECK(hipEventRecord(t0, m->stream));
+  run_steps(b, nsteps, 1);
+  SF_CHECK(hipEventRecord(t1, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  b->timing = false;
+  double tot = 0.0;
+  for (int k = 0; k + 1 < b->ev_used; k += 2) {
+    float ms = 0.f;
+    SF_CHECK(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
+    tot += ms;
+  }
+  float all = 0.f;
+  SF_CHECK(hipEventElapsedTime(&all, t0, t1));
+  *stiffness_ms = tot / std::max(1, b->ev_used / 2);
+  *step_ms = (double)all / nsteps;
+  for (auto& e : b->ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  b->ev.clear();
+  b->ev_used = 0;
+}
+
+// Autotuner: times every (shots per workgroup, assembly) candidate for `trial_steps` eager steps
+// from rest, keeps the fastest and rewinds the batch. Returns the chosen step time in ms.
+extern "C" double sf_batch_autotune(long h, int trial_steps) {
+  Batch* b = batch_of(h);
+  if (!b->ready) fatal("sf_batch_autotune", "sf_batch_finalize first");
+  double best = 1e300;
+  int best_sc = b->sc, best_mode = b->mode;
+  for (int mode : {2, 0})
+    for (int sc : {2, 4, 8}) {
+      if (b->S % sc) continue;
+      b->sc = sc;
+      b->mode = mode;
+      drop_graph(b);
+      zero_state(b);
+      run_steps(b, 2, 1);  // warm-up (first step runs the plain predictor)
+      double ks = 0.0, st = 0.0;
+      sf_batch_time(h, trial_steps, &ks, &st);
+      if (st < best) {
+        best = st;
+        best_sc = sc;
+        best_mode = mode;
+      }
+    }
+  b->sc = best_sc;
+  b->mode = best_mode;
+  drop_graph(b);
+  zero_state(b);
+  return best;
+}
+
+extern "C" void sf_batch_destroy(long h) {
+  Batch* b = batch_of(h);
+  SF_CHECK(hipStreamSynchronize(b->m->stream));
+  drop_graph(b);
+  bfree(b->d_nstep); bfree(b->d_it); bfree(b->u); bfree(b->v); bfree(b->a);
+  bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
+  bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);
+  delete b;
+}

@@ -17,7 +17,7 @@
 // The arithmetic order of every sum follows the reference (the five terms of each
 // contraction left to right, etc.). This file is compiled twice: with -ffp-contract=off
 // (SF_EXACT=1, same rounding as the CPU reference) and with FMA contraction (SF_EXACT=0).
-#include "kernels.h"
+#include "elastic_point.h"
 
 #ifndef SF_EXACT
 #define SF_EXACT 1
@@ -342,17 +342,11 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
     }
   }
   const int irr = A.irregular[e];
-  realw xixl = 0, xiyl = 0, xizl = 0, etaxl = 0, etayl = 0, etazl = 0, gaxl = 0, gayl = 0, gazl = 0, jacl;
+  JacPt J = {};
+  realw jacl = A.jacobian_regular;
   if (irr) {
-    const int64_t qq = (int64_t)(irr - 1) * 125 + q;
-    const realw* J = A.jac;
-    const int64_t P = A.jac_plane;
-    xixl = J[qq]; xiyl = J[P + qq]; xizl = J[2 * P + qq];
-    etaxl = J[3 * P + qq]; etayl = J[4 * P + qq]; etazl = J[5 * P + qq];
-    gaxl = J[6 * P + qq]; gayl = J[7 * P + qq]; gazl = J[8 * P + qq];
-    jacl = J[9 * P + qq];
-  } else {
-    jacl = A.jacobian_regular;
+    J = load_jac(A.jac, A.jac_plane, (int64_t)(irr - 1) * 125 + q);
+    jacl = J.jac;
   }
   const float2 km = A.kappamu[ep];
   realw Rv[6][3], Sn[6], fcv[3], fckv[3];
@@ -372,107 +366,37 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
   __syncthreads();
 
   const realw ONE_THIRD = (realw)1 / (realw)3;
-  const realw FOUR_THIRDS = (realw)4 / (realw)3;
-  const int b1 = 5 * j + 25 * k;  // f(l, j, k) = s[b1 + l]
-  const int b2 = i + 25 * k;      // f(i, l, k) = s[b2 + 5 l]
-  const int b3 = i + 5 * j;       // f(i, j, l) = s[b3 + 25 l]
-  realw tx1, ty1, tz1, tx2, ty2, tz2, tx3, ty3, tz3;
-  {
-    const realw *sx = sU[0], *sy = sU[1], *sz = sU[2];
-    const realw h1 = G.hp[i * 5], h2 = G.hp[j * 5], h3 = G.hp[k * 5];
-    tx1 = h1 * sx[b1]; ty1 = h1 * sy[b1]; tz1 = h1 * sz[b1];
-    tx2 = h2 * sx[b2]; ty2 = h2 * sy[b2]; tz2 = h2 * sz[b2];
-    tx3 = h3 * sx[b3]; ty3 = h3 * sy[b3]; tz3 = h3 * sz[b3];
-#pragma unroll
-    for (int l = 1; l < 5; l++) {
-      const realw a1 = G.hp[i * 5 + l], a2 = G.hp[j * 5 + l], a3 = G.hp[k * 5 + l];
-      tx1 = tx1 + a1 * sx[b1 + l]; ty1 = ty1 + a1 * sy[b1 + l]; tz1 = tz1 + a1 * sz[b1 + l];
-      tx2 = tx2 + a2 * sx[b2 + 5 * l]; ty2 = ty2 + a2 * sy[b2 + 5 * l]; tz2 = tz2 + a2 * sz[b2 + 5 * l];
-      tx3 = tx3 + a3 * sx[b3 + 25 * l]; ty3 = ty3 + a3 * sy[b3 + 25 * l]; tz3 = tz3 + a3 * sz[b3 + 25 * l];
-    }
-  }
-  realw duxdx, duxdy, duxdz, duydx, duydy, duydz, duzdx, duzdy, duzdz;
-  if (irr) {
-    duxdx = xixl * tx1 + etaxl * tx2 + gaxl * tx3;
-    duxdy = xiyl * tx1 + etayl * tx2 + gayl * tx3;
-    duxdz = xizl * tx1 + etazl * tx2 + gazl * tx3;
-    duydx = xixl * ty1 + etaxl * ty2 + gaxl * ty3;
-    duydy = xiyl * ty1 + etayl * ty2 + gayl * ty3;
-    duydz = xizl * ty1 + etazl * ty2 + gazl * ty3;
-    duzdx = xixl * tz1 + etaxl * tz2 + gaxl * tz3;
-    duzdy = xiyl * tz1 + etayl * tz2 + gayl * tz3;
-    duzdz = xizl * tz1 + etazl * tz2 + gazl * tz3;
-  } else {
-    const realw xr = A.xix_regular;
-    duxdx = xr * tx1; duxdy = xr * tx2; duxdz = xr * tx3;
-    duydx = xr * ty1; duydy = xr * ty2; duydz = xr * ty3;
-    duzdx = xr * tz1; duzdy = xr * tz2; duzdz = xr * tz3;
-  }
+  const Grad9 tg = tile_gradient<1>(sU[0], sU[1], sU[2], G, i, j, k);
+  const Du9 du = map_gradient(tg, irr != 0, J, A.xix_regular);
 
   realw e_xx = 0, e_yy = 0, e_xy = 0, e_xz = 0, e_yz = 0, e_tr = 0;
   if (STRAIN) {
-    realw sxx, sxy, sxz, syx, syy, syz, szx, szy, szz;
-    if (ATT) {
-      // strain at u + dt v: continue each contraction with the dt*v tile
-      const realw *wx = sU[3], *wy = sU[4], *wz = sU[5];
-      realw a1 = tx1, a2 = tx2, a3 = tx3, c1 = ty1, c2 = ty2, c3 = ty3, d1 = tz1, d2 = tz2, d3 = tz3;
-#pragma unroll
-      for (int l = 0; l < 5; l++) {
-        const realw h1 = G.hp[i * 5 + l], h2 = G.hp[j * 5 + l], h3 = G.hp[k * 5 + l];
-        a1 = a1 + wx[b1 + l] * h1; c1 = c1 + wy[b1 + l] * h1; d1 = d1 + wz[b1 + l] * h1;
-        a2 = a2 + wx[b2 + 5 * l] * h2; c2 = c2 + wy[b2 + 5 * l] * h2; d2 = d2 + wz[b2 + 5 * l] * h2;
-        a3 = a3 + wx[b3 + 25 * l] * h3; c3 = c3 + wy[b3 + 25 * l] * h3; d3 = d3 + wz[b3 + 25 * l] * h3;
-      }
-      if (irr) {
-        sxx = xixl * a1 + etaxl * a2 + gaxl * a3;
-        sxy = xiyl * a1 + etayl * a2 + gayl * a3;
-        sxz = xizl * a1 + etazl * a2 + gazl * a3;
-        syx = xixl * c1 + etaxl * c2 + gaxl * c3;
-        syy = xiyl * c1 + etayl * c2 + gayl * c3;
-        syz = xizl * c1 + etazl * c2 + gazl * c3;
-        szx = xixl * d1 + etaxl * d2 + gaxl * d3;
-        szy = xiyl * d1 + etayl * d2 + gayl * d3;
-        szz = xizl * d1 + etazl * d2 + gazl * d3;
-      } else {
-        const realw xr = A.xix_regular;
-        sxx = xr * a1; sxy = xr * a2; sxz = xr * a3;
-        syx = xr * c1; syy = xr * c2; syz = xr * c3;
-        szx = xr * d1; szy = xr * d2; szz = xr * d3;
-      }
-    } else {
-      sxx = duxdx; sxy = duxdy; sxz = duxdz; syx = duydx; syy = duydy; syz = duydz;
-      szx = duzdx; szy = duzdy; szz = duzdz;
-    }
-    e_tr = (sxx + syy + szz);
-    const realw templ = ONE_THIRD * (sxx + syy + szz);
+    Du9 st = du;
+    if (ATT) st = map_gradient(tile_gradient_continue<1>(tg, sU[3], sU[4], sU[5], G, i, j, k), irr != 0, J,
+                               A.xix_regular);
+    e_tr = (st.xx + st.yy + st.zz);
+    const realw templ = ONE_THIRD * (st.xx + st.yy + st.zz);
     if (A.eps_tr3 && active) A.eps_tr3[ep] = templ;
-    e_xx = sxx - templ;
-    e_yy = syy - templ;
-    e_xy = (realw)0.5 * (sxy + syx);
-    e_xz = (realw)0.5 * (szx + sxz);
-    e_yz = (realw)0.5 * (szy + syz);
+    e_xx = st.xx - templ;
+    e_yy = st.yy - templ;
+    e_xy = (realw)0.5 * (st.xy + st.yx);
+    e_xz = (realw)0.5 * (st.zx + st.xz);
+    e_yz = (realw)0.5 * (st.zy + st.yz);
   }
 
   const realw kappal = km.x, mul = km.y;
-  const realw lp2m = kappal + FOUR_THIRDS * mul;
-  const realw lambdal = lp2m - (realw)2 * mul;
-  realw sigma_xx = lp2m * duxdx + lambdal * (duydy + duzdz);
-  realw sigma_yy = lp2m * duydy + lambdal * (duxdx + duzdz);
-  realw sigma_zz = lp2m * duzdz + lambdal * (duxdx + duydy);
-  realw sigma_xy = mul * (duxdy + duydx);
-  realw sigma_xz = mul * (duzdx + duxdz);
-  realw sigma_yz = mul * (duzdy + duydz);
+  Sigma6 sg = iso_stress(du, kappal, mul);
 
   if (ATT) {
     const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
     const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
     const realw rtr = (Rv[5][0] + Rv[5][1]) + Rv[5][2];
-    sigma_xx = sigma_xx - rxx - rtr;
-    sigma_yy = sigma_yy - ryy - rtr;
-    sigma_zz = sigma_zz + rxx + ryy - rtr;
-    sigma_xy = sigma_xy - ((Rv[2][0] + Rv[2][1]) + Rv[2][2]);
-    sigma_xz = sigma_xz - ((Rv[3][0] + Rv[3][1]) + Rv[3][2]);
-    sigma_yz = sigma_yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
+    sg.xx = sg.xx - rxx - rtr;
+    sg.yy = sg.yy - ryy - rtr;
+    sg.zz = sg.zz + rxx + ryy - rtr;
+    sg.xy = sg.xy - ((Rv[2][0] + Rv[2][1]) + Rv[2][2]);
+    sg.xz = sg.xz - ((Rv[3][0] + Rv[3][1]) + Rv[3][2]);
+    sg.yz = sg.yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
     if (active) {
       // memory-variable update (second-order RK) with old strain Sn and new strain Snp1
       const realw Snp1[6] = {e_xx, e_yy, e_xy, e_xz, e_yz, e_tr};
@@ -495,30 +419,8 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
   }
 
   {
-    const realw syx = sigma_xy, szx = sigma_xz, szy = sigma_yz;
     realw t[9];
-    if (irr) {
-      t[0] = jacl * (sigma_xx * xixl + syx * xiyl + szx * xizl);
-      t[1] = jacl * (sigma_xy * xixl + sigma_yy * xiyl + szy * xizl);
-      t[2] = jacl * (sigma_xz * xixl + sigma_yz * xiyl + sigma_zz * xizl);
-      t[3] = jacl * (sigma_xx * etaxl + syx * etayl + szx * etazl);
-      t[4] = jacl * (sigma_xy * etaxl + sigma_yy * etayl + szy * etazl);
-      t[5] = jacl * (sigma_xz * etaxl + sigma_yz * etayl + sigma_zz * etazl);
-      t[6] = jacl * (sigma_xx * gaxl + syx * gayl + szx * gazl);
-      t[7] = jacl * (sigma_xy * gaxl + sigma_yy * gayl + szy * gazl);
-      t[8] = jacl * (sigma_xz * gaxl + sigma_yz * gayl + sigma_zz * gazl);
-    } else {
-      const realw xr = A.xix_regular;
-      t[0] = jacl * sigma_xx * xr;
-      t[1] = jacl * sigma_xy * xr;
-      t[2] = jacl * sigma_xz * xr;
-      t[3] = jacl * syx * xr;
-      t[4] = jacl * sigma_yy * xr;
-      t[5] = jacl * sigma_yz * xr;
-      t[6] = jacl * szx * xr;
-      t[7] = jacl * szy * xr;
-      t[8] = jacl * sigma_zz * xr;
-    }
+    stress_products(sg, irr != 0, J, jacl, A.xix_regular, t);
 #pragma unroll
     for (int m = 0; m < 9; m++) sT[m][p] = t[m];
   }
@@ -526,20 +428,7 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
   if (!active) return;
 
   realw f[3];
-#pragma unroll
-  for (int cc = 0; cc < 3; cc++) {
-    const realw *s1 = sT[cc], *s2 = sT[3 + cc], *s3 = sT[6 + cc];
-    realw n1 = G.hpw[i] * s1[b1];
-    realw n2 = G.hpw[j] * s2[b2];
-    realw n3 = G.hpw[k] * s3[b3];
-#pragma unroll
-    for (int l = 1; l < 5; l++) {
-      n1 = n1 + G.hpw[l * 5 + i] * s1[b1 + l];
-      n2 = n2 + G.hpw[l * 5 + j] * s2[b2 + 5 * l];
-      n3 = n3 + G.hpw[l * 5 + k] * s3[b3 + 25 * l];
-    }
-    f[cc] = G.wxy[j * 5 + k] * n1 + G.wxy[i * 5 + k] * n2 + G.wxy[i * 5 + j] * n3;
-  }
+  tile_divergence<1>(&sT[0][0], 128, G, i, j, k, f);
   if (MODE == 1) {
     A.local_out[ep] = f[0];
     A.local_out[nl + ep] = f[1];

@@ -53,6 +53,27 @@ void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, 
 void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,
                            bool exact);
 
+// batched many-shot stiffness (k_batch.hip): fields interleaved shot-fastest,
+// value (g, c, s) at ((int64_t)g * 3 + c) * S + s
+struct BatchArgs {
+  int nelem;
+  const int* elems;
+  int ngroups;             // S / shots-per-workgroup
+  int S;                   // padded shot count (row stride)
+  const int* ibool;
+  const int* irregular;
+  realw xix_regular, jacobian_regular;
+  const realw* jac;
+  int64_t jac_plane;
+  const float2* kappamu;
+  const realw* u;
+  realw* a;
+  const int* nstep;        // per shot slot, decreasing (padding slots 0)
+  const int* it;           // device step counter, 1-based current step
+};
+// sc = shots per workgroup (2, 4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour)
+void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
+
 struct AcousticArgs {
   int nelem;
   const int* elems;

@@ -150,5 +150,17 @@ def config4(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int 
     return Workload(f"config4: {n}^3 elastic adjoint, {nstep} steps", db, src, rec, nstep, dt)
 
 
+def gaussian_bump(db, center=None, sigma=None) -> np.ndarray:
+    """Smooth interior Gaussian exp(-|x - c|^2 / (2 sigma^2)) at every local point, (nspec, 125);
+    defaults: c = centre of the mesh, sigma = 1/8 of its smallest side."""
+    ib = db.ibool.reshape(db.nspec, 125)
+    x, y, z = db.xstore[ib].astype(np.float64), db.ystore[ib].astype(np.float64), db.zstore[ib].astype(np.float64)
+    lo = np.array([db.xstore.min(), db.ystore.min(), db.zstore.min()], np.float64)
+    hi = np.array([db.xstore.max(), db.ystore.max(), db.zstore.max()], np.float64)
+    c = (lo + hi) / 2 if center is None else np.asarray(center, np.float64)
+    s = (hi - lo).min() / 8 if sigma is None else float(sigma)
+    return np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (2 * s * s))
+
+
 def nglob_box(nx, ny, nz) -> int:
     return (4 * nx + 1) * (4 * ny + 1) * (4 * nz + 1)
