@@ -49,21 +49,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stiffness_bytes_per_launch(db, attenuation: bool) -> float:
+def stiffness_bytes_per_launch(db, attenuation=None) -> float:
     """Algorithmic HBM bytes of one element-stiffness launch (unique-traffic model).
 
     Per global point of the elastic domain: displ read (12 B), accel read + write (24 B),
     plus veloc read (12 B) with attenuation. Per local point of the elastic elements:
     ibool + kappa + mu (12 B); with attenuation the 18 memory variables read + written
-    (144 B), the 6 stored strains read + written (48 B) and the 6 SLS factors read (24 B).
+    (144 B), the 6 stored strains read + written (48 B) and the 6 SLS factors, read as a 2-byte
+    row index into a table of distinct rows when there are at most 65535 (else 24 B).
     Regular elements carry no Jacobian arrays; irregular ones add 40 B per local point.
     """
     import numpy as np
     el = db.ispec_is_elastic
     nloc_el = int(el.sum()) * 125
     nglob_el = int(np.unique(db.ibool[el]).size)
-    per_glob = 12 + 24 + (12 if attenuation else 0)
-    per_loc = 12 + (144 + 48 + 24 if attenuation else 0)
+    per_glob = 12 + 24 + (12 if attenuation is not None else 0)
+    per_loc = 12
+    if attenuation is not None:
+        rows = np.unique(np.concatenate([attenuation.factor_common.reshape(-1, 3),
+                                         attenuation.factor_common_kappa.reshape(-1, 3)], 1), axis=0).shape[0]
+        per_loc += 144 + 48 + (2 if rows <= 65535 else 24)
     nirr_el = int((db.irregular_element_number[el] > 0).sum()) * 125
     return float(per_glob * nglob_el + per_loc * nloc_el + 40 * nirr_el)
 
@@ -237,7 +242,7 @@ def main():
 
     nglob = db.nglob
     value = nglob * args.steps / wall
-    kbytes = stiffness_bytes_per_launch(db, w.attenuation is not None)
+    kbytes = stiffness_bytes_per_launch(db, w.attenuation)
     kavg_s = tot.value / max(cnt.value, 1) / 1e3
     achieved = kbytes / kavg_s / 1e9
     traffic, tsrc = measured_traffic(args.assembly, args.exact) if args.scale == 1.0 else (None, None)

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.h"
@@ -421,6 +422,57 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
 
 namespace sf {
 static void build_elastic_assembly(Mesh* m);
+
+// Distinct (factor_common, factor_common_kappa) rows, compared bit for bit, and a 16-bit row index
+// per local point; gives up (planes only) beyond 65535 distinct rows.
+static void build_factor_table(Mesh* m, const realw* fc, const realw* fck, int64_t nloc) {
+  struct Key {
+    uint32_t w[6];
+    bool operator==(const Key& o) const { return std::memcmp(w, o.w, sizeof(w)) == 0; }
+  };
+  struct Hash {
+    size_t operator()(const Key& k) const {
+      uint64_t h = 1469598103934665603ull;
+      for (uint32_t v : k.w) h = (h ^ v) * 1099511628211ull;
+      return (size_t)h;
+    }
+  };
+  std::unordered_map<Key, int, Hash> rows;
+  std::vector<Key> order;
+  std::vector<uint16_t> idx((size_t)nloc);
+  Key last{};
+  int last_id = -1;
+  for (int64_t q = 0; q < nloc; q++) {
+    Key k;
+    std::memcpy(k.w, fc + 3 * q, 12);
+    std::memcpy(k.w + 3, fck + 3 * q, 12);
+    if (last_id >= 0 && k == last) {
+      idx[q] = (uint16_t)last_id;
+      continue;
+    }
+    auto it = rows.find(k);
+    int id;
+    if (it == rows.end()) {
+      if (order.size() >= 65535) return;  // too many distinct rows: keep the planes
+      id = (int)order.size();
+      rows.emplace(k, id);
+      order.push_back(k);
+    } else {
+      id = it->second;
+    }
+    idx[q] = (uint16_t)id;
+    last = k;
+    last_id = id;
+  }
+  const int n = (int)order.size();
+  std::vector<realw> tab(6 * (size_t)n);
+  for (int r = 0; r < n; r++)
+    for (int c = 0; c < 6; c++) std::memcpy(&tab[(size_t)c * n + r], &order[r].w[c], 4);
+  m->fc_table = dupload(tab.data(), 6 * (int64_t)n);
+  m->fc_index = dupload(idx.data(), nloc);
+  m->fc_rows = n;
+}
+
 static void build_colors(Mesh* m, const std::vector<int>& elems, int*& d_list, int* off, int& ncolor);
 }  // namespace sf
 
@@ -527,6 +579,7 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
       }
     m->factor_common = dupload(F.data(), 3 * nloc);
     m->factor_common_kappa = dupload(Fk.data(), 3 * nloc);
+    build_factor_table(m, factor_common, factor_common_kappa, nloc);
     for (int s = 0; s < 3; s++) {
       m->alphaval[s] = alphaval[s];
       m->betaval[s] = betaval[s];
@@ -643,6 +696,17 @@ extern "C" void FC_FUNC_(prepare_fields_acoustic_device, PREPARE_FIELDS_ACOUSTIC
       }
     m->nspec_acoustic = (int)lst.size();
     m->acoustic_elems = dupload(lst.data(), (int64_t)lst.size());
+    // range of the global points of acoustic elements: outside it the potentials stay zero, so the
+    // Newmark updates skip it (a fluid layer on top of a solid is a small slice of NGLOB)
+    int gmin = m->nglob, gmax = -1;
+    for (int e : lst)
+      for (int p = 0; p < 125; p++) {
+        const int g = H->ibool[(int64_t)e * 125 + p];
+        gmin = std::min(gmin, g);
+        gmax = std::max(gmax, g);
+      }
+    m->ac_g0 = gmax < 0 ? 0 : (gmin / 4) * 4;
+    m->ac_n = gmax < 0 ? 0 : (int64_t)gmax + 1 - m->ac_g0;
     if (m->assembly == 1) {
       build_csr(m, lst, m->asm_ac_ptr, m->asm_ac_idx);
       m->asm_ac_buf = dmalloc<realw>(nloc);
@@ -810,7 +874,7 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
     dfree(m->ac[f].pot); dfree(m->ac[f].potd); dfree(m->ac[f].potdd);
     dfree(m->att[f].R); dfree(m->att[f].eps); dfree(m->eps_trace_over_3[f]);
   }
-  dfree(m->factor_common); dfree(m->factor_common_kappa);
+  dfree(m->factor_common); dfree(m->factor_common_kappa); dfree(m->fc_table); dfree(m->fc_index);
   free_face_points(m->abs_el); free_face_points(m->abs_ac); free_face_points(m->free_ac);
   free_face_points(m->coupling);
   dfree(m->b_absorb_field); dfree(m->b_absorb_potential);

@@ -29,7 +29,9 @@ static void flush_el(Mesh* m, int f) {
 }
 static void flush_ac(Mesh* m, int f) {
   if (m->pending_k3_ac[f]) {
-    launch_corrector(m->stream, m->nglob, m->ac[f].potd, m->ac[f].potdd, m->rmass_ac, m->pending_dt2_ac[f]);
+    const int64_t g0 = m->ac_g0;
+    launch_corrector(m->stream, m->ac_n, m->ac[f].potd + g0, m->ac[f].potdd + g0, m->rmass_ac + g0,
+                     m->pending_dt2_ac[f]);
     m->pending_k3_ac[f] = 0;
   }
 }
@@ -60,6 +62,9 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
   a.eps = m->att[f].eps;
   a.fc = m->factor_common;
   a.fck = m->factor_common_kappa;
+  a.fc_table = m->fc_table;
+  a.fc_index = m->fc_index;
+  a.fc_rows = m->fc_rows;
   a.eps_tr3 = m->simulation_type == 3 ? m->eps_trace_over_3[f] : nullptr;
   a.nloc = (int64_t)m->nspec * 125;
   for (int s = 0; s < 3; s++) {
@@ -130,12 +135,14 @@ extern "C" void FC_FUNC_(update_displacement_ac_cuda, UPDATE_DISPLACEMENT_AC_CUD
   const int f = field_of(*FORWARD_OR_ADJOINT);
   AcField& a = m->ac[f];
   if (!a.pot) fatal("update_displacement_ac_cuda", "acoustic field not prepared");
+  const int64_t g0 = m->ac_g0;  // only the acoustic range changes (see Mesh::ac_g0)
   if (m->pending_k3_ac[f]) {
-    launch_corrector_predictor(m->stream, m->nglob, a.pot, a.potd, a.potdd, m->rmass_ac, m->pending_dt2_ac[f],
-                               *deltat_F, *deltatover2_F, *deltatsqover2_F);
+    launch_corrector_predictor(m->stream, m->ac_n, a.pot + g0, a.potd + g0, a.potdd + g0, m->rmass_ac + g0,
+                               m->pending_dt2_ac[f], *deltat_F, *deltatover2_F, *deltatsqover2_F);
     m->pending_k3_ac[f] = 0;
   } else {
-    launch_predictor(m->stream, m->nglob, a.pot, a.potd, a.potdd, *deltat_F, *deltatover2_F, *deltatsqover2_F);
+    launch_predictor(m->stream, m->ac_n, a.pot + g0, a.potd + g0, a.potdd + g0, *deltat_F, *deltatover2_F,
+                     *deltatsqover2_F);
   }
 }
 

@@ -6,7 +6,10 @@
 // (-ffp-contract=off) reproduces the CPU solver bit for bit whichever kernel runs it.
 //
 // Element tiles live in LDS with the value of point q = i + 5 j + 25 k at base[q * PS]: PS = 1 for
-// one field per workgroup, PS = shots per workgroup for shot-interleaved tiles.
+// one field per workgroup, PS = shots per workgroup for shot-interleaved tiles. The GLL tables
+// (hp[i*5+l] = hprime, hpw[i*5+l] = hprimewgll, wxy[a*5+b] = wgllwgll) are read through pointers:
+// the kernels stage them in LDS so a lane reads each coefficient when it uses it instead of
+// holding its 33 coefficients in registers.
 #pragma once
 
 #include "kernels.h"
@@ -24,17 +27,17 @@ struct JacPt {  // inverse-Jacobian terms and determinant at one point (irregula
 // tempx1 = sum_l hprime(i,l) u(l,j,k), tempx2 = sum_l hprime(j,l) u(i,l,k),
 // tempx3 = sum_l hprime(k,l) u(i,j,l), summed left to right.
 template <int PS>
-__device__ __forceinline__ Grad9 tile_gradient(const realw* sx, const realw* sy, const realw* sz, const GLLk& G,
+__device__ __forceinline__ Grad9 tile_gradient(const realw* sx, const realw* sy, const realw* sz, const realw* hp,
                                                int i, int j, int k) {
   const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
   Grad9 t;
-  const realw h1 = G.hp[i * 5], h2 = G.hp[j * 5], h3 = G.hp[k * 5];
+  const realw h1 = hp[i * 5], h2 = hp[j * 5], h3 = hp[k * 5];
   t.x1 = h1 * sx[b1 * PS]; t.y1 = h1 * sy[b1 * PS]; t.z1 = h1 * sz[b1 * PS];
   t.x2 = h2 * sx[b2 * PS]; t.y2 = h2 * sy[b2 * PS]; t.z2 = h2 * sz[b2 * PS];
   t.x3 = h3 * sx[b3 * PS]; t.y3 = h3 * sy[b3 * PS]; t.z3 = h3 * sz[b3 * PS];
 #pragma unroll
   for (int l = 1; l < 5; l++) {
-    const realw a1 = G.hp[i * 5 + l], a2 = G.hp[j * 5 + l], a3 = G.hp[k * 5 + l];
+    const realw a1 = hp[i * 5 + l], a2 = hp[j * 5 + l], a3 = hp[k * 5 + l];
     const int q1 = (b1 + l) * PS, q2 = (b2 + 5 * l) * PS, q3 = (b3 + 25 * l) * PS;
     t.x1 = t.x1 + a1 * sx[q1]; t.y1 = t.y1 + a1 * sy[q1]; t.z1 = t.z1 + a1 * sz[q1];
     t.x2 = t.x2 + a2 * sx[q2]; t.y2 = t.y2 + a2 * sy[q2]; t.z2 = t.z2 + a2 * sz[q2];
@@ -47,11 +50,11 @@ __device__ __forceinline__ Grad9 tile_gradient(const realw* sx, const realw* sy,
 // is continued with the five terms of the dt*v tile.
 template <int PS>
 __device__ __forceinline__ Grad9 tile_gradient_continue(Grad9 t, const realw* wx, const realw* wy, const realw* wz,
-                                                        const GLLk& G, int i, int j, int k) {
+                                                        const realw* hp, int i, int j, int k) {
   const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
 #pragma unroll
   for (int l = 0; l < 5; l++) {
-    const realw h1 = G.hp[i * 5 + l], h2 = G.hp[j * 5 + l], h3 = G.hp[k * 5 + l];
+    const realw h1 = hp[i * 5 + l], h2 = hp[j * 5 + l], h3 = hp[k * 5 + l];
     const int q1 = (b1 + l) * PS, q2 = (b2 + 5 * l) * PS, q3 = (b3 + 25 * l) * PS;
     t.x1 = t.x1 + wx[q1] * h1; t.y1 = t.y1 + wy[q1] * h1; t.z1 = t.z1 + wz[q1] * h1;
     t.x2 = t.x2 + wx[q2] * h2; t.y2 = t.y2 + wy[q2] * h2; t.z2 = t.z2 + wz[q2] * h2;
@@ -134,23 +137,30 @@ __device__ __forceinline__ void stress_products(const Sigma6& s, bool irr, const
 // f_c = wgllwgll_yz(j,k) sum_l hprimewgll(l,i) t1_c(l,j,k) + wgllwgll_xz(i,k) sum_l hprimewgll(l,j) t2_c(i,l,k)
 //     + wgllwgll_xy(i,j) sum_l hprimewgll(l,k) t3_c(i,j,l); plane m of the tile at sT + m * plane.
 template <int PS>
-__device__ __forceinline__ void tile_divergence(const realw* sT, int plane, const GLLk& G, int i, int j, int k,
-                                                realw f[3]) {
+__device__ __forceinline__ void tile_divergence(const realw* sT, int plane, const realw* hpw, const realw* wxy, int i,
+                                                int j, int k, realw f[3]) {
   const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
 #pragma unroll
   for (int cc = 0; cc < 3; cc++) {
     const realw *s1 = sT + cc * plane, *s2 = sT + (3 + cc) * plane, *s3 = sT + (6 + cc) * plane;
-    realw n1 = G.hpw[i] * s1[b1 * PS];
-    realw n2 = G.hpw[j] * s2[b2 * PS];
-    realw n3 = G.hpw[k] * s3[b3 * PS];
+    realw n1 = hpw[i] * s1[b1 * PS];
+    realw n2 = hpw[j] * s2[b2 * PS];
+    realw n3 = hpw[k] * s3[b3 * PS];
 #pragma unroll
     for (int l = 1; l < 5; l++) {
-      n1 = n1 + G.hpw[l * 5 + i] * s1[(b1 + l) * PS];
-      n2 = n2 + G.hpw[l * 5 + j] * s2[(b2 + 5 * l) * PS];
-      n3 = n3 + G.hpw[l * 5 + k] * s3[(b3 + 25 * l) * PS];
+      n1 = n1 + hpw[l * 5 + i] * s1[(b1 + l) * PS];
+      n2 = n2 + hpw[l * 5 + j] * s2[(b2 + 5 * l) * PS];
+      n3 = n3 + hpw[l * 5 + k] * s3[(b3 + 25 * l) * PS];
     }
-    f[cc] = G.wxy[j * 5 + k] * n1 + G.wxy[i * 5 + k] * n2 + G.wxy[i * 5 + j] * n3;
+    f[cc] = wxy[j * 5 + k] * n1 + wxy[i * 5 + k] * n2 + wxy[i * 5 + j] * n3;
   }
+}
+
+// stage the 75 GLL coefficients of G in LDS (lanes 0..74 of the workgroup; needs a barrier before use)
+__device__ __forceinline__ void stage_gll(realw* sG, const GLLk& G, int lane) {
+  if (lane < 25) sG[lane] = G.hp[lane];
+  else if (lane < 50) sG[lane] = G.hpw[lane - 25];
+  else if (lane < 75) sG[lane] = G.wxy[lane - 50];
 }
 
 __device__ __forceinline__ JacPt load_jac(const realw* J, int64_t P, int64_t q) {

@@ -43,6 +43,8 @@ __global__ void __launch_bounds__(256) k_elastic_batch(BatchArgs B, GLLk G) {
   constexpr int PL = 125 * SC;                    // one plane of the tile
   __shared__ realw sU[3 * PL];                    // displacement tile [c][p][s]
   __shared__ realw sT[9 * PL];                    // stress products   [m][p][s]
+  __shared__ realw sG[75];                        // hprime, hprimewgll, wgllwgll
+  stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int b = xcd_remap_b(blockIdx.x, gridDim.x);
   const int ie = b / B.ngroups;
@@ -76,7 +78,7 @@ __global__ void __launch_bounds__(256) k_elastic_batch(BatchArgs B, GLLk G) {
     const int p = pl + ps * NP;
     if (p < 125) {
       const int k = p / 25, j = (p / 5) % 5, i = p % 5;
-      const Grad9 tg = tile_gradient<SC>(sU + sl, sU + PL + sl, sU + 2 * PL + sl, G, i, j, k);
+      const Grad9 tg = tile_gradient<SC>(sU + sl, sU + PL + sl, sU + 2 * PL + sl, sG, i, j, k);
       JacPt J = {};
       realw jacl = B.jacobian_regular;
       if (irr) {
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(256) k_elastic_batch(BatchArgs B, GLLk G) {
     if (p < 125) {
       const int k = p / 25, j = (p / 5) % 5, i = p % 5;
       realw f[3];
-      tile_divergence<SC>(sT + sl, PL, G, i, j, k, f);
+      tile_divergence<SC>(sT + sl, PL, sG + 25, sG + 50, i, j, k, f);
       realw* a = B.a + (int64_t)gq[ps] * 3 * S + s;
       if (MODE == 2) {
         a[0] = a[0] + (-f[0]);

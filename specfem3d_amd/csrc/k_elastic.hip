@@ -323,8 +323,10 @@ template <bool STRAIN, bool ATT, int MODE>
 __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
   __shared__ realw sU[6][128];  // u (0..2) and dt*v (3..5)
   __shared__ realw sT[9][128];  // stress products t1 (0..2), t2 (3..5), t3 (6..8)
+  __shared__ realw sG[75];      // hprime, hprimewgll, wgllwgll
   const int p = threadIdx.x;
   const bool active = p < 125;
+  stage_gll(sG, G, p);
   const int ie = xcd_remap(blockIdx.x, gridDim.x);  // grid = one workgroup per listed element
   const int e = A.elems[ie];
   const int q = active ? p : 124;
@@ -357,22 +359,31 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
       for (int s = 0; s < 3; s++) Rv[cc][s] = A.R[(cc * 3 + s) * nl + ep];
 #pragma unroll
     for (int cc = 0; cc < 6; cc++) Sn[cc] = A.eps[cc * nl + ep];
+    if (A.fc_index) {
+      const int r = A.fc_index[ep];
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
-      fcv[s] = A.fc[s * nl + ep];
-      fckv[s] = A.fck[s * nl + ep];
+      for (int s = 0; s < 3; s++) {
+        fcv[s] = A.fc_table[s * A.fc_rows + r];
+        fckv[s] = A.fc_table[(3 + s) * A.fc_rows + r];
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        fcv[s] = A.fc[s * nl + ep];
+        fckv[s] = A.fck[s * nl + ep];
+      }
     }
   }
   __syncthreads();
 
   const realw ONE_THIRD = (realw)1 / (realw)3;
-  const Grad9 tg = tile_gradient<1>(sU[0], sU[1], sU[2], G, i, j, k);
+  const Grad9 tg = tile_gradient<1>(sU[0], sU[1], sU[2], sG, i, j, k);
   const Du9 du = map_gradient(tg, irr != 0, J, A.xix_regular);
 
   realw e_xx = 0, e_yy = 0, e_xy = 0, e_xz = 0, e_yz = 0, e_tr = 0;
   if (STRAIN) {
     Du9 st = du;
-    if (ATT) st = map_gradient(tile_gradient_continue<1>(tg, sU[3], sU[4], sU[5], G, i, j, k), irr != 0, J,
+    if (ATT) st = map_gradient(tile_gradient_continue<1>(tg, sU[3], sU[4], sU[5], sG, i, j, k), irr != 0, J,
                                A.xix_regular);
     e_tr = (st.xx + st.yy + st.zz);
     const realw templ = ONE_THIRD * (st.xx + st.yy + st.zz);
@@ -428,7 +439,7 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
   if (!active) return;
 
   realw f[3];
-  tile_divergence<1>(&sT[0][0], 128, G, i, j, k, f);
+  tile_divergence<1>(&sT[0][0], 128, sG + 25, sG + 50, i, j, k, f);
   if (MODE == 1) {
     A.local_out[ep] = f[0];
     A.local_out[nl + ep] = f[1];

@@ -39,6 +39,9 @@ struct ElasticArgs {
   realw* eps;              // 6 planes of nloc
   const realw* fc;         // 3 planes
   const realw* fck;        // 3 planes
+  const realw* fc_table;   // or: 6 planes of fc_rows distinct factor rows (fc 0..2, fck 3..5) ...
+  const uint16_t* fc_index;  // ... and a row index per local point (null: use the planes)
+  int fc_rows;
   realw* eps_tr3;          // nloc or null
   int64_t nloc;            // nspec*125
   realw alpha[3], beta[3], gamma[3];

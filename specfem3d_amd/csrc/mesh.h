@@ -88,6 +88,8 @@ struct Mesh {
   realw* kappa_ac = nullptr;       // kappastore for acoustic elements
   realw* rmass = nullptr;          // 3*nglob inverse mass (elastic)
   realw* rmass_ac = nullptr;       // nglob inverse mass (acoustic)
+  int64_t ac_g0 = 0, ac_n = 0;     // acoustic points lie in [ac_g0, ac_g0 + ac_n) (ac_g0 a multiple of 4);
+                                   // the acoustic Newmark streams cover only this range
 
   // wavefields: [0] forward/adjoint, [1] backward/reconstructed
   Wavefield el[2];
@@ -97,6 +99,12 @@ struct Mesh {
   AttField att[2];
   realw* factor_common = nullptr;  // 3 planes (mu)
   realw* factor_common_kappa = nullptr;  // 3 planes
+  // the same factors as a table of distinct (factor_common[3], factor_common_kappa[3]) rows and a
+  // 16-bit row index per local point (when at most 65535 distinct rows exist; the factors depend
+  // only on the point's Q values, which are few): 2 B per point instead of 24 B in the kernel
+  realw* fc_table = nullptr;       // 6 planes of fc_rows
+  uint16_t* fc_index = nullptr;    // nloc
+  int fc_rows = 0;
   realw alphaval[N_SLS], betaval[N_SLS], gammaval[N_SLS];
   realw b_alphaval[N_SLS], b_betaval[N_SLS], b_gammaval[N_SLS];
   realw* eps_trace_over_3[2] = {nullptr, nullptr};
