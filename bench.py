@@ -36,7 +36,7 @@ def measured_traffic(assembly: int, exact: int):
     """
     if exact:
         return None, None
-    name = os.path.join("profiles", f"k_elastic_config2_{['atomic', 'ordered', 'coloured'][assembly]}.md")
+    name = os.path.join("profiles", f"k_elastic_pt_config2_{['atomic', 'ordered', 'coloured'][assembly]}.md")
     try:
         txt = open(os.path.join(HERE, name)).read()
     except OSError:
@@ -95,6 +95,8 @@ def batched_measurement(args) -> dict:
     from specfem3d_amd.batch import BatchSolver
     from specfem3d_amd.solver import GPUSolver
     w = W.config3(scale=args.batch_scale, nshots=args.batch_shots)
+    if args.batch_max_steps > 0:
+        w.shots = [(s, r, min(n, args.batch_max_steps)) for s, r, n in w.shots]
     db = w.db
     nsteps = [n for _, _, n in w.shots]
     log(f"[bench] batched workload {w.name}: NGLOB {db.nglob}, record lengths {min(nsteps)}-{max(nsteps)}")
@@ -116,15 +118,16 @@ def batched_measurement(args) -> dict:
     # plain single-shot path on the same mesh (first shot), timed the same way
     src, rec, n = w.shots[0]
     ps = min(args.plain_steps, n)
+    pw = min(10, ps // 2)
     g = GPUSolver(db, src, rec, nstep=ps, exact=False, assembly=0)
-    for _ in range(10):
+    for _ in range(pw):
         g.step()
     g.synchronize()
     tp = time.perf_counter()
-    for _ in range(ps - 10):
+    for _ in range(ps - pw):
         g.step()
     g.synchronize()
-    plain = db.nglob * (ps - 10) / (time.perf_counter() - tp)
+    plain = db.nglob * (ps - pw) / (time.perf_counter() - tp)
     g.close()
     log(f"[bench] batched {value:.3e} GLL-pt updates/s in {wall:.2f}s (stiffness {kms:.3f} ms/step, step {sms:.3f} ms "
         f"with all shots live), plain single shot {plain:.3e}, ratio {value / plain:.2f}, finite={finite}")
@@ -183,6 +186,8 @@ def main():
     ap.add_argument("--batch-shots", type=int, default=32, help="shots of the batched 64x64x32 workload (0: skip)")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="shrink the batched mesh (tests only)")
     ap.add_argument("--plain-steps", type=int, default=300, help="steps of the single-shot run on the batched mesh")
+    ap.add_argument("--batch-max-steps", type=int, default=0,
+                    help="clip every shot's record length to this many steps (profiling runs; 0: full records)")
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the checkpointed kernel run on the 48^3 adjoint workload (0: skip)")
     ap.add_argument("--adjoint-scale", type=float, default=1.0, help="shrink the adjoint mesh (tests only)")
@@ -248,7 +253,7 @@ def main():
     traffic, tsrc = measured_traffic(args.assembly, args.exact) if args.scale == 1.0 else (None, None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_elastic (viscoelastic 3-SLS stiffness)", "bytes_per_launch": kbytes,
+                "kernel": "k_elastic_pt (viscoelastic 3-SLS stiffness)", "bytes_per_launch": kbytes,
                 "avg_launch_ms": round(kavg_s * 1e3, 4), "traffic_source": tsrc}
 
     cpu = None

@@ -246,7 +246,8 @@ void unregister_host_array_(realw* h_array);
    colouring with a plain read-modify-write scatter (deterministic, no atomics). */
 void sf_set_options(long* Mesh_pointer, int exact, int assembly);
 /* tuning switch by name: "elastic_shape" 0 = column-per-lane stiffness kernel (five elements per
-   workgroup), 1 = point-per-lane (one element per workgroup, the default); unknown names are fatal */
+   workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0/1/2 =
+   scheduling of the attenuation state in the point kernel (same results); unknown names are fatal */
 void sf_set_tuning(long* Mesh_pointer, const char* name, int value);
 /* device-resident checkpoints for checkpointed kernel runs (the files of
    save_forward_arrays_undoatt / read_forward_arrays_undoatt, save_forward_arrays.f90:102-179 and

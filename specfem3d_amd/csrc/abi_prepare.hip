@@ -936,13 +936,17 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
 }
 
 // Tuning switches read by the step calls: "elastic_shape" (0 column-per-lane stiffness kernel,
-// 1 point-per-lane, the default). Unknown names are a fatal error.
+// 1 point-per-lane, the default), "att_variant" (0, 1, 2: attenuation-state scheduling of the
+// point kernel, same results). Unknown names are a fatal error.
 extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
   if (std::strcmp(name, "elastic_shape") == 0) {
     if (value < 0 || value > 1) fatal("sf_set_tuning", "elastic_shape must be 0 or 1");
     m->elastic_shape = value;
+  } else if (std::strcmp(name, "att_variant") == 0) {
+    if (value < 0 || value > 2) fatal("sf_set_tuning", "att_variant must be 0, 1 or 2");
+    m->att_variant = value;
   } else {
     fatal("sf_set_tuning", "unknown tuning switch");
   }

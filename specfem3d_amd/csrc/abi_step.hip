@@ -86,10 +86,10 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
     for (int c = 0; c < m->el_ncolor; c++) {
       a.elems = m->el_color_elems + m->el_color_off[c];
       a.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
-      launch_elastic(m->stream, a, strain, att, m->exact != 0, 2, m->elastic_shape);
+      launch_elastic(m->stream, a, strain, att, m->exact != 0, 2, m->elastic_shape, m->att_variant);
     }
   } else {
-    launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->elastic_shape);
+    launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->elastic_shape, m->att_variant);
   }
   if (timed) {
     SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count + 1], m->stream));

@@ -75,24 +75,46 @@ static __device__ __forceinline__ int load_it(const int* it) {
 
 // Newmark update of every live (row, shot): predictor only on the first step, otherwise the
 // deferred corrector of the previous step fused with the predictor (k_update.hip arithmetic).
-__global__ void k_batch_update(int64_t nrow, int S, const int* __restrict__ nstep, const int* __restrict__ itp,
-                               realw* __restrict__ u, realw* __restrict__ v, realw* __restrict__ a,
-                               const realw* __restrict__ rm, realw dt2p, realw dt, realw dt2, realw dtsq2, int first) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t row = t / S;
-  const int s = (int)(t - row * S);
+// A lane takes four consecutive shots of one row (a float4); a workgroup takes 256 / (S/4) rows,
+// so the row and quad of a lane come from 32-bit arithmetic on threadIdx. Finished shots keep
+// their values (their lanes store what they loaded).
+__global__ void __launch_bounds__(256) k_batch_update(int64_t nrow, int S, int rows_per_block,
+                                                      const int* __restrict__ nstep, const int* __restrict__ itp,
+                                                      realw* __restrict__ u, realw* __restrict__ v,
+                                                      realw* __restrict__ a, const realw* __restrict__ rm, realw dt2p,
+                                                      realw dt, realw dt2, realw dtsq2, int first) {
+  const int nq = S >> 2;
+  const int rl = (int)threadIdx.x / nq;
+  const int qd = (int)threadIdx.x - rl * nq;
+  if (rl >= rows_per_block) return;
+  const int64_t row = (int64_t)blockIdx.x * rows_per_block + rl;
   if (row >= nrow) return;
-  if (nstep[s] < load_it(itp)) return;
-  realw U = u[t], V = v[t], A = a[t];
-  if (!first) {
-    A = A * rm[row];
-    V = V + dt2p * A;
-  }
-  U = U + dt * V + dtsq2 * A;
-  V = V + dt2 * A;
-  u[t] = U;
-  v[t] = V;
-  a[t] = 0.f;
+  const int it = load_it(itp);
+  const int s = 4 * qd;
+  if (nstep[s] < it) return;                       // shots are sorted: the whole quad has finished
+  const bool l1 = nstep[s + 1] >= it, l2 = nstep[s + 2] >= it, l3 = nstep[s + 3] >= it;
+  const int64_t o = row * S + s;
+  const float4 U0 = *reinterpret_cast<const float4*>(u + o);
+  const float4 V0 = *reinterpret_cast<const float4*>(v + o);
+  const float4 A0 = *reinterpret_cast<const float4*>(a + o);
+  const realw M = first ? (realw)0 : rm[row];
+  float4 U = U0, V = V0, A = A0;
+#define SF_BU(c)                           \
+  if (!first) {                            \
+    A.c = A.c * M;                         \
+    V.c = V.c + dt2p * A.c;                \
+  }                                        \
+  U.c = U.c + dt * V.c + dtsq2 * A.c;      \
+  V.c = V.c + dt2 * A.c;                   \
+  A.c = 0.f;
+  SF_BU(x) SF_BU(y) SF_BU(z) SF_BU(w)
+#undef SF_BU
+  if (!l1) { U.y = U0.y; V.y = V0.y; A.y = A0.y; }
+  if (!l2) { U.z = U0.z; V.z = V0.z; A.z = A0.z; }
+  if (!l3) { U.w = U0.w; V.w = V0.w; A.w = A0.w; }
+  *reinterpret_cast<float4*>(u + o) = U;
+  *reinterpret_cast<float4*>(v + o) = V;
+  *reinterpret_cast<float4*>(a + o) = A;
 }
 
 // Stacey (k_stacey_el arithmetic) for every (face point of one round, live shot)
@@ -100,9 +122,9 @@ __global__ void k_batch_stacey(int n, int S, const int* __restrict__ ig, const r
                                const realw* __restrict__ jw, const realw* __restrict__ rvp,
                                const realw* __restrict__ rvs, const int* __restrict__ nstep,
                                const int* __restrict__ itp, const realw* __restrict__ veloc, realw* accel) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = (int)(t / S);
-  const int s = (int)(t - (int64_t)q * S);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // n * S < 2^31 (checked by the caller)
+  const int q = t / S;
+  const int s = t - q * S;
   if (q >= n) return;
   if (nstep[s] < load_it(itp)) return;
   const int64_t b = (int64_t)ig[q] * 3 * S + s;
@@ -258,14 +280,16 @@ static void enqueue_step(Batch* b, hipStream_t st, bool first) {
   Mesh* m = b->m;
   const int64_t nrow = 3 * (int64_t)m->nglob;
   const realw dt = b->dt, dt2 = b->dt2, dtsq2 = b->dtsq2;
-  hipLaunchKernelGGL(k_batch_update, sf_grid(nrow * b->S, 256), dim3(256), 0, st, nrow, b->S, b->d_nstep, b->d_it,
-                     b->u, b->v, b->a, m->rmass, dt2, dt, dt2, dtsq2, first ? 1 : 0);
+  const int rpb = 256 / (b->S / 4);
+  hipLaunchKernelGGL(k_batch_update, sf_grid((nrow + rpb - 1) / rpb, 1), dim3(256), 0, st, nrow, b->S, rpb,
+                     b->d_nstep, b->d_it, b->u, b->v, b->a, m->rmass, dt2, dt, dt2, dtsq2, first ? 1 : 0);
   SF_LAUNCH_CHECK("k_batch_update");
   if (m->stacey) {
     const FacePoints& f = m->abs_el;
     for (int r = 0; r < f.nrounds; r++) {
       const int o = f.round_off[r], cnt = f.round_off[r + 1] - o;
       if (cnt <= 0) continue;
+      if ((int64_t)cnt * b->S >= 0x7fffffff) fatal("sf_batch", "too many Stacey points x shots");
       hipLaunchKernelGGL(k_batch_stacey, sf_grid((int64_t)cnt * b->S, 256), dim3(256), 0, st, cnt, b->S,
                          f.iglob + o, f.normal + 3 * o, f.jacw + o, f.imp_p + o, f.imp_s + o, b->d_nstep, b->d_it,
                          b->v, b->a);
@@ -347,6 +371,7 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   b->nshots = nshots;
   ensure_element_colors(m);
   b->S = (nshots + 7) / 8 * 8;
+  if (b->S > 1024) fatal("sf_batch_create", "at most 1024 shots per batch");
   b->nstep_user.assign(nstep, nstep + nshots);
   for (int i = 0; i < nshots; i++)
     if (nstep[i] <= 0) fatal("sf_batch_create", "record lengths must be positive");
@@ -473,11 +498,11 @@ extern "C" void sf_batch_finalize(long h) {
   zero_state(b);
 }
 
-// shots per workgroup (2, 4 or 8; must divide the padded shot count) and assembly (0 atomic, 2 coloured)
+// shots per workgroup (4 or 8; divides the padded shot count) and assembly (0 atomic, 2 coloured)
 extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) {
   Batch* b = batch_of(h);
-  if (shots_per_group != 2 && shots_per_group != 4 && shots_per_group != 8)
-    fatal("sf_batch_set_options", "shots per workgroup must be 2, 4 or 8");
+  if (shots_per_group != 4 && shots_per_group != 8)
+    fatal("sf_batch_set_options", "shots per workgroup must be 4 or 8");
   if (assembly != 0 && assembly != 2) fatal("sf_batch_set_options", "assembly must be 0 (atomic) or 2 (coloured)");
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   drop_graph(b);
@@ -582,7 +607,7 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
   double best = 1e300;
   int best_sc = b->sc, best_mode = b->mode;
   for (int mode : {2, 0})
-    for (int sc : {2, 4, 8}) {
+    for (int sc : {4, 8}) {
       if (b->S % sc) continue;
       b->sc = sc;
       b->mode = mode;

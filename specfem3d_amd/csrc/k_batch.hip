@@ -2,11 +2,9 @@
 //
 // Many independent shots share one mesh. Their wavefields are interleaved shot-fastest: the value
 // (global point g, component c, shot s) lives at ((int64_t)g * 3 + c) * S + s, S = padded shot
-// count. A workgroup of 256 lanes takes one element and a group of SC shots: lane = (point slot,
-// shot) with the shot fastest, so every gather/scatter of a point row is SC consecutive floats, and
-// the element's ibool, moduli and Jacobian terms are loaded once for the SC shots (the lanes of one
-// point read the same address). Workgroups of one element's shot groups are adjacent in the grid,
-// so the groups share the L2 lines of each row. The per-point arithmetic is elastic_point.h, the
+// count. A workgroup takes one element and a group of SC shots (SC = 4 or 8): the element's ibool,
+// moduli and Jacobian terms are loaded once for the SC shots, and workgroups of one element's shot
+// groups are adjacent in the grid, so the groups share the L2 lines of each row. The per-point arithmetic is elastic_point.h, the
 // same code as the single-shot kernel, so a batched shot reproduces its single-shot run.
 //
 // Shots are stored in decreasing order of record length; a shot is live at step `it` while
@@ -36,13 +34,16 @@ __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-template <int SC, int MODE>
-__global__ void __launch_bounds__(256) k_elastic_batch(BatchArgs B, GLLk G) {
-  constexpr int NP = 256 / SC;                    // point slots per pass
-  constexpr int NPASS = (125 + NP - 1) / NP;
+// NQ quads of 4 shots per workgroup (SC = 4 NQ shots): lane = (point slot p in 0..127, quad q),
+// threads = 128 NQ. Each lane carries its point for 4 consecutive shots: the row segment of a
+// point is one 16-byte load/store per component, the element tile reads are 16-byte LDS reads
+// (the 4 shots of one point are adjacent), and each GLL coefficient read serves 4 shots.
+template <int NQ, int MODE>
+__global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G) {
+  constexpr int SC = 4 * NQ;
   constexpr int PL = 125 * SC;                    // one plane of the tile
-  __shared__ realw sU[3 * PL];                    // displacement tile [c][p][s]
-  __shared__ realw sT[9 * PL];                    // stress products   [m][p][s]
+  __shared__ __attribute__((aligned(16))) realw sU[3 * PL];  // displacement tile [c][p][s]
+  __shared__ __attribute__((aligned(16))) realw sT[9 * PL];  // stress products   [m][p][s]
   __shared__ realw sG[75];                        // hprime, hprimewgll, wgllwgll
   stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -50,71 +51,79 @@ __global__ void __launch_bounds__(256) k_elastic_batch(BatchArgs B, GLLk G) {
   const int ie = b / B.ngroups;
   const int s0 = (b - ie * B.ngroups) * SC;
   if (B.nstep[s0] < it) return;                   // every shot of this group has finished
-  const int sl = threadIdx.x % SC, pl = threadIdx.x / SC;
-  const int s = s0 + sl;
-  const bool live = B.nstep[s] >= it;
+  const int q = threadIdx.x % NQ, p = threadIdx.x / NQ;
+  const int sq = s0 + 4 * q;                      // first shot of this lane's quad
+  const bool live0 = B.nstep[sq] >= it;           // shots sorted: later ones finish first
+  bool live[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) live[r] = B.nstep[sq + r] >= it;
+  const bool active = p < 125;
+  const int pp = active ? p : 124;
   const int e = B.elems[ie];
   const int irr = B.irregular[e];
   const int64_t S = B.S;
-
-  int gq[NPASS];
-#pragma unroll
-  for (int ps = 0; ps < NPASS; ps++) {
-    const int p = pl + ps * NP;
-    gq[ps] = 0;
-    if (p < 125) {
-      const int g = B.ibool[(int64_t)e * 125 + p];
-      gq[ps] = g;
-      const int64_t row = (int64_t)g * 3 * S + s;
-      sU[(0 * 125 + p) * SC + sl] = live ? B.u[row] : (realw)0;
-      sU[(1 * 125 + p) * SC + sl] = live ? B.u[row + S] : (realw)0;
-      sU[(2 * 125 + p) * SC + sl] = live ? B.u[row + 2 * S] : (realw)0;
+  const int g = B.ibool[(int64_t)e * 125 + pp];
+  const int64_t row = (int64_t)g * 3 * S + sq;
+  const int ts = 4 * q;                           // tile slot of the quad
+  {
+    float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;
+    if (live0) {
+      u0 = *reinterpret_cast<const float4*>(B.u + row);
+      u1 = *reinterpret_cast<const float4*>(B.u + row + S);
+      u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
+    }
+    if (active) {
+      *reinterpret_cast<float4*>(sU + (0 * 125 + p) * SC + ts) = u0;
+      *reinterpret_cast<float4*>(sU + (1 * 125 + p) * SC + ts) = u1;
+      *reinterpret_cast<float4*>(sU + (2 * 125 + p) * SC + ts) = u2;
     }
   }
+  JacPt J = {};
+  realw jacl = B.jacobian_regular;
+  if (irr) {
+    J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
+    jacl = J.jac;
+  }
+  const float2 km = B.kappamu[(int64_t)e * 125 + pp];
   __syncthreads();
 
+  const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
+  realw t[4][9];
 #pragma unroll
-  for (int ps = 0; ps < NPASS; ps++) {
-    const int p = pl + ps * NP;
-    if (p < 125) {
-      const int k = p / 25, j = (p / 5) % 5, i = p % 5;
-      const Grad9 tg = tile_gradient<SC>(sU + sl, sU + PL + sl, sU + 2 * PL + sl, sG, i, j, k);
-      JacPt J = {};
-      realw jacl = B.jacobian_regular;
-      if (irr) {
-        J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + p);
-        jacl = J.jac;
-      }
-      const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
-      const float2 km = B.kappamu[(int64_t)e * 125 + p];
-      const Sigma6 sg = iso_stress(du, km.x, km.y);
-      realw t[9];
-      stress_products(sg, irr != 0, J, jacl, B.xix_regular, t);
+  for (int r = 0; r < 4; r++) {
+    const Grad9 tg = tile_gradient<SC>(sU + ts + r, sU + PL + ts + r, sU + 2 * PL + ts + r, sG, i, j, k);
+    const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
+    const Sigma6 sg = iso_stress(du, km.x, km.y);
+    stress_products(sg, irr != 0, J, jacl, B.xix_regular, t[r]);
+  }
+  if (active) {
 #pragma unroll
-      for (int m = 0; m < 9; m++) sT[(m * 125 + p) * SC + sl] = t[m];
-    }
+    for (int m = 0; m < 9; m++)
+      *reinterpret_cast<float4*>(sT + (m * 125 + p) * SC + ts) = make_float4(t[0][m], t[1][m], t[2][m], t[3][m]);
   }
   __syncthreads();
-  if (!live) return;
+  if (!active || !live0) return;
 
+  realw f[4][3];
 #pragma unroll
-  for (int ps = 0; ps < NPASS; ps++) {
-    const int p = pl + ps * NP;
-    if (p < 125) {
-      const int k = p / 25, j = (p / 5) % 5, i = p % 5;
-      realw f[3];
-      tile_divergence<SC>(sT + sl, PL, sG + 25, sG + 50, i, j, k, f);
-      realw* a = B.a + (int64_t)gq[ps] * 3 * S + s;
-      if (MODE == 2) {
-        a[0] = a[0] + (-f[0]);
-        a[S] = a[S] + (-f[1]);
-        a[2 * S] = a[2 * S] + (-f[2]);
-      } else {
-        atomicAdd(a, -f[0]);
-        atomicAdd(a + S, -f[1]);
-        atomicAdd(a + 2 * S, -f[2]);
-      }
+  for (int r = 0; r < 4; r++) tile_divergence<SC>(sT + ts + r, PL, sG + 25, sG + 50, i, j, k, f[r]);
+  realw* a = B.a + row;
+  if (MODE == 2) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float4 A = *reinterpret_cast<const float4*>(a + c * S);
+      if (live[0]) A.x = A.x + (-f[0][c]);
+      if (live[1]) A.y = A.y + (-f[1][c]);
+      if (live[2]) A.z = A.z + (-f[2][c]);
+      if (live[3]) A.w = A.w + (-f[3][c]);
+      *reinterpret_cast<float4*>(a + c * S) = A;
     }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (live[r]) atomicAdd(a + c * S + r, -f[r][c]);
   }
 }
 
@@ -125,14 +134,13 @@ void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int m
   if (a.nelem <= 0 || a.ngroups <= 0) return;
   const int64_t nb = (int64_t)a.nelem * a.ngroups;
   if (nb > 0x7fffffff) fatal("launch_elastic_batch", "grid too large");
-  dim3 grid((unsigned)nb), block(256);
+  dim3 grid((unsigned)nb);
   const GLLk G = g_gll_host;
-#define SF_KB(C, M) hipLaunchKernelGGL((k_elastic_batch<C, M>), grid, block, 0, st, a, G)
+#define SF_KB(NQ, M) hipLaunchKernelGGL((k_elastic_batch<NQ, M>), grid, dim3(128 * NQ), 0, st, a, G)
 #define SF_KBM(M)                  \
-  if (sc == 2) SF_KB(2, M);        \
-  else if (sc == 4) SF_KB(4, M);   \
-  else if (sc == 8) SF_KB(8, M);   \
-  else fatal("launch_elastic_batch", "shots per workgroup must be 2, 4 or 8");
+  if (sc == 4) SF_KB(1, M);        \
+  else if (sc == 8) SF_KB(2, M);   \
+  else fatal("launch_elastic_batch", "shots per workgroup must be 4 or 8");
   if (mode == 2) {
     SF_KBM(2)
   } else {

@@ -313,14 +313,41 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
   }
 }
 
+// stored strain (6 planes) and SLS factors (table row or planes) of one point
+__device__ __forceinline__ void load_att_state(const ElasticArgs& A, int64_t ep, int64_t nl, realw Sn[6],
+                                               realw fcv[3], realw fckv[3]) {
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++) Sn[cc] = A.eps[cc * nl + ep];
+  if (A.fc_index) {
+    const int r = A.fc_index[ep];
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = A.fc_table[s * A.fc_rows + r];
+      fckv[s] = A.fc_table[(3 + s) * A.fc_rows + r];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = A.fc[s * nl + ep];
+      fckv[s] = A.fck[s * nl + ep];
+    }
+  }
+}
+
 // Point-per-lane variant: a 128-thread workgroup (two wave64s) processes one element and lane p
 // holds the GLL point p = i + 5 j + 25 k (lanes 125..127 shadow point 124 and store nothing).
 // All three contractions read the element tile from LDS, so no lane carries a column of state
 // between phases; the per-point state (memory variables, stored strain, SLS factors, moduli,
 // Jacobian terms) is loaded before the first barrier and is in flight while the tile is
 // gathered. Same arithmetic, in the same order, as k_elastic.
-template <bool STRAIN, bool ATT, int MODE>
-__global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
+// AV selects how the attenuation state is scheduled (the autotuner's choice; same results):
+// 0 every per-point array is loaded before the first barrier; 1 only the memory variables are,
+// the stored strain and SLS factors are loaded where the memory update needs them (fewer live
+// registers); 2 as 1 with the register budget capped for 5 waves per SIMD.
+template <bool STRAIN, bool ATT, int MODE, int AV>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(AV == 2 ? 5 : 1)))
+k_elastic_pt(ElasticArgs A, GLLk G) {
+  constexpr bool LATE = ATT && (AV == 1 || AV == 2);
   __shared__ realw sU[6][128];  // u (0..2) and dt*v (3..5)
   __shared__ realw sT[9][128];  // stress products t1 (0..2), t2 (3..5), t3 (6..8)
   __shared__ realw sG[75];      // hprime, hprimewgll, wgllwgll
@@ -357,22 +384,7 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
     for (int cc = 0; cc < 6; cc++)
 #pragma unroll
       for (int s = 0; s < 3; s++) Rv[cc][s] = A.R[(cc * 3 + s) * nl + ep];
-#pragma unroll
-    for (int cc = 0; cc < 6; cc++) Sn[cc] = A.eps[cc * nl + ep];
-    if (A.fc_index) {
-      const int r = A.fc_index[ep];
-#pragma unroll
-      for (int s = 0; s < 3; s++) {
-        fcv[s] = A.fc_table[s * A.fc_rows + r];
-        fckv[s] = A.fc_table[(3 + s) * A.fc_rows + r];
-      }
-    } else {
-#pragma unroll
-      for (int s = 0; s < 3; s++) {
-        fcv[s] = A.fc[s * nl + ep];
-        fckv[s] = A.fck[s * nl + ep];
-      }
-    }
+    if (!LATE) load_att_state(A, ep, nl, Sn, fcv, fckv);
   }
   __syncthreads();
 
@@ -410,6 +422,7 @@ __global__ void __launch_bounds__(128) k_elastic_pt(ElasticArgs A, GLLk G) {
     sg.yz = sg.yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
     if (active) {
       // memory-variable update (second-order RK) with old strain Sn and new strain Snp1
+      if (LATE) load_att_state(A, ep, nl, Sn, fcv, fckv);
       const realw Snp1[6] = {e_xx, e_yy, e_xy, e_xz, e_yz, e_tr};
 #pragma unroll
       for (int s = 0; s < 3; s++) {
@@ -539,16 +552,20 @@ __global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, r
 }  // namespace SF_VNS
 using namespace SF_VNS;
 
-void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool att, int mode, int shape) {
+void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool att, int mode, int shape,
+                         int att_variant) {
   if (a.nelem <= 0) return;
   const GLLk G = g_gll_host;
   if (shape == 1) {
     dim3 grid(a.nelem), block(128);
-#define SF_K(S, T, O) hipLaunchKernelGGL((k_elastic_pt<S, T, O>), grid, block, 0, s, a, G)
-#define SF_KM(O)                      \
-  if (att) SF_K(true, true, O);       \
-  else if (strain) SF_K(true, false, O); \
-  else SF_K(false, false, O);
+#define SF_K(S, T, O, V) hipLaunchKernelGGL((k_elastic_pt<S, T, O, V>), grid, block, 0, s, a, G)
+#define SF_KM(O)                                   \
+  if (att) {                                       \
+    if (att_variant == 1) SF_K(true, true, O, 1);  \
+    else if (att_variant == 2) SF_K(true, true, O, 2); \
+    else SF_K(true, true, O, 0);                   \
+  } else if (strain) SF_K(true, false, O, 0);      \
+  else SF_K(false, false, O, 0);
     if (mode == 1) {
       SF_KM(1)
     } else if (mode == 2) {

@@ -51,7 +51,9 @@ struct ElasticArgs {
 // (the element list is one colour: its elements share no global point); shape: 0 five elements
 // per workgroup with a GLL column per lane (k_elastic), 1 one element per workgroup with a GLL
 // point per lane (k_elastic_pt)
-void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode, int shape);
+// att_variant: scheduling of the attenuation state in the point kernel (k_elastic_pt's AV)
+void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode, int shape,
+                    int att_variant);
 // deviatoric strain planes 0..4 and trace/3 of A.displ; with_trace also writes plane 5 = 3 * trace/3
 void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,
                            bool exact);
@@ -74,7 +76,7 @@ struct BatchArgs {
   const int* nstep;        // per shot slot, decreasing (padding slots 0)
   const int* it;           // device step counter, 1-based current step
 };
-// sc = shots per workgroup (2, 4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour)
+// sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour)
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
 
 struct AcousticArgs {

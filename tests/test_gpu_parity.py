@@ -18,11 +18,12 @@ from conftest import GOLDEN, rel_l2, small_elastic_box, smooth_warp
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, shape=0):
+def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, shape=0, att_variant=0):
     from specfem3d_amd.solver import GPUSolver
-    print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly} shape={shape}", flush=True)
+    print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly} shape={shape} "
+          f"att_variant={att_variant}", flush=True)
     g = GPUSolver(db, src, rec, nstep=nstep, exact=exact, assembly=assembly, attenuation=attenuation,
-                  tuning={"elastic_shape": shape})
+                  tuning={"elastic_shape": shape, "att_variant": att_variant})
     out = g.run().copy()
     g.close()
     return out
@@ -60,6 +61,16 @@ def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     assert np.array_equal(out, ref), rel_l2(out, ref)
     fast = run_gpu(w.db, w.sources, w.receivers, 200, False, 0, w.attenuation, shape=shape)
     assert rel_l2(fast, ref) < 1e-5
+
+
+@pytest.mark.parametrize("att_variant", [1, 2])
+def test_attenuation_schedules_bitwise(oracle_mod, att_variant):
+    """The point kernel's attenuation-state schedules give the oracle's results bit for bit."""
+    from specfem3d_amd import workloads as W
+    w = W.config2(scale=0.125, nstep=150, with_attenuation=True, nrec_side=3)
+    ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 150, w.attenuation)
+    out = run_gpu(w.db, w.sources, w.receivers, 150, True, 1, w.attenuation, shape=1, att_variant=att_variant)
+    assert np.array_equal(out, ref), rel_l2(out, ref)
 
 
 @pytest.mark.parametrize("shape", [0, 1])
