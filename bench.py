@@ -181,7 +181,7 @@ def main():
     ap.add_argument("--exact", type=int, default=0, help="1: kernels without FMA (CPU rounding)")
     ap.add_argument("--assembly", type=int, default=0,
                     help="0 atomic scatter, 1 ordered (CPU element order), 2 coloured (no atomics)")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU oracle sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-shots", type=int, default=32, help="shots of the batched 64x64x32 workload (0: skip)")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="shrink the batched mesh (tests only)")

@@ -280,6 +280,9 @@ void sf_batch_displ(long batch, int shot, realw* out);
 void sf_batch_reset(long batch);
 void sf_batch_time(long batch, int nsteps, double* stiffness_ms, double* step_ms);
 void sf_batch_destroy(long batch);
+/* launch order (0-based elastic elements, each once) of the atomic and coloured stiffness launches,
+   e.g. a locality (Morton) order from the host; the ordered assembly keeps the processing list */
+void sf_set_element_order(long* Mesh_pointer, int n, const int* order);
 /* greedy first-fit element colouring of assembly = 2 (host only): color[q] for elems[q], elements
    sharing a global point differ; returns the number of colours or -1 (more than 32) */
 int sf_color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* color);

@@ -240,3 +240,79 @@ class KernelSolver(GPUSolver):
 
     def _zero_kernels(self):
         self.L.sf_zero_kernels(self.MP)
+
+
+# ------------------------------------------------------------------------------ time reversal
+class ReversalKernelSolver(GPUSolver):
+    """SIMULATION_TYPE=3 by time-reversed reconstruction, the reference's classic kernel run
+    (iterate_time.F90:233-290, compute_forces_viscoelastic_calling_routine.F90:694-870).
+
+    The forward run (``record_forward``: SIMULATION_TYPE=1 with SAVE_FORWARD) keeps the Stacey
+    traction record of every step (write_abs of compute_stacey_viscoelastic.F90:466-478) and its
+    last frame (save_forward_arrays). The kernel run starts the backward field from that frame and
+    steps it with b_deltat = -DT (prepare_timerun.F90:424-429), replaying the records in reverse
+    (read_abs NSTEP-it+1) and adding the sources at time (NSTEP-it) DT - t0
+    (compute_add_sources_viscoelastic.F90:650-690), while the adjoint field steps forward. The
+    checkpointed ``KernelSolver`` replaces this scheme; it stays available for drop-in hosts and is
+    tested by reconstructing the forward wavefield."""
+
+    def __init__(self, db, sources, receivers, nstep: int, exact: bool = False, assembly: int = 0, device: int = 0,
+                 tuning: dict | None = None):
+        if db.nspec_acoustic:
+            raise NotImplementedError("time-reversed reconstruction is driven for elastic domains only")
+        super().__init__(db, sources, receivers, nstep=nstep, exact=exact, assembly=assembly, simulation_type=3,
+                         nlength=nstep, device=device, tuning=tuning)
+        self.b_c = {k: C.c_float(v) for k, v in dict(
+            dt=np.float32(-self.deltat), dt2=np.float32(np.float32(-self.deltat) / np.float32(2)),
+            dtsq2=np.float32(np.float32(-self.deltat) * np.float32(-self.deltat) / np.float32(2))).items()}
+
+    @staticmethod
+    def record_forward(db, sources, receivers, nstep: int, exact: bool = False, assembly: int = 0,
+                       keep_displ: bool = False, device: int = 0):
+        """Forward run with SAVE_FORWARD: returns (seismograms, Stacey records (nstep, 3*25*nfaces),
+        last frame (displ, veloc, accel)[, displacement of every step])."""
+        g = GPUSolver(db, sources, receivers, nstep=nstep, exact=exact, assembly=assembly, save_forward=True,
+                      device=device)
+        recs, disp = [], []
+        for _ in range(nstep):
+            g.step()
+            recs.append(g.b_absorb_field.copy())
+            if keep_displ:
+                disp.append(g.fields()[0])
+        seis = g.seis_d[: g.seismo_current].copy()
+        last = g.fields()
+        g.close()
+        return (seis, np.stack(recs), last) + ((np.stack(disp),) if keep_displ else ())
+
+    def start_from(self, last):
+        """read_forward_arrays: the backward field starts at the forward run's last frame."""
+        n3 = 3 * self.db.nglob
+        self._last = [np.ascontiguousarray(a, np.float32) for a in last]
+        self.L.transfer_b_fields_to_device_(C.byref(C.c_int(n3)), *(abi.ptr(a) for a in self._last), self.MP)
+        self.it = 0
+
+    def backward_step(self, records: np.ndarray):
+        """One reversed step of the backward field (FORWARD_OR_ADJOINT = 3)."""
+        L, MP = self.L, self.MP
+        self.it += 1
+        it = self.it
+        three, zero = self._ref(3), self._ref(0)
+        bdt, bdt2, bdtsq2 = (C.byref(self.b_c[k]) for k in ("dt", "dt2", "dtsq2"))
+        L.update_displacement_cuda_(MP, bdt, bdt2, bdtsq2, three)
+        strain, att = C.c_int(1), C.c_int(0)
+        for iphase in (1, 2):
+            ip = C.c_int(iphase)
+            L.compute_forces_viscoelastic_cuda_(MP, C.byref(ip), bdt, zero, zero, C.byref(strain), C.byref(att), three)
+            if iphase == 1:
+                self._rec = np.ascontiguousarray(records[self.nstep - it], np.float32)
+                L.compute_stacey_viscoelastic_cuda_(MP, C.byref(ip), abi.ptr(self._rec), three)
+                if self.stf_table is not None:
+                    stf = np.ascontiguousarray(self.stf_table[self.nstep - it], np.float64)
+                    L.compute_add_sources_el_s3_cuda_(MP, stf.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(stf))))
+        L.kernel_3_a_cuda_(MP, bdt2, bdt2, zero, three)
+
+    def b_displ(self) -> np.ndarray:
+        n3 = 3 * self.db.nglob
+        out = np.zeros(n3, np.float32)
+        self.L.transfer_b_displ_from_device_(C.byref(C.c_int(n3)), abi.ptr(out), self.MP)
+        return out

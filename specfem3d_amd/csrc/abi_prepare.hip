@@ -600,7 +600,8 @@ namespace sf {
 void ensure_element_colors(Mesh* m) {
   if (m->el_color_elems || m->nspec_elastic <= 0 || !m->elastic_elems) return;
   std::vector<int> lst(m->nspec_elastic);
-  SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+  const int* src = m->elastic_launch ? m->elastic_launch : m->elastic_elems;  // same colours as the mesh's own
+  SF_CHECK(hipMemcpy(lst.data(), src, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
   build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
 }
 
@@ -867,7 +868,7 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   SF_CHECK(hipStreamSynchronize(m->stream));
   free_checkpoints(m);
   dfree(m->ibool); dfree(m->irregular); dfree(m->jac); dfree(m->elastic_elems); dfree(m->acoustic_elems);
-  dfree(m->el_color_elems); dfree(m->ac_color_elems);
+  dfree(m->el_color_elems); dfree(m->ac_color_elems); dfree(m->elastic_launch);
   dfree(m->kappamu); dfree(m->rho); dfree(m->kappa_ac); dfree(m->rmass); dfree(m->rmass_ac);
   for (int f = 0; f < 2; f++) {
     dfree(m->el[f].displ); dfree(m->el[f].veloc); dfree(m->el[f].accel);
@@ -922,11 +923,7 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
     }
   }
   if (assembly == 2) {
-    if (!m->el_color_elems && m->nspec_elastic > 0 && m->elastic_elems) {
-      std::vector<int> lst(m->nspec_elastic);
-      SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
-      build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
-    }
+    if (!m->el_color_elems) ensure_element_colors(m);
     if (!m->ac_color_elems && m->nspec_acoustic > 0 && m->acoustic_elems) {
       std::vector<int> lst(m->nspec_acoustic);
       SF_CHECK(hipMemcpy(lst.data(), m->acoustic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
@@ -949,6 +946,29 @@ extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
     m->att_variant = value;
   } else {
     fatal("sf_set_tuning", "unknown tuning switch");
+  }
+}
+
+// Launch order of the elastic elements for the atomic and coloured stiffness launches (results do
+// not depend on it except for float-atomic rounding): `order` lists the 0-based elastic elements,
+// each once. Hosts pass a locality order (e.g. Morton order of element centres) so that elements
+// sharing points run close together and their shared rows meet in L2. Re-colours when coloured.
+extern "C" void sf_set_element_order(long* Mesh_pointer, int n, const int* order) {
+  Mesh* m = from_handle(Mesh_pointer);
+  HostSide* H = host_of(m);
+  if (n != m->nspec_elastic) fatal("sf_set_element_order", "the order must list every elastic element once");
+  std::vector<unsigned char> seen((size_t)m->nspec, 0);
+  for (int q = 0; q < n; q++) {
+    const int e = order[q];
+    if (e < 0 || e >= m->nspec || !H->is_el[e] || seen[e]) fatal("sf_set_element_order", "not a permutation of the elastic elements");
+    seen[e] = 1;
+  }
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  dfree(m->elastic_launch);
+  m->elastic_launch = dupload(order, (int64_t)n);
+  if (m->el_color_elems) {
+    std::vector<int> lst(order, order + n);
+    build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
   }
 }
 

@@ -46,7 +46,9 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
   ElasticArgs a;
   std::memset(&a, 0, sizeof(a));
   a.nelem = m->nspec_elastic;
-  a.elems = m->elastic_elems;
+  // atomic launches may walk the elements in the host's locality order; the ordered assembly keeps
+  // the processing list (its gather sums in that order)
+  a.elems = (m->elastic_launch && m->assembly != 1) ? m->elastic_launch : m->elastic_elems;
   a.ibool = m->ibool;
   a.irregular = m->irregular;
   a.xix_regular = m->xix_regular;

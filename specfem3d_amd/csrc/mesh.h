@@ -78,6 +78,8 @@ struct Mesh {
   // domains
   int nspec_elastic = 0, nspec_acoustic = 0;
   int* elastic_elems = nullptr;    // element lists in processing order
+  int* elastic_launch = nullptr;   // optional locality order of the elastic elements for the atomic
+                                   // and coloured launches (sf_set_element_order); null: elastic_elems
   int* acoustic_elems = nullptr;
   unsigned char* h_is_elastic = nullptr;
   unsigned char* h_is_acoustic = nullptr;

@@ -26,6 +26,23 @@ from .gll import get_gll
 from .mesh.database import Database
 
 
+def morton_order(db, elems: np.ndarray) -> np.ndarray:
+    """``elems`` sorted along a Morton (Z-order) curve of their centres: elements that share points
+    end up close together in the launch order (their shared rows meet in L2)."""
+    elems = np.asarray(elems, np.int64)
+    if elems.size == 0:
+        return elems.astype(np.int32)
+    ib = db.ibool.reshape(db.nspec, 125)[elems]
+    c = np.stack([db.xstore[ib].mean(1), db.ystore[ib].mean(1), db.zstore[ib].mean(1)], 1).astype(np.float64)
+    lo, hi = c.min(0), c.max(0)
+    q = np.floor((c - lo) / np.maximum(hi - lo, 1e-30) * 1023.0 + 0.5).astype(np.uint64)
+    key = np.zeros(len(elems), np.uint64)
+    for bit in range(10):
+        for d in range(3):
+            key |= ((q[:, d] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + d)
+    return elems[np.argsort(key, kind="stable")].astype(np.int32)
+
+
 def _fortran_phase_list(elems: np.ndarray) -> tuple[int, np.ndarray]:
     """phase_ispec_inner_*(num, 2): column 1 outer (none at NPROC=1), column 2 inner, 1-based."""
     n = max(len(elems), 1)
@@ -40,7 +57,8 @@ class GPUSolver:
     def __init__(self, db: Database, sources=None, receivers=None, nstep: int = 1000,
                  attenuation=None, exact: bool = True, assembly: int = 0, simulation_type: int = 1,
                  save_forward: bool = False, save_seismograms=("d",), nlength: int | None = None,
-                 device: int = 0, tuning: dict | None = None, undo_attenuation: bool = False):
+                 device: int = 0, tuning: dict | None = None, undo_attenuation: bool = False,
+                 element_order: str | None = "morton"):
         self.L = abi.load()
         self.db = db
         self.sources = sources
@@ -58,6 +76,7 @@ class GPUSolver:
         self.save = save_seismograms
         self.tuning = dict(tuning or {})
         self.undo_att = bool(undo_attenuation)
+        self.element_order = element_order
         self._prepare(exact, assembly, save_forward, device)
 
     # ------------------------------------------------------------------------------ set-up
@@ -165,6 +184,9 @@ class GPUSolver:
                         self.MP, A.i(nrec), A.i(self.nstep), A.fa(R.hxir.astype(np.float32).ravel()),
                         A.fa(R.hetar.astype(np.float32).ravel()), A.fa(R.hgammar.astype(np.float32).ravel()),
                         A.i(nrec), A.ia(np.zeros(nrec, np.int32)), A.ia(rec_ispec))
+        if self.elastic and self.element_order == "morton":
+            order = morton_order(db, db.elastic_elements())
+            L.sf_set_element_order(self.MP, C.c_int(len(order)), abi.ptr(order))
         n3 = 3 * db.nglob
         self.displ = np.zeros(n3, np.float32)
         self.veloc = np.zeros(n3, np.float32)
