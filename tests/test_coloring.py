@@ -45,3 +45,25 @@ def test_subset_and_warped_mesh():
     n, col = color(db, elems)
     assert 1 <= n <= 27
     check_proper(db, elems, col)
+
+
+def test_morton_launch_order_is_a_local_permutation():
+    """The host's Morton launch order lists every element once and keeps face neighbours close in
+    launch position: far more neighbour pairs lie within 32 launch slots than in the x-fastest
+    processing order, where every z neighbour is a whole plane (64 elements) away."""
+    from specfem3d_amd.solver import morton_order
+    db = build_box(8, 8, 8, (0, 8000, 0, 8000, -8000, 0), material=homogeneous(2300, 2800, 1500))
+    ref = np.arange(db.nspec)
+    mo = morton_order(db, ref)
+    assert sorted(mo.tolist()) == ref.tolist()
+    pos = np.empty(db.nspec, np.int64)
+    pos[mo] = np.arange(db.nspec)
+    e = np.arange(db.nspec)
+    ex, ey, ez = e % 8, (e // 8) % 8, e // 64
+    pairs = [(e[ex < 7], e[ex < 7] + 1), (e[ey < 7], e[ey < 7] + 8), (e[ez < 7], e[ez < 7] + 64)]
+    near_ref = np.mean(np.concatenate([np.abs(b - a) for a, b in pairs]) < 32)
+    near_mo = np.mean(np.concatenate([np.abs(pos[b] - pos[a]) for a, b in pairs]) < 32)
+    assert near_ref < 0.7 and near_mo > 0.8
+    n, col = color(db, mo)
+    assert n == 8
+    check_proper(db, mo, col)
