@@ -530,6 +530,15 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
       }
     m->nspec_elastic = (int)lst.size();
     m->elastic_elems = dupload(lst.data(), (int64_t)lst.size());
+    int gmin = m->nglob, gmax = -1;
+    for (int e : lst)
+      for (int p = 0; p < 125; p++) {
+        const int g = H->ibool[(int64_t)e * 125 + p];
+        gmin = std::min(gmin, g);
+        gmax = std::max(gmax, g);
+      }
+    m->el_g0 = gmax < 0 ? 0 : (gmin / 4) * 4;
+    m->el_n = gmax < 0 ? 0 : (int64_t)gmax + 1 - m->el_g0;
   }
   m->el[0].displ = dmalloc<realw>(3 * n);
   m->el[0].veloc = dmalloc<realw>(3 * n);

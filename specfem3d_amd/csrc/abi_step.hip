@@ -23,7 +23,9 @@ static inline int field_of(int foa) { return foa == 3 ? 1 : 0; }
 
 static void flush_el(Mesh* m, int f) {
   if (m->pending_k3_el[f]) {
-    launch_corrector(m->stream, 3 * (int64_t)m->nglob, m->el[f].veloc, m->el[f].accel, m->rmass, m->pending_dt2_el[f]);
+    const int64_t o = 3 * m->el_g0;  // only the elastic range changes (see Mesh::el_g0)
+    launch_corrector(m->stream, 3 * m->el_n, m->el[f].veloc + o, m->el[f].accel + o, m->rmass + o,
+                     m->pending_dt2_el[f]);
     m->pending_k3_el[f] = 0;
   }
 }
@@ -117,14 +119,15 @@ extern "C" void FC_FUNC_(update_displacement_cuda, UPDATE_DISPLACMENT_CUDA)(long
   Mesh* m = from_handle(Mesh_pointer);
   const int f = field_of(*FORWARD_OR_ADJOINT);
   check_field(m, f, "update_displacement_cuda");
-  const int64_t n = 3 * (int64_t)m->nglob;
+  const int64_t n = 3 * m->el_n, o = 3 * m->el_g0;  // only the elastic range changes
   Wavefield& w = m->el[f];
   if (m->pending_k3_el[f]) {
-    launch_corrector_predictor(m->stream, n, w.displ, w.veloc, w.accel, m->rmass, m->pending_dt2_el[f], *deltat_F,
-                               *deltatover2_F, *deltatsqover2_F);
+    launch_corrector_predictor(m->stream, n, w.displ + o, w.veloc + o, w.accel + o, m->rmass + o,
+                               m->pending_dt2_el[f], *deltat_F, *deltatover2_F, *deltatsqover2_F);
     m->pending_k3_el[f] = 0;
   } else {
-    launch_predictor(m->stream, n, w.displ, w.veloc, w.accel, *deltat_F, *deltatover2_F, *deltatsqover2_F);
+    launch_predictor(m->stream, n, w.displ + o, w.veloc + o, w.accel + o, *deltat_F, *deltatover2_F,
+                     *deltatsqover2_F);
   }
 }
 

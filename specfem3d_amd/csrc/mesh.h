@@ -90,6 +90,8 @@ struct Mesh {
   realw* kappa_ac = nullptr;       // kappastore for acoustic elements
   realw* rmass = nullptr;          // 3*nglob inverse mass (elastic)
   realw* rmass_ac = nullptr;       // nglob inverse mass (acoustic)
+  int64_t el_g0 = 0, el_n = 0;     // elastic points lie in [el_g0, el_g0 + el_n) (el_g0 a multiple of 4);
+                                   // the elastic Newmark streams cover only this range
   int64_t ac_g0 = 0, ac_n = 0;     // acoustic points lie in [ac_g0, ac_g0 + ac_n) (ac_g0 a multiple of 4);
                                    // the acoustic Newmark streams cover only this range
 
