@@ -247,7 +247,11 @@ void unregister_host_array_(realw* h_array);
 void sf_set_options(long* Mesh_pointer, int exact, int assembly);
 /* tuning switch by name: "elastic_shape" 0 = column-per-lane stiffness kernel (five elements per
    workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0/1/2 =
-   scheduling of the attenuation state in the point kernel (same results); unknown names are fatal */
+   scheduling of the attenuation state in the point kernel (same results); "acoustic_kernel_field"
+   0 = compute_kernels_acoustic_cuda pairs the adjoint potential with the reconstructed field as the
+   reference does (compute_kernels.f90:269,325), 1 = it uses the adjoint potential_dot_dot instead
+   (the misfit gradient when the adjoint field enters the fluid through the coupling);
+   unknown names are fatal */
 void sf_set_tuning(long* Mesh_pointer, const char* name, int value);
 /* device-resident checkpoints for checkpointed kernel runs (the files of
    save_forward_arrays_undoatt / read_forward_arrays_undoatt, save_forward_arrays.f90:102-179 and

@@ -300,7 +300,8 @@ def element_strain(db, displ, precision="f32"):
 
 def _snapshot(s: OracleSolver) -> dict:
     """State saved per time subset (save_forward_arrays_undoatt, save_forward_arrays.f90:102-179)."""
-    snap = {"it": s.it, "displ": s.displ.copy(), "veloc": s.veloc.copy(), "accel": s.accel.copy()}
+    snap = {"it": s.it, "displ": s.displ.copy(), "veloc": s.veloc.copy(), "accel": s.accel.copy(),
+            "pot": s.pot.copy(), "potd": s.potd.copy(), "potdd": s.potdd.copy()}
     if s.att is not None:
         snap["R"] = [r.copy() for r in s.R]
     return snap
@@ -314,6 +315,9 @@ def _restore(s: OracleSolver, snap: dict):
     s.displ[:] = snap["displ"]
     s.veloc[:] = snap["veloc"]
     s.accel[:] = snap["accel"]
+    s.pot[:] = snap["pot"]
+    s.potd[:] = snap["potd"]
+    s.potdd[:] = snap["potdd"]
     if s.att is not None:
         for r, v in zip(s.R, snap["R"]):
             r[:] = v
@@ -333,7 +337,7 @@ def _recompute_strain(s: OracleSolver):
 
 
 def kernel_run(db, sources, receivers, nstep: int, adj: np.ndarray, nt_dump: int, precision: str = "f64",
-               attenuation=None):
+               attenuation=None, with_acoustic: bool = False, acoustic_adjoint: str = "potential"):
     """Checkpointed sensitivity-kernel run at NPROC=1 (elastic domains).
 
     Follows iterate_time_undoatt.F90:29-632 for SIMULATION_TYPE=1 with SAVE_FORWARD (snapshot at the
@@ -345,9 +349,20 @@ def kernel_run(db, sources, receivers, nstep: int, adj: np.ndarray, nt_dump: int
     accumulate after every adjoint step (compute_kernels.f90:67-222) with the reconstructed strain
     recomputed from b_displ.
 
+    In fluid elements the buffered fields are b_potential and b_potential_dot_dot and the
+    acoustic kernels accumulate as compute_kernels_ac (compute_kernels.f90:257-337), pairing the
+    adjoint ``potential_acoustic`` with them (``acoustic_adjoint="potential"``, :269 and :325).
+    ``acoustic_adjoint="potential_dot_dot"`` pairs the adjoint potential_dot_dot instead: the
+    adjoint potential is driven by the adjoint displacement through the coupling
+    (compute_coupling_acoustic_el.f90:105-110), so the misfit gradient needs its second time
+    derivative.
+
     Returns (forward seismograms (nstep, nrec, 3), rho_kl, mu_kl, kappa_kl) -- the raw time
-    integrals before save_adjoint_kernels' scaling.
+    integrals before save_adjoint_kernels' scaling -- plus (rho_ac_kl, kappa_ac_kl) when
+    ``with_acoustic``.
     """
+    if acoustic_adjoint not in ("potential", "potential_dot_dot"):
+        raise ValueError("acoustic_adjoint must be 'potential' or 'potential_dot_dot'")
     nt_dump = min(nt_dump, nstep)
     fwd = OracleSolver(db, sources, receivers, precision, attenuation=attenuation)
     fwd.stf_table = sources.stf_table(nstep)
@@ -375,6 +390,17 @@ def kernel_run(db, sources, receivers, nstep: int, adj: np.ndarray, nt_dump: int
     elems = db.elastic_elements()
     ib = np.ascontiguousarray(db.ibool.reshape(-1), np.int32)
     adj = np.ascontiguousarray(adj, R)
+    ac_elems = db.acoustic_elements()
+    rho_ac_kl, kappa_ac_kl = np.zeros(ne, R), np.zeros(ne, R)
+    fna = getattr(L, "orc_compute_kernels_ac" + b.sfx)
+    g = None
+    if len(ac_elems):
+        from specfem3d_amd.gll import get_gll
+        g = get_gll()
+        hp = np.ascontiguousarray(g.hprime, R)
+        irr = np.ascontiguousarray(db.irregular_element_number, np.int32)
+        jac9 = np.ascontiguousarray(db.jac9.reshape(-1), R)
+        rho = np.ascontiguousarray(db.rhostore.reshape(-1), R)
     it = 0
     for i in range(1, nsub + 1):
         slot = nsub - i
@@ -383,15 +409,22 @@ def kernel_run(db, sources, receivers, nstep: int, adj: np.ndarray, nt_dump: int
         buf = []
         for _ in range(n_i):
             b.step()
-            buf.append(b.displ.copy())
+            buf.append((b.displ.copy(), b.pot.copy(), b.potdd.copy()))
         _recompute_strain(adjs)          # compute_strain_att before the adjoint subset
         for m in range(n_i):
             it += 1
-            bd = buf[n_i - 1 - m]
+            bd, bp, bpdd = buf[n_i - 1 - m]
             adjs.step(adjoint_source=adj[nstep - it] if it < nstep else None)
-            be = element_strain(db, bd, precision)
-            e = adjs.eps
-            fn(C.c_int(len(elems)), _p(elems), _p(ib), _p(adjs.accel), _p(bd), _p(e[0]), _p(e[1]), _p(e[2]),
-               _p(e[3]), _p(e[4]), _p(adjs.eps_tr3), _p(be[0]), _p(be[1]), _p(be[2]), _p(be[3]), _p(be[4]),
-               _p(be[5]), adjs._r(adjs.deltat), _p(rho_kl), _p(mu_kl), _p(kappa_kl))
+            if len(elems):
+                be = element_strain(db, bd, precision)
+                e = adjs.eps
+                fn(C.c_int(len(elems)), _p(elems), _p(ib), _p(adjs.accel), _p(bd), _p(e[0]), _p(e[1]), _p(e[2]),
+                   _p(e[3]), _p(e[4]), _p(adjs.eps_tr3), _p(be[0]), _p(be[1]), _p(be[2]), _p(be[3]), _p(be[4]),
+                   _p(be[5]), adjs._r(adjs.deltat), _p(rho_kl), _p(mu_kl), _p(kappa_kl))
+            if len(ac_elems):
+                chi = adjs.potdd if acoustic_adjoint == "potential_dot_dot" else adjs.pot
+                fna(C.c_int(len(ac_elems)), _p(ac_elems), _p(ib), _p(irr), adjs._r(db.xix_regular), _p(jac9), _p(hp),
+                    _p(rho), _p(chi), _p(bp), _p(bpdd), adjs._r(adjs.deltat), _p(rho_ac_kl), _p(kappa_ac_kl))
+    if with_acoustic:
+        return seis, rho_kl, mu_kl, kappa_kl, rho_ac_kl, kappa_ac_kl
     return seis, rho_kl, mu_kl, kappa_kl

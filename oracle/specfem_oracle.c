@@ -650,6 +650,60 @@ void FN(orc_compute_kernels_el)(int nelem, const int *elems, const int *ibool, c
     }
 }
 
+/* acoustic sensitivity kernels: compute_kernels.f90 compute_kernels_ac (rho_ac_kl, kappa_ac_kl) with
+ * the gradient of compute_gradient_in_acoustic_fast_Deville (compute_gradient_in_acoustic.f90):
+ * grad chi / rho from the three contractions, mapped by the inverse Jacobian (irregular) or
+ * xix_regular, then multiplied back by rho; rho_ac_kl += dt grad(chi_adj).grad(chi_b),
+ * kappa_ac_kl += dt chi_adj chi_b''. */
+static void acoustic_gradient(const R *hp, const R *chi, int irr, R xix_regular, const R *jac9,
+                              const R *rhostore_e, R *gx, R *gy, R *gz)
+{
+    R t1[125], t2[125], t3[125];
+    grad_ref(hp, chi, t1, t2, t3);
+    for (int p = 0; p < 125; p++) {
+        R rho_invl = (R)1 / rhostore_e[p];
+        if (irr) {
+            const R *J = jac9 + (int64_t)(irr - 1) * 9 * 125;
+            R xixl = J[0 * 125 + p], xiyl = J[1 * 125 + p], xizl = J[2 * 125 + p];
+            R etaxl = J[3 * 125 + p], etayl = J[4 * 125 + p], etazl = J[5 * 125 + p];
+            R gaxl = J[6 * 125 + p], gayl = J[7 * 125 + p], gazl = J[8 * 125 + p];
+            gx[p] = (t1[p] * xixl + t2[p] * etaxl + t3[p] * gaxl) * rho_invl;
+            gy[p] = (t1[p] * xiyl + t2[p] * etayl + t3[p] * gayl) * rho_invl;
+            gz[p] = (t1[p] * xizl + t2[p] * etazl + t3[p] * gazl) * rho_invl;
+        } else {
+            gx[p] = t1[p] * xix_regular * rho_invl;
+            gy[p] = t2[p] * xix_regular * rho_invl;
+            gz[p] = t3[p] * xix_regular * rho_invl;
+        }
+    }
+}
+
+void FN(orc_compute_kernels_ac)(int nelem, const int *elems, const int *ibool, const int *irregular, R xix_regular,
+                                const R *jac9, const R *hprime, const R *rhostore, const R *pot, const R *b_pot,
+                                const R *b_potdd, R deltat, R *rho_ac_kl, R *kappa_ac_kl)
+{
+    R chi[125], bchi[125], ax[125], ay[125], az[125], bx[125], by[125], bz[125];
+    for (int ie = 0; ie < nelem; ie++) {
+        int e = elems[ie];
+        const int *ib = ibool + (int64_t)e * 125;
+        for (int p = 0; p < 125; p++) {
+            bchi[p] = b_pot[ib[p]];
+            chi[p] = pot[ib[p]];
+        }
+        const R *rho_e = rhostore + (int64_t)e * 125;
+        acoustic_gradient(hprime, bchi, irregular[e], xix_regular, jac9, rho_e, bx, by, bz);
+        acoustic_gradient(hprime, chi, irregular[e], xix_regular, jac9, rho_e, ax, ay, az);
+        for (int p = 0; p < 125; p++) {
+            int64_t ep = (int64_t)e * 125 + p;
+            R rhol = rho_e[p];
+            R a1 = rhol * ax[p], a2 = rhol * ay[p], a3 = rhol * az[p];
+            R b1 = rhol * bx[p], b2 = rhol * by[p], b3 = rhol * bz[p];
+            rho_ac_kl[ep] = rho_ac_kl[ep] + deltat * (a1 * b1 + a2 * b2 + a3 * b3);
+            kappa_ac_kl[ep] = kappa_ac_kl[ep] + deltat * pot[ib[p]] * b_potdd[ib[p]];
+        }
+    }
+}
+
 /* max norm of a vector field and NaN flag: check_stability.f90 (max over points of |u|) */
 R FN(orc_max_norm)(int64_t nglob, const R *f)
 {

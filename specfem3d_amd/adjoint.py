@@ -12,10 +12,12 @@ with UNDO_ATTENUATION_AND_OR_PML, by forward checkpointing and recomputation
   ``compute_strain_cuda``), recomputes the subset keeping every step's displacement in an HBM
   history (``sf_history_store``), then steps the adjoint field through the subset with the
   reconstructed displacement read back time-reversed (``sf_history_load``) and accumulates the
-  kernels (``compute_kernels_elastic_cuda``) after each adjoint step.
+  kernels (``compute_kernels_elastic_cuda``, and ``compute_kernels_acoustic_cuda`` in fluid
+  elements) after each adjoint step.
 
-``finalize_kernels`` applies save_adjoint_kernels.f90:276-412 (rho, mu, kappa, rhop, alpha, beta);
-``write_kernel_bin`` writes them as Fortran unformatted records. The torch.autograd wrapper used by
+``finalize_kernels`` applies save_adjoint_kernels.f90:276-412 (rho, mu, kappa, rhop, alpha, beta),
+``finalize_acoustic_kernels`` save_adjoint_kernels.f90:188-222 (rho_ac, kappa_ac, rhop_ac,
+alpha_ac); ``write_kernel_bin`` writes them as Fortran unformatted records. The torch.autograd wrapper used by
 the gradient tests lives in ``specfem3d_amd.autograd``.
 """
 from __future__ import annotations
@@ -86,6 +88,25 @@ def finalize_kernels(db, rho_kl, mu_kl, kappa_kl, mustore=None, kappastore=None)
     return out
 
 
+def finalize_acoustic_kernels(db, rho_ac_kl, kappa_ac_kl) -> dict:
+    """save_adjoint_kernels.f90:188-222 (fluid elements): the accumulated integrals are kept positive
+    and become the relative kernels rho_ac = rho_ac_kl / rho, kappa_ac = kappa_ac_kl / kappa, with
+    rhop_ac = rho_ac + kappa_ac and alpha_ac = 2 kappa_ac."""
+    shp = (db.nspec, 5, 5, 5)
+    rho = np.asarray(db.rhostore, np.float32).reshape(shp)
+    kap = np.asarray(db.kappastore, np.float32).reshape(shp)
+    rk, kk = (np.asarray(a, np.float32).reshape(shp) for a in (rho_ac_kl, kappa_ac_kl))
+    ac = db.ispec_is_acoustic
+    out = {k: np.zeros(shp, np.float32) for k in ("rho_ac", "kappa_ac", "rhop_ac", "alpha_ac")}
+    r = (np.float32(1.0) / rho[ac]) * rk[ac]
+    k = (np.float32(1.0) / kap[ac]) * kk[ac]
+    out["rho_ac"][ac] = r
+    out["kappa_ac"][ac] = k
+    out["rhop_ac"][ac] = r + k
+    out["alpha_ac"][ac] = np.float32(2.0) * k
+    return out
+
+
 def write_kernel_bin(path: str, arr: np.ndarray) -> None:
     """One Fortran unformatted sequential record (4-byte length markers), as save_adjoint_kernels
     writes ``proc000000_<name>_kernel.bin`` (the array in (i,j,k,ispec) order)."""
@@ -109,13 +130,25 @@ def read_kernel_bin(path: str) -> np.ndarray:
 class KernelSolver(GPUSolver):
     """Checkpointed sensitivity-kernel run on the GPU (SIMULATION_TYPE=3, undo-attenuation loop).
 
-    Elastic (and viscoelastic) domains; the forward/adjoint field is ``FORWARD_OR_ADJOINT`` 1 and
-    the reconstructed forward field 3, as in the reference."""
+    Elastic, viscoelastic and coupled acoustic-elastic meshes (sources and receivers in solid
+    elements); the forward/adjoint field is ``FORWARD_OR_ADJOINT`` 1 and the reconstructed forward
+    field 3, as in the reference. Each step runs the acoustic domain first, then the elastic one
+    (compute_forces_acoustic_calling, then compute_forces_viscoelastic_calling).
+
+    ``acoustic_kernels="reference"`` accumulates the fluid kernels as the reference does, from the
+    adjoint potential (compute_kernels.f90:269,325). Because the adjoint potential is driven by the
+    adjoint displacement through the coupling, the misfit gradient needs its second time derivative
+    instead: ``"gradient"`` uses the adjoint potential_dot_dot (same arithmetic otherwise)."""
 
     def __init__(self, db, sources, receivers, nstep: int, nt_dump: int = 64, attenuation=None,
-                 exact: bool = False, assembly: int = 0, device: int = 0, tuning: dict | None = None):
-        if db.nspec_acoustic:
-            raise NotImplementedError("checkpointed kernel runs cover elastic domains only")
+                 exact: bool = False, assembly: int = 0, device: int = 0, tuning: dict | None = None,
+                 acoustic_kernels: str = "reference"):
+        if acoustic_kernels not in ("reference", "gradient"):
+            raise ValueError("acoustic_kernels must be 'reference' or 'gradient'")
+        tuning = dict(tuning or {})
+        tuning["acoustic_kernel_field"] = int(acoustic_kernels == "gradient")
+        if not db.nspec_elastic:
+            raise NotImplementedError("kernel runs need solid elements (sources and receivers are elastic)")
         if receivers is None or receivers.n == 0:
             raise ValueError("a kernel run needs receivers (the adjoint sources)")
         super().__init__(db, sources, receivers, nstep=nstep, attenuation=attenuation, exact=exact,
@@ -124,9 +157,28 @@ class KernelSolver(GPUSolver):
         self.nt_dump = max(1, min(int(nt_dump), self.nstep))
         self.nsub = math.ceil(self.nstep / self.nt_dump)
         self.forward_done = False
+        self.acoustic_kernels = None   # raw (rho_ac_kl, kappa_ac_kl) of the last adjoint() in fluid meshes
 
     def _subset_len(self, slot: int) -> int:
         return self.nt_dump if slot < self.nsub - 1 else self.nstep - (self.nsub - 1) * self.nt_dump
+
+    def _acoustic_step(self, foa: int):
+        """Fluid part of one step of field ``foa`` (compute_forces_acoustic_calling /
+        _backward_calling, undo-attenuation Stacey): free surface, stiffness, Stacey and the
+        solid-to-fluid coupling, corrector, free surface."""
+        L, MP = self.L, self.MP
+        f, zero = self._ref(foa), self._ref(0)
+        dt2 = C.byref(self._c["dt2"])
+        ncf = C.c_int(self.db.coupling_faces.n)
+        L.acoustic_enforce_free_surf_cuda_(MP, zero, f)
+        for iphase in (1, 2):
+            ip = C.c_int(iphase)
+            L.compute_forces_acoustic_cuda_(MP, C.byref(ip), zero, zero, f)
+            if iphase == 1:
+                L.compute_stacey_acoustic_undoatt_cuda_(MP, C.byref(ip), f)
+                L.compute_coupling_ac_el_cuda_(MP, C.byref(ip), C.byref(ncf), f)
+        L.kernel_3_acoustic_cuda_(MP, dt2, dt2, f)
+        L.acoustic_enforce_free_surf_cuda_(MP, zero, f)
 
     def _b_step(self, fwd_it: int):
         """One step of the reconstructed field (compute_forces_viscoelastic_backward_calling with
@@ -134,14 +186,21 @@ class KernelSolver(GPUSolver):
         L, MP = self.L, self.MP
         three, zero = self._ref(3), self._ref(0)
         dt, dt2, dtsq2 = (C.byref(self._c[k]) for k in ("dt", "dt2", "dtsq2"))
+        if self.acoustic:
+            L.update_displacement_ac_cuda_(MP, dt, dt2, dtsq2, three)
         L.update_displacement_cuda_(MP, dt, dt2, dtsq2, three)
+        if self.acoustic:
+            self._acoustic_step(3)
         strain = C.c_int(1)
         att = C.c_int(int(self.att is not None))
+        ncf = C.c_int(self.db.coupling_faces.n)
         for iphase in (1, 2):
             ip = C.c_int(iphase)
             L.compute_forces_viscoelastic_cuda_(MP, C.byref(ip), dt, zero, zero, C.byref(strain), C.byref(att), three)
             if iphase == 1:
                 L.compute_stacey_viscoelastic_undoatt_cuda_(MP, C.byref(ip), three)
+                if self.acoustic:
+                    L.compute_coupling_el_ac_cuda_(MP, C.byref(ip), C.byref(ncf), three)
                 if self.stf_table is not None:
                     stf = np.ascontiguousarray(self.stf_table[fwd_it - 1], np.float64)
                     L.compute_add_sources_el_s3_cuda_(MP, stf.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(stf))))
@@ -152,16 +211,23 @@ class KernelSolver(GPUSolver):
         L, MP = self.L, self.MP
         one, zero = self._ref(1), self._ref(0)
         dt, dt2, dtsq2 = (C.byref(self._c[k]) for k in ("dt", "dt2", "dtsq2"))
+        if self.acoustic:
+            L.update_displacement_ac_cuda_(MP, dt, dt2, dtsq2, one)
         L.update_displacement_cuda_(MP, dt, dt2, dtsq2, one)
+        if self.acoustic:
+            self._acoustic_step(1)
         strain = C.c_int(1)
         att = C.c_int(int(self.att is not None))
         nrec = C.c_int(self.receivers.n)
         ntr = C.c_int(self.nstep)
+        ncf = C.c_int(self.db.coupling_faces.n)
         for iphase in (1, 2):
             ip = C.c_int(iphase)
             L.compute_forces_viscoelastic_cuda_(MP, C.byref(ip), dt, zero, zero, C.byref(strain), C.byref(att), one)
             if iphase == 1:
                 L.compute_stacey_viscoelastic_undoatt_cuda_(MP, C.byref(ip), one)
+                if self.acoustic:
+                    L.compute_coupling_el_ac_cuda_(MP, C.byref(ip), C.byref(ncf), one)
                 if it < self.nstep:
                     L.add_sources_el_sim_type_2_or_3_(MP, abi.ptr(adj), C.byref(nrec), C.byref(nrec), C.byref(ntr),
                                                       C.byref(C.c_int(it)))
@@ -213,11 +279,17 @@ class KernelSolver(GPUSolver):
                 L.sf_history_load(MP, C.c_int(3), C.c_int(n_i - 1 - m))
                 self._adj_step(it, adj)
                 L.compute_kernels_elastic_cuda_(MP, C.byref(dtk))
+                if self.acoustic:
+                    L.compute_kernels_acoustic_cuda_(MP, C.byref(dtk))
         L.sf_history_reserve(MP, C.c_int(0))
         nloc = self.db.nspec * 125
         rk, mk, kk = (np.zeros(nloc, np.float32) for _ in range(3))
         L.transfer_kernels_el_to_host_(MP, abi.ptr(rk), abi.ptr(mk), abi.ptr(kk), None, C.byref(C.c_int(self.db.nspec)))
         shp = (self.db.nspec, 5, 5, 5)
+        if self.acoustic:
+            ra, ka = np.zeros(nloc, np.float32), np.zeros(nloc, np.float32)
+            L.transfer_kernels_ac_to_host_(MP, abi.ptr(ra), abi.ptr(ka), C.byref(C.c_int(self.db.nspec)))
+            self.acoustic_kernels = (ra.reshape(shp), ka.reshape(shp))
         return rk.reshape(shp), mk.reshape(shp), kk.reshape(shp)
 
     # -- helpers
@@ -237,6 +309,10 @@ class KernelSolver(GPUSolver):
             self.L.transfer_b_fields_to_device_(A.i(n3), abi.ptr(z), abi.ptr(z), abi.ptr(z), self.MP)
         else:
             self.L.transfer_fields_el_to_device_(A.i(n3), abi.ptr(z), abi.ptr(z), abi.ptr(z), self.MP)
+        if self.acoustic:
+            n = A.i(self.db.nglob)
+            xfer = self.L.transfer_b_fields_ac_to_device_ if foa == 3 else self.L.transfer_fields_ac_to_device_
+            xfer(n, abi.ptr(z), abi.ptr(z), abi.ptr(z), self.MP)
 
     def _zero_kernels(self):
         self.L.sf_zero_kernels(self.MP)

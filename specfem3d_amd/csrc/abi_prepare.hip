@@ -953,6 +953,11 @@ extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
   } else if (std::strcmp(name, "att_variant") == 0) {
     if (value < 0 || value > 2) fatal("sf_set_tuning", "att_variant must be 0, 1 or 2");
     m->att_variant = value;
+  } else if (std::strcmp(name, "acoustic_kernel_field") == 0) {
+    // which adjoint array compute_kernels_acoustic_cuda pairs with the reconstructed field:
+    // 0 = potential_acoustic (compute_kernels.f90:269 and :325), 1 = potential_dot_dot_acoustic
+    if (value < 0 || value > 1) fatal("sf_set_tuning", "acoustic_kernel_field must be 0 or 1");
+    m->ac_kernel_dd = value;
   } else {
     fatal("sf_set_tuning", "unknown tuning switch");
   }

@@ -529,8 +529,15 @@ extern "C" void FC_FUNC_(compute_strain_cuda, COMPUTE_strain_CUDA)(long* Mesh_po
 
 extern "C" void FC_FUNC_(compute_kernels_acoustic_cuda, COMPUTE_KERNELS_ACOUSTIC_CUDA)(long* Mesh_pointer,
                                                                                      realw* deltat_f) {
-  (void)Mesh_pointer; (void)deltat_f;
-  fatal("compute_kernels_acoustic_cuda", "acoustic sensitivity kernels are not supported");
+  Mesh* m = from_handle(Mesh_pointer);
+  if (!m->ac[0].pot || !m->ac[1].pot) fatal("compute_kernels_acoustic_cuda", "acoustic fields not prepared");
+  if (!m->rho_ac_kl) fatal("compute_kernels_acoustic_cuda", "acoustic kernel arrays not prepared");
+  flush_ac(m, 0);
+  flush_ac(m, 1);
+  launch_compute_kernels_ac(m->stream, m->nspec_acoustic, m->acoustic_elems, m->ibool, m->irregular, m->xix_regular,
+                            m->jac, (int64_t)m->nspec_irregular * 125, m->rho,
+                            m->ac_kernel_dd ? m->ac[0].potdd : m->ac[0].pot, m->ac[1].pot,
+                            m->ac[1].potdd, *deltat_f, m->rho_ac_kl, m->kappa_ac_kl);
 }
 
 // ---------------------------------------------------------------- norms (check_fields_cuda.cu:375-470)

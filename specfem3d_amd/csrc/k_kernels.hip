@@ -29,6 +29,70 @@ __global__ void k_compute_kernels_el(int nelem, const int* __restrict__ elems, c
   kappa_kl[ep] = kappa_kl[ep] + deltat * ((realw)9 * etr3[ep] * btr3[ep]);
 }
 
+// compute_kernels.f90:257-337 (compute_kernels_ac), one workgroup per fluid element:
+//   rho_ac_kl   += dt (grad chi_adj . grad chi_b)   (gradients as compute_gradient_in_acoustic:
+//                 contractions, inverse Jacobian, 1/rho, then times rho again)
+//   kappa_ac_kl += dt chi_adj chi_b''
+__device__ __forceinline__ void ac_gradient(const realw* s, const realw* hp, int i, int j, int k, int irr,
+                                            const realw* jac, int64_t P, int64_t q, realw xr, realw rho_invl,
+                                            realw* g) {
+  const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
+  realw t1 = hp[i * 5] * s[b1], t2 = hp[j * 5] * s[b2], t3 = hp[k * 5] * s[b3];
+#pragma unroll
+  for (int l = 1; l < 5; l++) {
+    t1 = t1 + hp[i * 5 + l] * s[b1 + l];
+    t2 = t2 + hp[j * 5 + l] * s[b2 + 5 * l];
+    t3 = t3 + hp[k * 5 + l] * s[b3 + 25 * l];
+  }
+  if (irr) {
+    const realw xixl = jac[q], xiyl = jac[P + q], xizl = jac[2 * P + q], etaxl = jac[3 * P + q];
+    const realw etayl = jac[4 * P + q], etazl = jac[5 * P + q], gaxl = jac[6 * P + q], gayl = jac[7 * P + q];
+    const realw gazl = jac[8 * P + q];
+    g[0] = (t1 * xixl + t2 * etaxl + t3 * gaxl) * rho_invl;
+    g[1] = (t1 * xiyl + t2 * etayl + t3 * gayl) * rho_invl;
+    g[2] = (t1 * xizl + t2 * etazl + t3 * gazl) * rho_invl;
+  } else {
+    g[0] = t1 * xr * rho_invl;
+    g[1] = t2 * xr * rho_invl;
+    g[2] = t3 * xr * rho_invl;
+  }
+}
+
+__global__ void __launch_bounds__(128) k_compute_kernels_ac(int nelem, const int* __restrict__ elems,
+                                                            const int* __restrict__ ibool,
+                                                            const int* __restrict__ irregular, realw xix_regular,
+                                                            const realw* __restrict__ jac, int64_t jac_plane,
+                                                            const realw* __restrict__ rhostore,
+                                                            const realw* __restrict__ pot,
+                                                            const realw* __restrict__ b_pot,
+                                                            const realw* __restrict__ b_potdd, GLLk G, realw deltat,
+                                                            realw* rho_ac_kl, realw* kappa_ac_kl) {
+  __shared__ realw sb[125], sa[125], hp[25];
+  const int p = threadIdx.x;
+  if (p < 25) hp[p] = G.hp[p];
+  const int e = elems[blockIdx.x];
+  const int64_t ep = (int64_t)e * 125 + (p < 125 ? p : 124);
+  const int g = ibool[ep];
+  if (p < 125) {
+    sb[p] = b_pot[g];
+    sa[p] = pot[g];
+  }
+  __syncthreads();
+  if (p >= 125) return;
+  const int k = p / 25, j = (p / 5) % 5, i = p % 5;
+  const int irr = irregular[e];
+  const int64_t q = (int64_t)(irr > 0 ? irr - 1 : 0) * 125 + p;
+  const realw rhol = rhostore[ep];
+  const realw rho_invl = (realw)1 / rhol;
+  realw bg[3], ag[3];
+  ac_gradient(sb, hp, i, j, k, irr, jac, jac_plane, q, xix_regular, rho_invl, bg);
+  ac_gradient(sa, hp, i, j, k, irr, jac, jac_plane, q, xix_regular, rho_invl, ag);
+  const realw a1 = rhol * ag[0], a2 = rhol * ag[1], a3 = rhol * ag[2];
+  const realw b1 = rhol * bg[0], b2 = rhol * bg[1], b3 = rhol * bg[2];
+  rho_ac_kl[ep] = rho_ac_kl[ep] + deltat * (a1 * b1 + a2 * b2 + a3 * b3);
+  kappa_ac_kl[ep] = kappa_ac_kl[ep] + deltat * sa[p] * b_potdd[g];
+}
+
 // ordered assembly: for each global point, subtract its element contributions in increasing
 // element order (the order of the reference's serial element loop). ptr/idx is a CSR list of
 // local slots e*125+p per global point, sorted by element.
@@ -79,6 +143,17 @@ void launch_compute_kernels_el(hipStream_t s, int nelem, const int* elems, const
   hipLaunchKernelGGL(k_compute_kernels_el, sf_grid((int64_t)nelem * 125, 256), dim3(256), 0, s, nelem, elems, ibool,
                      accel, b_displ, eps, eps_tr3, b_eps, b_eps_tr3, nloc, deltat, rho_kl, mu_kl, kappa_kl);
   SF_LAUNCH_CHECK("k_compute_kernels_el");
+}
+
+void launch_compute_kernels_ac(hipStream_t s, int nelem, const int* elems, const int* ibool, const int* irregular,
+                               realw xix_regular, const realw* jac, int64_t jac_plane, const realw* rhostore,
+                               const realw* pot, const realw* b_pot, const realw* b_potdd, realw deltat,
+                               realw* rho_ac_kl, realw* kappa_ac_kl) {
+  if (nelem <= 0) return;
+  hipLaunchKernelGGL(k_compute_kernels_ac, dim3((unsigned)nelem), dim3(128), 0, s, nelem, elems, ibool, irregular,
+                     xix_regular, jac, jac_plane, rhostore, pot, b_pot, b_potdd, g_gll_host, deltat, rho_ac_kl,
+                     kappa_ac_kl);
+  SF_LAUNCH_CHECK("k_compute_kernels_ac");
 }
 
 void launch_ordered_gather(hipStream_t s, int nglob, int ncomp, const int* ptr, const int* idx, const realw* local,

@@ -122,6 +122,10 @@ void launch_compute_kernels_el(hipStream_t s, int nelem, const int* elems, const
                                const realw* b_displ, const realw* eps, const realw* eps_tr3, const realw* b_eps,
                                const realw* b_eps_tr3, int64_t nloc, realw deltat, realw* rho_kl, realw* mu_kl,
                                realw* kappa_kl);
+void launch_compute_kernels_ac(hipStream_t s, int nelem, const int* elems, const int* ibool, const int* irregular,
+                               realw xix_regular, const realw* jac, int64_t jac_plane, const realw* rhostore,
+                               const realw* pot, const realw* b_pot, const realw* b_potdd, realw deltat,
+                               realw* rho_ac_kl, realw* kappa_ac_kl);
 void launch_max_norm(hipStream_t s, int64_t npts, int ncomp, const realw* f, realw* out);
 
 }  // namespace sf

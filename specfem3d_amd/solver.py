@@ -143,6 +143,9 @@ class GPUSolver:
                 A.ia(ph), A.i(0), A.i(fr.n), A.ia(fr.ispec + 1), A.ia(fr.ijk.ravel() + 1), A.i(0), A.fa([0]),
                 A.i(int(self.elastic)), A.i(cp.n), A.ia(cp.ispec + 1), A.ia(cp.ijk.ravel() + 1),
                 A.fa(cp.normal.ravel()), A.fa(cp.jacobian2Dw.ravel()), A.i(0), A.i(0), A.ia([0]))
+            if self.sim == 3:
+                # reconstructed potentials and the acoustic kernel arrays
+                L.prepare_fields_acoustic_adj_dev_(self.MP, A.i(0))
         if self.elastic:
             n, ph = _fortran_phase_list(db.elastic_elements())
             att = self.att

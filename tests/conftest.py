@@ -48,6 +48,29 @@ def small_elastic_box(n=4, warp=None, stacey=True, dt=0.004, nrec=6, seed=0):
     return db, src, rec
 
 
+def small_coupled_box(n=4, dt=0.004, nrec=6, seed=0, warp=None):
+    """Box with a fluid top layer (free surface on top) over a gradient solid, Stacey on the sides and
+    bottom, one CMT source and receivers in the solid (test helper)."""
+    from specfem3d_amd.mesh.box import build_box, gradient_grf_model, homogeneous
+    from specfem3d_amd import sources as S
+    L = 500.0 * n
+    ext = (0.0, L, 0.0, L, -L, 0.0)
+    db = build_box(n, n, n, ext, material=gradient_grf_model(ext, seed=seed, corr_len=700.0),
+                   acoustic_top_layers=1, acoustic_material=homogeneous(1020.0, 1500.0, 0.0),
+                   stacey_faces=(1, 2, 3, 4, 5), warp=warp)
+    db.finalize(dt, stacey=True)
+    rng = np.random.default_rng(seed)
+    cmt = S.CMTSource(x=float(rng.uniform(0.3, 0.7) * L), y=float(rng.uniform(0.3, 0.7) * L),
+                      depth_km=float(rng.uniform(0.45, 0.75) * L / 1000.0), tshift=0.0, hdur=4.0,
+                      Mxx=1e13, Myy=-2e13, Mzz=0.7e13, Mxy=0.3e13, Mxz=-0.5e13, Myz=0.2e13)
+    src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
+    st = [S.Station(f"R{i}", "T", float(rng.uniform(0.1, 0.9) * L), float(rng.uniform(0.1, 0.9) * L), 0.0,
+                    float(rng.uniform(0.35, 0.6) * L)) for i in range(nrec)]
+    rec = S.setup_receivers(db, st)
+    assert db.ispec_is_elastic[src.ispec].all() and db.ispec_is_elastic[rec.ispec].all()
+    return db, src, rec
+
+
 def smooth_warp(L, amp=0.06):
     """Smooth interior displacement of the anchors (keeps the box faces planar): irregular elements."""
     def w(x, y, z):
