@@ -73,16 +73,32 @@ def stiffness_bytes_per_launch(db, attenuation=None) -> float:
     return float(per_glob * nglob_el + per_loc * nloc_el + 40 * nirr_el)
 
 
-def batch_stiffness_bytes(db, nshots: int) -> float:
+def batch_stiffness_bytes(db, nshots: int, fused: bool = False) -> float:
     """Algorithmic HBM bytes of one batched stiffness pass with every shot live: per shot and global
     point u read (12 B) and accel read + write (24 B); per local point, shared by the shots, ibool +
-    kappa + mu (12 B) and 40 B of Jacobian terms in irregular elements."""
+    kappa + mu (12 B) and 40 B of Jacobian terms in irregular elements. With the Newmark update fused
+    into the pass (assembly 3) the pass is the whole step: u, v, accel each read and written
+    (72 B per shot and point) plus the inverse mass (12 B per point)."""
     import numpy as np
     el = db.ispec_is_elastic
     nloc = int(el.sum()) * 125
     nglob_el = int(np.unique(db.ibool[el]).size)
     nirr = int((db.irregular_element_number[el] > 0).sum()) * 125
-    return float(36 * nglob_el * nshots + 12 * nloc + 40 * nirr)
+    per_shot = 72 if fused else 36
+    return float(per_shot * nglob_el * nshots + 12 * nloc + 40 * nirr + (12 * nglob_el if fused else 0))
+
+
+def batch_step_bytes(db, shot_steps: int, nsteps: int) -> float:
+    """Algorithmic bytes of a whole batched run (the unique-traffic model: every array read and written
+    once per step at its natural size): 72 B per live shot and point per step for u, v, accel, plus per
+    step the shared ibool + kappa + mu (12 B per local point), irregular Jacobian terms (40 B) and the
+    inverse mass (12 B per point)."""
+    import numpy as np
+    el = db.ispec_is_elastic
+    nloc = int(el.sum()) * 125
+    nglob_el = int(np.unique(db.ibool[el]).size)
+    nirr = int((db.irregular_element_number[el] > 0).sum()) * 125
+    return float(72 * nglob_el * shot_steps + (12 * nloc + 40 * nirr + 12 * nglob_el) * nsteps)
 
 
 def batched_measurement(args) -> dict:
@@ -113,8 +129,9 @@ def batched_measurement(args) -> dict:
     finite = all(bool(np.isfinite(b.seismograms(i)).all()) for i in range(len(w.shots)))
     b.close()
     value = db.nglob * float(sum(nsteps)) / wall
-    kbytes = batch_stiffness_bytes(db, len(w.shots))
+    kbytes = batch_stiffness_bytes(db, len(w.shots), fused=choice["assembly"] == 3)
     ach = kbytes / (kms / 1e3) / 1e9
+    run_gbs = batch_step_bytes(db, sum(nsteps), max(nsteps)) / wall / 1e9
     # plain single-shot path on the same mesh (first shot), timed the same way
     src, rec, n = w.shots[0]
     ps = min(args.plain_steps, n)
@@ -134,7 +151,12 @@ def batched_measurement(args) -> dict:
     return {"workload": w.name, "value": value, "unit": "GLL-point updates/s", "wall_s": round(wall, 3),
             "shot_steps": int(sum(nsteps)), "plain_single_shot": plain, "ratio_vs_plain": value / plain,
             "autotuned": choice, "seismograms_finite": finite,
-            "roofline": {"bound": "hbm", "kernel": "k_elastic_batch (all shots live)", "achieved": round(ach, 1),
+            "run_roofline": {"bound": "hbm", "achieved": round(run_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(run_gbs / HBM_PEAK_GBS, 4),
+                             "bytes_per_update": round(batch_step_bytes(db, sum(nsteps), max(nsteps))
+                                                       / (db.nglob * float(sum(nsteps))), 2)},
+            "roofline": {"bound": "hbm", "kernel": "k_elastic_batch (all shots live"
+                         + (", Newmark update fused)" if choice["assembly"] == 3 else ")"), "achieved": round(ach, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                          "bytes_per_launch": kbytes, "avg_pass_ms": round(kms, 4), "step_ms": round(sms, 4)}}
 

@@ -274,6 +274,9 @@ void sf_batch_add_source(long batch, int shot, int ispec, const realw* sourcearr
 void sf_batch_add_receiver(long batch, int shot, int ispec, const double* hxi, const double* heta,
                            const double* hgamma, const double* nu);
 void sf_batch_finalize(long batch);
+/* shots per workgroup 4 or 8; assembly 0 = atomic scatter, 2 = one launch per element colour,
+   3 = coloured with the Newmark update fused into the colour launches (same results as 2); only
+   at step 1 (a new batch or after sf_batch_reset) */
 void sf_batch_set_options(long batch, int shots_per_group, int assembly);
 void sf_batch_get_options(long batch, int* shots_per_group, int* assembly);
 double sf_batch_autotune(long batch, int trial_steps);

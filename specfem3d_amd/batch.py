@@ -21,7 +21,9 @@ from .solver import GPUSolver
 class BatchSolver:
     def __init__(self, db, shots, exact: bool = False, device: int = 0, shots_per_group: int | None = None,
                  assembly: int | None = None):
-        """``shots``: list of (SourceSet, ReceiverSet, nstep), one per shot."""
+        """``shots``: list of (SourceSet, ReceiverSet, nstep), one per shot. ``assembly``: 0 atomic
+        scatter, 2 one launch per element colour, 3 coloured with the Newmark update fused into the
+        colour launches (same bits as 2); None keeps the library default until ``autotune``."""
         if not shots:
             raise ValueError("no shots")
         self.db = db

@@ -640,6 +640,7 @@ static void build_colors(Mesh* m, const std::vector<int>& elems, int*& d_list, i
   std::vector<int> col(elems.size());
   ncolor = color_elements(m->nglob, (int)elems.size(), H->ibool.data(), elems.data(), col.data());
   if (ncolor < 0) fatal("build_colors", "element colouring needs more than 32 colours");
+  m->color_version++;
   for (int c = 0; c <= 32; c++) off[c] = 0;
   for (size_t q = 0; q < elems.size(); q++) off[col[q] + 1]++;
   for (int c = 0; c < 32; c++) off[c + 1] += off[c];

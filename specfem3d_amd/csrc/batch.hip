@@ -9,8 +9,10 @@
 //     a prefix of the row and finished shots cost no memory traffic;
 //   * one step = fused corrector/predictor, Stacey rounds, source rounds, stiffness (one launch per
 //     element colour with plain read-modify-writes, or one atomic launch), receivers, and a
-//     one-lane kernel that advances the device step counter. Every kernel reads the counter, so
-//     a group of steps is captured once into a hipGraph and replayed;
+//     one-lane kernel that advances the device step counter. Assembly mode 3 instead records the
+//     receivers first and folds the corrector/predictor into the coloured stiffness launches
+//     (k_batch.hip), which removes the separate pass over u, v, a. Every kernel reads the counter,
+//     so a group of steps is captured once into a hipGraph and replayed;
 //   * the per-point arithmetic and the order of every sum equal the single-shot path (same
 //     kernels' expressions, same colour order, same face rounds), so each shot of a batch is
 //     bit-identical to its single-shot run with the same options.
@@ -29,7 +31,7 @@ void ensure_element_colors(Mesh* m);
 struct Batch {
   Mesh* m = nullptr;
   int nshots = 0, S = 0;
-  int sc = 4, mode = 2;          // shots per workgroup, assembly (0 atomic, 2 coloured)
+  int sc = 4, mode = 2;          // shots per workgroup, assembly (0 atomic, 2 coloured, 3 fused)
   int max_nstep = 0;
   realw dt = 0.f, dt2 = 0.f, dtsq2 = 0.f;  // deltat, deltatover2, deltatsqover2
   std::vector<int> nstep_user;   // per user shot
@@ -38,6 +40,8 @@ struct Batch {
   int* d_it = nullptr;
   int h_it = 1;                  // host mirror of the step counter
   realw *u = nullptr, *v = nullptr, *a = nullptr;
+  unsigned char* flags = nullptr;  // fused mode: BF_* per local point (first/last colour, cleared)
+  int flags_version = -1;        // Mesh::color_version the flags were built for (-1: stale)
   // sources: (source, point) entries sorted into rounds (the k-th source of each shot)
   struct Src { int shot, ispec; std::vector<realw> arr, stf; };
   std::vector<Src> srcs;
@@ -245,6 +249,57 @@ static void zero_state(Batch* b) {
   SF_CHECK(hipStreamSynchronize(m->stream));
 }
 
+// Point flags of the fused mode: for each global point, the element of its lowest colour writes
+// first (unless Stacey or a source adds to the point before the colours), the element of its
+// highest colour finishes the Newmark update and clears `a` where those passes add.
+static void build_point_flags(Batch* b) {
+  Mesh* m = b->m;
+  ensure_element_colors(m);
+  const int ne = m->nspec_elastic;
+  std::vector<int> lst(ne), hib((size_t)m->nspec * 125);
+  SF_CHECK(hipMemcpy(lst.data(), m->el_color_elems, sizeof(int) * ne, hipMemcpyDeviceToHost));
+  SF_CHECK(hipMemcpy(hib.data(), m->ibool, sizeof(int) * hib.size(), hipMemcpyDeviceToHost));
+  std::vector<unsigned char> forced((size_t)m->nglob, 0);
+  if (m->stacey && m->abs_el.n > 0) {
+    std::vector<int> ig(m->abs_el.n);
+    SF_CHECK(hipMemcpy(ig.data(), m->abs_el.iglob, sizeof(int) * ig.size(), hipMemcpyDeviceToHost));
+    for (int g : ig) forced[g] = 1;
+  }
+  for (const Batch::Src& q : b->srcs)
+    for (int p = 0; p < 125; p++) forced[hib[(size_t)q.ispec * 125 + p]] = 1;
+  std::vector<int> cmin((size_t)m->nglob, 1 << 30), cmax((size_t)m->nglob, -1), col(ne);
+  for (int c = 0; c < m->el_ncolor; c++)
+    for (int q = m->el_color_off[c]; q < m->el_color_off[c + 1]; q++) {
+      const int e = lst[q];
+      col[q] = c;
+      for (int p = 0; p < 125; p++) {
+        const int g = hib[(size_t)e * 125 + p];
+        cmin[g] = std::min(cmin[g], c);
+        cmax[g] = std::max(cmax[g], c);
+      }
+    }
+  std::vector<unsigned char> fl((size_t)m->nspec * 125, 0);
+  std::vector<int> nlast((size_t)m->nglob, 0);
+  for (int q = 0; q < ne; q++) {
+    const int e = lst[q], c = col[q];
+    for (int p = 0; p < 125; p++) {
+      const int g = hib[(size_t)e * 125 + p];
+      unsigned char f = forced[g] ? BF_ZERO : 0;
+      if (c == cmin[g] && !forced[g]) f |= BF_FIRST;
+      if (c == cmax[g]) {
+        f |= BF_LAST;
+        nlast[g]++;
+      }
+      fl[(size_t)e * 125 + p] = f;
+    }
+  }
+  for (int g = 0; g < m->nglob; g++)
+    if (cmax[g] >= 0 && nlast[g] != 1) fatal("sf_batch", "element colouring shares a point inside a colour");
+  bfree(b->flags);
+  b->flags = bupload(fl.data(), (int64_t)fl.size());
+  b->flags_version = m->color_version;
+}
+
 static void enqueue_stiffness(Batch* b, hipStream_t st) {
   Mesh* m = b->m;
   BatchArgs A;
@@ -262,11 +317,17 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
   A.a = b->a;
   A.nstep = b->d_nstep;
   A.it = b->d_it;
-  if (b->mode == 2) {
+  A.v = b->v;
+  A.rmass = m->rmass;
+  A.flags = b->flags;
+  A.dt = b->dt;
+  A.dt2 = b->dt2;
+  A.dtsq2 = b->dtsq2;
+  if (b->mode == 2 || b->mode == 3) {
     for (int c = 0; c < m->el_ncolor; c++) {
       A.elems = m->el_color_elems + m->el_color_off[c];
       A.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
-      launch_elastic_batch(st, A, b->sc, 2, m->exact != 0);
+      launch_elastic_batch(st, A, b->sc, b->mode, m->exact != 0);
     }
   } else {
     A.elems = m->elastic_elems;
@@ -276,14 +337,26 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
 }
 
 // one time step; `first` selects the plain predictor (no pending corrector)
+static void enqueue_seismo(Batch* b, hipStream_t st) {
+  if (b->nrec <= 0) return;
+  hipLaunchKernelGGL(k_batch_seismo, dim3(b->nrec), dim3(64), 0, st, b->nrec, b->S, b->rec_slot, b->rec_ig,
+                     b->rec_h, b->rec_nu, b->d_nstep, b->d_it, b->u, b->seis);
+  SF_LAUNCH_CHECK("k_batch_seismo");
+}
+
 static void enqueue_step(Batch* b, hipStream_t st, bool first) {
   Mesh* m = b->m;
-  const int64_t nrow = 3 * (int64_t)m->nglob;
-  const realw dt = b->dt, dt2 = b->dt2, dtsq2 = b->dtsq2;
-  const int rpb = 256 / (b->S / 4);
-  hipLaunchKernelGGL(k_batch_update, sf_grid((nrow + rpb - 1) / rpb, 1), dim3(256), 0, st, nrow, b->S, rpb,
-                     b->d_nstep, b->d_it, b->u, b->v, b->a, m->rmass, dt2, dt, dt2, dtsq2, first ? 1 : 0);
-  SF_LAUNCH_CHECK("k_batch_update");
+  const bool fused = b->mode == 3;
+  if (fused) {
+    enqueue_seismo(b, st);  // u already holds this step's predicted displacement
+  } else {
+    const int64_t nrow = 3 * (int64_t)m->nglob;
+    const realw dt = b->dt, dt2 = b->dt2, dtsq2 = b->dtsq2;
+    const int rpb = 256 / (b->S / 4);
+    hipLaunchKernelGGL(k_batch_update, sf_grid((nrow + rpb - 1) / rpb, 1), dim3(256), 0, st, nrow, b->S, rpb,
+                       b->d_nstep, b->d_it, b->u, b->v, b->a, m->rmass, dt2, dt, dt2, dtsq2, first ? 1 : 0);
+    SF_LAUNCH_CHECK("k_batch_update");
+  }
   if (m->stacey) {
     const FacePoints& f = m->abs_el;
     for (int r = 0; r < f.nrounds; r++) {
@@ -311,11 +384,7 @@ static void enqueue_step(Batch* b, hipStream_t st, bool first) {
     SF_CHECK(hipEventRecord(b->ev[b->ev_used + 1], st));
     b->ev_used += 2;
   }
-  if (b->nrec > 0) {
-    hipLaunchKernelGGL(k_batch_seismo, dim3(b->nrec), dim3(64), 0, st, b->nrec, b->S, b->rec_slot, b->rec_ig,
-                       b->rec_h, b->rec_nu, b->d_nstep, b->d_it, b->u, b->seis);
-    SF_LAUNCH_CHECK("k_batch_seismo");
-  }
+  if (!fused) enqueue_seismo(b, st);
   hipLaunchKernelGGL(k_batch_tick, dim3(1), dim3(64), 0, st, b->d_it);
   SF_LAUNCH_CHECK("k_batch_tick");
 }
@@ -324,6 +393,10 @@ static void enqueue_step(Batch* b, hipStream_t st, bool first) {
 static void run_steps(Batch* b, int n, int graph_steps) {
   Mesh* m = b->m;
   n = std::min(n, b->max_nstep - b->h_it + 1);
+  if (b->mode == 3 && (!b->flags || b->flags_version != m->color_version)) {
+    drop_graph(b);
+    build_point_flags(b);
+  }
   while (n > 0) {
     if (b->h_it == 1 || graph_steps <= 1 || n < graph_steps) {
       enqueue_step(b, m->stream, b->h_it == 1);
@@ -494,16 +567,21 @@ extern "C" void sf_batch_finalize(long h) {
     b->rec_nu = bupload(nu.data(), (int64_t)nu.size());
     b->seis = balloc<realw>((int64_t)b->max_nstep * std::max(1, b->nrec) * 3);
   }
+  b->flags_version = -1;  // sources changed: the fused mode's cleared points change
   b->ready = true;
   zero_state(b);
 }
 
-// shots per workgroup (4 or 8; divides the padded shot count) and assembly (0 atomic, 2 coloured)
+// shots per workgroup (4 or 8; divides the padded shot count) and assembly (0 atomic, 2 coloured,
+// 3 coloured with the Newmark update fused). The modes keep different intermediate states, so the
+// batch must be at step 1 (new, or after sf_batch_reset).
 extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) {
   Batch* b = batch_of(h);
   if (shots_per_group != 4 && shots_per_group != 8)
     fatal("sf_batch_set_options", "shots per workgroup must be 4 or 8");
-  if (assembly != 0 && assembly != 2) fatal("sf_batch_set_options", "assembly must be 0 (atomic) or 2 (coloured)");
+  if (assembly != 0 && assembly != 2 && assembly != 3)
+    fatal("sf_batch_set_options", "assembly must be 0 (atomic), 2 (coloured) or 3 (coloured, fused update)");
+  if (b->h_it != 1) fatal("sf_batch_set_options", "options change only at step 1 (call sf_batch_reset first)");
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   drop_graph(b);
   b->sc = shots_per_group;
@@ -606,7 +684,7 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
   if (!b->ready) fatal("sf_batch_autotune", "sf_batch_finalize first");
   double best = 1e300;
   int best_sc = b->sc, best_mode = b->mode;
-  for (int mode : {2, 0})
+  for (int mode : {3, 2, 0})
     for (int sc : {4, 8}) {
       if (b->S % sc) continue;
       b->sc = sc;
@@ -634,6 +712,7 @@ extern "C" void sf_batch_destroy(long h) {
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   drop_graph(b);
   bfree(b->d_nstep); bfree(b->d_it); bfree(b->u); bfree(b->v); bfree(b->a);
+  bfree(b->flags);
   bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
   bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);
   delete b;

@@ -156,6 +156,24 @@ __device__ __forceinline__ void tile_divergence(const realw* sT, int plane, cons
   }
 }
 
+// one component cc of tile_divergence (same expressions)
+template <int PS>
+__device__ __forceinline__ realw tile_divergence_c(const realw* sT, int plane, const realw* hpw, const realw* wxy,
+                                                   int i, int j, int k, int cc) {
+  const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
+  const realw *s1 = sT + cc * plane, *s2 = sT + (3 + cc) * plane, *s3 = sT + (6 + cc) * plane;
+  realw n1 = hpw[i] * s1[b1 * PS];
+  realw n2 = hpw[j] * s2[b2 * PS];
+  realw n3 = hpw[k] * s3[b3 * PS];
+#pragma unroll
+  for (int l = 1; l < 5; l++) {
+    n1 = n1 + hpw[l * 5 + i] * s1[(b1 + l) * PS];
+    n2 = n2 + hpw[l * 5 + j] * s2[(b2 + 5 * l) * PS];
+    n3 = n3 + hpw[l * 5 + k] * s3[(b3 + 25 * l) * PS];
+  }
+  return wxy[j * 5 + k] * n1 + wxy[i * 5 + k] * n2 + wxy[i * 5 + j] * n3;
+}
+
 // stage the 75 GLL coefficients of G in LDS (lanes 0..74 of the workgroup; needs a barrier before use)
 __device__ __forceinline__ void stage_gll(realw* sG, const GLLk& G, int lane) {
   if (lane < 25) sG[lane] = G.hp[lane];

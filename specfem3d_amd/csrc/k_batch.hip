@@ -12,6 +12,20 @@
 // lengths cost nothing after a shot ends). `it` is read from device memory so a captured graph
 // of the step can be replayed.
 //
+// MODE 3 fuses the Newmark update into the coloured stiffness launches. Within a colour no two
+// elements share a point, and the colours of a step run in order, so for every global point the
+// element of its highest colour (flag BF_LAST) holds the last contribution: it finishes the
+// point there -- a *= rmass, v += dt/2 a (corrector), then the next step's predictor
+// u += dt v + dt^2/2 a, v += dt/2 a -- because every other element containing the point has
+// already gathered u in an earlier colour. The element of the lowest colour (BF_FIRST) writes its
+// contribution instead of adding it, so `a` is never cleared except at points that the Stacey and
+// source passes add to before the colours (BF_ZERO: cleared by the finisher, never BF_FIRST). A
+// point of a single element is gathered, computed and finished in one workgroup with no `a`
+// traffic. The sums run in the same order as the separate update path (Stacey, sources, colours
+// in order, then the corrector), so the results are the same bits. At a shot's last step the
+// finisher applies the corrector only, leaving the state of the last step as the single-shot run
+// does.
+//
 // Compiled twice like k_elastic.hip: SF_EXACT=1 without FMA contraction, SF_EXACT=0 with it.
 #include "elastic_point.h"
 
@@ -28,6 +42,21 @@
 
 namespace sf {
 namespace SF_VNS {
+
+// corrector of the finished step, then (cont) the predictor of the next one; never contracted, so
+// the fast build rounds like the separate update kernel (k_update.hip)
+__device__ __forceinline__ void newmark_finish(realw& u, realw& v, realw& a, realw M, realw dt, realw dt2, realw dtsq2,
+                                               bool cont) {
+#pragma clang fp contract(off)
+  a = a * M;
+  v = v + dt2 * a;
+  if (cont) {
+    u = u + dt * v + dtsq2 * a;
+    v = v + dt2 * a;
+  }
+}
+
+__device__ __forceinline__ realw& q4(float4& x, int r) { return r == 0 ? x.x : r == 1 ? x.y : r == 2 ? x.z : x.w; }
 
 __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
   const int q = nb >> 3, r = nb & 7, x = b & 7;
@@ -104,26 +133,53 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   __syncthreads();
   if (!active || !live0) return;
 
-  realw f[4][3];
-#pragma unroll
-  for (int r = 0; r < 4; r++) tile_divergence<SC>(sT + ts + r, PL, sG + 25, sG + 50, i, j, k, f[r]);
+  // one force component at a time for the lane's four shots (keeps few values live)
   realw* a = B.a + row;
-  if (MODE == 2) {
+  const realw* hpw = sG + 25;
+  const realw* wxy = sG + 50;
+  unsigned fl = 0;
+  bool cont[4], ends = false;
+  if (MODE == 3) {
+    fl = B.flags[(int64_t)e * 125 + p];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      float4 A = *reinterpret_cast<const float4*>(a + c * S);
-      if (live[0]) A.x = A.x + (-f[0][c]);
-      if (live[1]) A.y = A.y + (-f[1][c]);
-      if (live[2]) A.z = A.z + (-f[2][c]);
-      if (live[3]) A.w = A.w + (-f[3][c]);
-      *reinterpret_cast<float4*>(a + c * S) = A;
+    for (int r = 0; r < 4; r++) {
+      cont[r] = live[r] && B.nstep[sq + r] > it;
+      ends = ends || (live[r] && !cont[r]);
     }
-  } else {
+  }
 #pragma unroll
-    for (int c = 0; c < 3; c++)
+  for (int c = 0; c < 3; c++) {
+    realw f[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) f[r] = tile_divergence_c<SC>(sT + ts + r, PL, hpw, wxy, i, j, k, c);
+    if (MODE == 0) {
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        if (live[r]) atomicAdd(a + c * S + r, -f[r][c]);
+        if (live[r]) atomicAdd(a + c * S + r, -f[r]);
+      continue;
+    }
+    const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
+    float4 A = (!first || !live[3]) ? *reinterpret_cast<const float4*>(a + c * S) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (live[r]) q4(A, r) = (first ? 0.f : q4(A, r)) + (-f[r]);
+    if (MODE == 2 || !(fl & BF_LAST)) {
+      *reinterpret_cast<float4*>(a + c * S) = A;
+      continue;
+    }
+    // fused mode, last colour of this point: finish the Newmark update of component c
+    const realw M = B.rmass[(int64_t)g * 3 + c];
+    float4 V = *reinterpret_cast<const float4*>(B.v + row + c * S);
+    float4 U = *reinterpret_cast<const float4*>(sU + (c * 125 + p) * SC + ts);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (!live[r]) continue;
+      newmark_finish(q4(U, r), q4(V, r), q4(A, r), M, B.dt, B.dt2, B.dtsq2, cont[r]);
+      if (cont[r]) q4(A, r) = 0.f;
+    }
+    if (cont[0]) *reinterpret_cast<float4*>(B.u + row + c * S) = U;
+    *reinterpret_cast<float4*>(B.v + row + c * S) = V;
+    if ((fl & BF_ZERO) || ends) *reinterpret_cast<float4*>(a + c * S) = A;
   }
 }
 
@@ -141,7 +197,10 @@ void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int m
   if (sc == 4) SF_KB(1, M);        \
   else if (sc == 8) SF_KB(2, M);   \
   else fatal("launch_elastic_batch", "shots per workgroup must be 4 or 8");
-  if (mode == 2) {
+  if (mode == 3) {
+    if (!a.v || !a.rmass || !a.flags) fatal("launch_elastic_batch", "fused update needs v, rmass and point flags");
+    SF_KBM(3)
+  } else if (mode == 2) {
     SF_KBM(2)
   } else {
     SF_KBM(0)

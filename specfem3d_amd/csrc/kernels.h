@@ -71,12 +71,19 @@ struct BatchArgs {
   const realw* jac;
   int64_t jac_plane;
   const float2* kappamu;
-  const realw* u;
+  realw* u;
   realw* a;
   const int* nstep;        // per shot slot, decreasing (padding slots 0)
   const int* it;           // device step counter, 1-based current step
+  // fused Newmark update (mode 3 only)
+  realw* v;
+  const realw* rmass;      // inverse mass, 3 per global point
+  const unsigned char* flags;  // per local point e*125+p: BF_FIRST | BF_LAST | BF_ZERO
+  realw dt, dt2, dtsq2;
 };
-// sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour)
+enum : unsigned char { BF_FIRST = 1, BF_LAST = 2, BF_ZERO = 4 };
+// sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour),
+// 3 one colour with the Newmark update fused (see k_batch.hip)
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
 
 struct AcousticArgs {

@@ -184,6 +184,7 @@ struct Mesh {
   // colour share no global point, colour c is elems[off[c] .. off[c+1])
   int el_ncolor = 0, ac_ncolor = 0;
   int el_color_off[33] = {0}, ac_color_off[33] = {0};
+  int color_version = 0;           // bumped whenever a colouring is (re)built
   int* el_color_elems = nullptr;
   int* ac_color_elems = nullptr;
 

@@ -1,7 +1,8 @@
 """Batched many-shot engine (BatchSolver / sf_batch_*) against single-shot runs and the oracle.
 
 * exact kernels + coloured assembly: every shot of the batch is bit-identical to its single-shot
-  GPUSolver run with the same options (same arithmetic, same colour order);
+  GPUSolver run with the same options (same arithmetic, same colour order), also with the Newmark
+  update fused into the coloured stiffness launches (assembly 3);
 * hipGraph replay: steps replayed from a captured group of 10 steps equal eager steps, for every
   step of every shot, with eager steps and host synchronisation between the replays;
 * fast kernels: each shot within 1e-5 rel-L2 of the FP32 oracle, for every autotuner candidate;
@@ -32,14 +33,21 @@ def single_shot(w, i, exact, assembly):
     return s, u
 
 
-def test_batch_shots_bitwise_equal_single_shot_runs(small_batch):
+@pytest.fixture(scope="module")
+def single_runs(small_batch):
+    return [single_shot(small_batch, i, True, 2) for i in range(len(small_batch.shots))]
+
+
+@pytest.mark.parametrize("assembly,sc", [(2, 4), (3, 4), (3, 8)])
+def test_batch_shots_bitwise_equal_single_shot_runs(small_batch, single_runs, assembly, sc):
     from specfem3d_amd.batch import BatchSolver
     w = small_batch
-    print(f"[gpu] batch of {len(w.shots)} shots, nstep {[s[2] for s in w.shots]}", flush=True)
-    b = BatchSolver(w.db, w.shots, exact=True, shots_per_group=4, assembly=2)
+    print(f"[gpu] batch of {len(w.shots)} shots, nstep {[s[2] for s in w.shots]}, assembly {assembly}, "
+          f"shots per group {sc}", flush=True)
+    b = BatchSolver(w.db, w.shots, exact=True, shots_per_group=sc, assembly=assembly)
     b.run(graph_steps=1)
     for i in range(len(w.shots)):
-        ref, uref = single_shot(w, i, True, 2)
+        ref, uref = single_runs[i]
         out = b.seismograms(i)
         assert np.abs(ref).max() > 0
         assert np.array_equal(out, ref), (i, rel_l2(out, ref))
@@ -47,11 +55,13 @@ def test_batch_shots_bitwise_equal_single_shot_runs(small_batch):
     b.close()
 
 
-def test_graph_replay_equals_eager_steps(small_batch):
+@pytest.mark.parametrize("assembly", [2, 3])
+def test_graph_replay_equals_eager_steps(small_batch, assembly):
     from specfem3d_amd.batch import BatchSolver
     w = small_batch
+    print(f"[gpu] graph replay, assembly {assembly}", flush=True)
     # coloured assembly: deterministic, so eager and replayed steps must agree bit for bit
-    b = BatchSolver(w.db, w.shots, exact=False, shots_per_group=8, assembly=2)
+    b = BatchSolver(w.db, w.shots, exact=False, shots_per_group=8, assembly=assembly)
     b.run(graph_steps=1)
     eager = [b.seismograms(i) for i in range(len(w.shots))]
     b.reset()
@@ -71,7 +81,7 @@ def test_autotuned_batch_matches_oracle(small_batch, oracle_mod):
     b = BatchSolver(w.db, w.shots, exact=False)
     choice = b.autotune(trial_steps=3)
     print(f"[gpu] autotuner choice {choice}", flush=True)
-    assert choice["shots_per_group"] in (4, 8) and choice["assembly"] in (0, 2)
+    assert choice["shots_per_group"] in (4, 8) and choice["assembly"] in (0, 2, 3)
     b.run(graph_steps=10)
     for i, (src, rec, n) in enumerate(w.shots):
         ref = oracle_mod.OracleSolver(w.db, src, rec, "f32").run(n)
