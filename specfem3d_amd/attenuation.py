@@ -158,8 +158,14 @@ def setup_attenuation(db, qmu, qkappa=None, min_resolved_period: float | None = 
     Qk = np.where(Qk <= 1e-5, ATTENUATION_COMP_MAXIMUM, Qk)
     Qk = np.clip(Qk, 1.0, ATTENUATION_COMP_MAXIMUM)
     # one fit per distinct (Q_mu, Q_kappa) pair, broadcast to the points
-    pairs, inv = np.unique(np.stack([Qm.ravel(), Qk.ravel()], 1), axis=0, return_inverse=True)
-    inv = inv.reshape(Qm.shape)
+    if Qm.size and (Qm == Qm.flat[0]).all() and (Qk == Qk.flat[0]).all():   # uniform Q: one fit
+        pairs = np.array([[Qm.flat[0], Qk.flat[0]]])
+        inv = np.zeros(Qm.shape, np.int64)
+    else:
+        key = Qm.ravel() + 1j * Qk.ravel()                          # 1-D sort, (Q_mu, Q_kappa) order
+        uk, inv = np.unique(key, return_inverse=True)
+        pairs = np.stack([uk.real, uk.imag], 1)
+        inv = inv.reshape(Qm.shape)
     tab_a = np.empty((len(pairs), N_SLS), np.float32)
     tab_b = np.empty((len(pairs), N_SLS), np.float32)
     tab_s = np.empty(len(pairs), np.float32)

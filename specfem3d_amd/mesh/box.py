@@ -150,9 +150,14 @@ def build_box(nx: int, ny: int, nz: int,
     vs = np.empty((nspec, 125), np.float32)
 
     def fill(mask, fn, fluid):
-        pts = np.unique(ib[mask])
+        loc = ib[mask]
+        seen = np.zeros(nglob, bool)
+        seen[loc] = True
+        pts = np.nonzero(seen)[0]                     # sorted distinct points of the domain
+        lut = np.empty(nglob, np.int64)
+        lut[pts] = np.arange(len(pts))
         r, p, s = fn(xstore[pts], ystore[pts], zstore[pts])
-        pos = np.searchsorted(pts, ib[mask])
+        pos = lut[loc]
         rho[mask] = np.asarray(r, np.float32)[pos]
         vp[mask] = np.asarray(p, np.float32)[pos]
         vs[mask] = 0.0 if fluid else np.asarray(s, np.float32)[pos]
@@ -225,12 +230,28 @@ def gradient_grf_model(extent, vp_top=2800.0, vp_bot=5000.0, vs_top=1500.0, vs_b
     ph = rng.uniform(0.0, 2 * np.pi, size=nmodes)
     amp_m = np.sqrt(2.0 / nmodes)
 
-    def grf(x, y, z):
-        x = np.asarray(x, np.float64); y = np.asarray(y, np.float64); z = np.asarray(z, np.float64)
+    def grf_block(x, y, z):
         out = np.zeros(np.shape(x))
         for m in range(nmodes):
             out += np.cos(k[m, 0] * x + k[m, 1] * y + k[m, 2] * z + ph[m])
         return amp_m * out
+
+    def grf(x, y, z):
+        x = np.asarray(x, np.float64); y = np.asarray(y, np.float64); z = np.asarray(z, np.float64)
+        n = x.size
+        if n < (1 << 20):
+            return grf_block(x, y, z)
+        # elementwise, so blocks on threads give the same values (numpy ufuncs release the GIL)
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+        xf, yf, zf = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+        out = np.empty(n)
+        step = 1 << 18
+        def work(a):
+            out[a:a + step] = grf_block(xf[a:a + step], yf[a:a + step], zf[a:a + step])
+        with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as ex:
+            list(ex.map(work, range(0, n, step)))
+        return out.reshape(np.shape(x))
 
     def f(x, y, z):
         depth_frac = np.clip((zt - np.asarray(z, np.float64)) / max(zt - z0, 1e-30), 0.0, 1.0)
