@@ -194,6 +194,49 @@ def adjoint_measurement(args) -> dict:
             "unit": "s (forward with checkpoints + adjoint with recompute)"}
 
 
+def large_measurement(args) -> dict:
+    """Largest single-GPU workload: 160x160x80 elements (131.9 M GLL points), 10 acoustic layers over a
+    3-SLS viscoelastic solid, one of its shots run for a few steps through the ABI call sequence:
+    throughput, stiffness roofline and device memory in use."""
+    import ctypes as C
+    import numpy as np
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.solver import GPUSolver
+    t0 = time.time()
+    n = args.large_steps + 3
+    w = W.config5(scale=args.large_scale, nshots=1, nstep=n)
+    db = w.db
+    log(f"[bench] built {w.name}: NGLOB {db.nglob} in {time.time() - t0:.1f}s")
+    g = GPUSolver(db, w.sources, w.receivers, nstep=n, attenuation=w.attenuation, exact=False, assembly=0)
+    for _ in range(3):
+        g.step()
+    g.synchronize()
+    free, used, total = C.c_float(0), C.c_float(0), C.c_float(0)
+    g.L.get_free_device_memory_(C.byref(free), C.byref(used), C.byref(total))
+    g.L.sf_timing_start(g.MP, C.c_int(args.large_steps))
+    t = time.perf_counter()
+    for _ in range(args.large_steps):
+        g.step()
+    g.synchronize()
+    wall = time.perf_counter() - t
+    tot, cnt = C.c_double(0.0), C.c_int(0)
+    g.L.sf_timing_read(g.MP, C.byref(tot), C.byref(cnt))
+    finite = bool(np.isfinite(g.seis_d[: g.seismo_current]).all())
+    g.close()
+    kavg = tot.value / max(cnt.value, 1)
+    kbytes = stiffness_bytes_per_launch(db, w.attenuation)
+    ach = kbytes / (kavg / 1e3) / 1e9
+    log(f"[bench] large: {args.large_steps} steps in {wall:.2f}s, stiffness {kavg:.2f} ms, device memory used "
+        f"{used.value:.0f} of {total.value:.0f} MB, finite={finite}")
+    return {"workload": w.name, "nglob": db.nglob, "value": db.nglob * args.large_steps / wall,
+            "unit": "GLL-point updates/s", "steps": args.large_steps, "ms_per_step": wall / args.large_steps * 1e3,
+            "device_mb_used": round(used.value), "seismograms_finite": finite,
+            "roofline": {"bound": "hbm", "kernel": "k_elastic_pt (viscoelastic 3-SLS stiffness)",
+                         "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_launch": kbytes,
+                         "avg_launch_ms": round(kavg, 4)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +256,10 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=4000,
                     help="steps of the checkpointed kernel run on the 48^3 adjoint workload (0: skip)")
     ap.add_argument("--adjoint-scale", type=float, default=1.0, help="shrink the adjoint mesh (tests only)")
+    ap.add_argument("--large-steps", type=int, default=0,
+                    help="steps of one shot of the 160x160x80 viscoelastic workload (0: skip; the mesh build "
+                         "takes minutes)")
+    ap.add_argument("--large-scale", type=float, default=1.0, help="shrink the large mesh (tests only)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -293,6 +340,7 @@ def main():
 
     adj = adjoint_measurement(args) if args.adjoint_steps > 0 else None
     batched = batched_measurement(args) if args.batch_shots > 0 else None
+    large = large_measurement(args) if args.large_steps > 0 else None
 
     out = {
         "metric": "GLL-point updates/s/GPU (forward, single shot)",
@@ -315,6 +363,8 @@ def main():
         "batched": batched,
         "adjoint": adj,
     }
+    if large is not None:
+        out["large"] = large
     print(json.dumps(out), flush=True)
 
 

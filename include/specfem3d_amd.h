@@ -287,6 +287,13 @@ void sf_batch_displ(long batch, int shot, realw* out);
 void sf_batch_reset(long batch);
 void sf_batch_time(long batch, int nsteps, double* stiffness_ms, double* step_ms);
 void sf_batch_destroy(long batch);
+/* host only: per-local-point flags of the fused batched mode for elements elems[q] of colour color[q]
+   (colours run in increasing order) and the points Stacey or sources add to before the colours
+   (forced[g] != 0): 1 = first writer, 2 = finisher of the Newmark update, 4 = finisher clears the
+   acceleration; 125 entries per element of ibool0; returns -1 if two elements of a colour share a
+   point */
+int sf_batch_point_flags(int nglob, int nelem, const int* ibool0, const int* elems, const int* color,
+                         const unsigned char* forced, unsigned char* flags);
 /* launch order (0-based elastic elements, each once) of the atomic and coloured stiffness launches,
    e.g. a locality (Morton) order from the host; the ordered assembly keeps the processing list */
 void sf_set_element_order(long* Mesh_pointer, int n, const int* order);

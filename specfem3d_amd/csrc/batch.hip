@@ -252,13 +252,42 @@ static void zero_state(Batch* b) {
 // Point flags of the fused mode: for each global point, the element of its lowest colour writes
 // first (unless Stacey or a source adds to the point before the colours), the element of its
 // highest colour finishes the Newmark update and clears `a` where those passes add.
+static int point_flags(int nglob, int nelem, const int* ibool0, const int* elems, const int* color,
+                       const unsigned char* forced, unsigned char* flags) {
+  std::vector<int> cmin((size_t)nglob, 1 << 30), cmax((size_t)nglob, -1), nlast((size_t)nglob, 0);
+  for (int q = 0; q < nelem; q++)
+    for (int p = 0; p < 125; p++) {
+      const int g = ibool0[(size_t)elems[q] * 125 + p];
+      cmin[g] = std::min(cmin[g], color[q]);
+      cmax[g] = std::max(cmax[g], color[q]);
+    }
+  for (int q = 0; q < nelem; q++) {
+    const int e = elems[q], c = color[q];
+    for (int p = 0; p < 125; p++) {
+      const int g = ibool0[(size_t)e * 125 + p];
+      unsigned char f = forced[g] ? BF_ZERO : 0;
+      if (c == cmin[g] && !forced[g]) f |= BF_FIRST;
+      if (c == cmax[g]) {
+        f |= BF_LAST;
+        nlast[g]++;
+      }
+      flags[(size_t)e * 125 + p] = f;
+    }
+  }
+  for (int g = 0; g < nglob; g++)
+    if (cmax[g] >= 0 && nlast[g] != 1) return -1;  // two elements of one colour share the point
+  return 0;
+}
+
 static void build_point_flags(Batch* b) {
   Mesh* m = b->m;
   ensure_element_colors(m);
   const int ne = m->nspec_elastic;
-  std::vector<int> lst(ne), hib((size_t)m->nspec * 125);
+  std::vector<int> lst(ne), hib((size_t)m->nspec * 125), col(ne);
   SF_CHECK(hipMemcpy(lst.data(), m->el_color_elems, sizeof(int) * ne, hipMemcpyDeviceToHost));
   SF_CHECK(hipMemcpy(hib.data(), m->ibool, sizeof(int) * hib.size(), hipMemcpyDeviceToHost));
+  for (int c = 0; c < m->el_ncolor; c++)
+    for (int q = m->el_color_off[c]; q < m->el_color_off[c + 1]; q++) col[q] = c;
   std::vector<unsigned char> forced((size_t)m->nglob, 0);
   if (m->stacey && m->abs_el.n > 0) {
     std::vector<int> ig(m->abs_el.n);
@@ -267,34 +296,9 @@ static void build_point_flags(Batch* b) {
   }
   for (const Batch::Src& q : b->srcs)
     for (int p = 0; p < 125; p++) forced[hib[(size_t)q.ispec * 125 + p]] = 1;
-  std::vector<int> cmin((size_t)m->nglob, 1 << 30), cmax((size_t)m->nglob, -1), col(ne);
-  for (int c = 0; c < m->el_ncolor; c++)
-    for (int q = m->el_color_off[c]; q < m->el_color_off[c + 1]; q++) {
-      const int e = lst[q];
-      col[q] = c;
-      for (int p = 0; p < 125; p++) {
-        const int g = hib[(size_t)e * 125 + p];
-        cmin[g] = std::min(cmin[g], c);
-        cmax[g] = std::max(cmax[g], c);
-      }
-    }
   std::vector<unsigned char> fl((size_t)m->nspec * 125, 0);
-  std::vector<int> nlast((size_t)m->nglob, 0);
-  for (int q = 0; q < ne; q++) {
-    const int e = lst[q], c = col[q];
-    for (int p = 0; p < 125; p++) {
-      const int g = hib[(size_t)e * 125 + p];
-      unsigned char f = forced[g] ? BF_ZERO : 0;
-      if (c == cmin[g] && !forced[g]) f |= BF_FIRST;
-      if (c == cmax[g]) {
-        f |= BF_LAST;
-        nlast[g]++;
-      }
-      fl[(size_t)e * 125 + p] = f;
-    }
-  }
-  for (int g = 0; g < m->nglob; g++)
-    if (cmax[g] >= 0 && nlast[g] != 1) fatal("sf_batch", "element colouring shares a point inside a colour");
+  if (point_flags(m->nglob, ne, hib.data(), lst.data(), col.data(), forced.data(), fl.data()) != 0)
+    fatal("sf_batch", "element colouring shares a point inside a colour");
   bfree(b->flags);
   b->flags = bupload(fl.data(), (int64_t)fl.size());
   b->flags_version = m->color_version;
@@ -716,4 +720,14 @@ extern "C" void sf_batch_destroy(long h) {
   bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
   bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);
   delete b;
+}
+
+// Host-only (no device needed): the fused mode's per-local-point flags for elements elems[q] of
+// colour color[q] (colours run in increasing order) and the points that Stacey or sources add to
+// before the colours (forced[g] != 0): bit 1 first writer, bit 2 finisher, bit 4 cleared by the
+// finisher. flags has 125 entries per element of ibool0. Returns 0, or -1 if two elements of one
+// colour share a point.
+extern "C" int sf_batch_point_flags(int nglob, int nelem, const int* ibool0, const int* elems, const int* color,
+                                    const unsigned char* forced, unsigned char* flags) {
+  return point_flags(nglob, nelem, ibool0, elems, color, forced, flags);
 }

@@ -13,7 +13,7 @@
 4. ``config4`` -- 48^3 elastic box for adjoint sensitivity kernels, 1 source, 4000 steps,
    64 receivers.
 5. ``config5`` -- 160 x 160 x 80 (2.05 M elements, 131.9 M GLL points) viscoelastic with an
-   acoustic top layer, 8 shots.
+   acoustic top layer (10 element layers), 8 shots, 6000 steps.
 
 Each builder accepts a ``scale`` to shrink the element counts for tests; the GLL-point count
 ``NGLOB = (4 nx + 1)(4 ny + 1)(4 nz + 1)``.
@@ -148,6 +148,35 @@ def config4(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int 
     src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
     rec = S.setup_receivers(db, receiver_grid(ext, nrec_side, 0.0))
     return Workload(f"config4: {n}^3 elastic adjoint, {nstep} steps", db, src, rec, nstep, dt)
+
+
+def config5(scale: float = 1.0, nshots: int = 8, nstep: int = 6000, dt: float = 0.006, seed: int = 5,
+            qmu: float = 100.0, nrec_side: int = 8) -> Workload:
+    """160 x 160 x 80 elements (scale 1): the top 10 layers acoustic (vp 1500, rho 1020) over the
+    gradient + GRF solid with Q_mu = 100 (3 SLS), ``nshots`` seeded random CMT sources below the sea
+    floor sharing a sea-floor receiver grid."""
+    nx = max(2, int(round(160 * scale)))
+    nz = max(8, int(round(80 * scale)))
+    h = 500.0
+    ext = _model_extent(nx, nx, nz, h)
+    nac = max(1, nz // 8)
+    mat = gradient_grf_model(ext, seed=seed, z_top=-nac * h)
+    db = build_box(nx, nx, nz, ext, material=mat, acoustic_top_layers=nac,
+                   acoustic_material=homogeneous(1020.0, 1500.0, 0.0))
+    db.finalize(dt)
+    from .attenuation import setup_attenuation
+    att = setup_attenuation(db, qmu=qmu, qkappa=9000.0)
+    rng = np.random.default_rng(seed)
+    seafloor = nac * h
+    rec = S.setup_receivers(db, receiver_grid(ext, nrec_side, z_below_top=seafloor + 1.0))
+    shots = []
+    for _ in range(nshots):
+        cmt = random_cmt(rng, ext, zmin_frac=0.3, zmax_frac=0.7, f0=(0.5, 2.0), z_top=0.0)
+        cmt.depth_km = max(cmt.depth_km, (seafloor + 2 * h) / 1000.0)
+        shots.append((S.setup_cmt_sources(db, [cmt], dt, ricker=True), rec, nstep))
+    return Workload(f"config5: {nx}x{nx}x{nz} acoustic-over-viscoelastic, {nshots} shots, {nstep} steps", db,
+                    shots[0][0], rec, nstep, dt, attenuation=att, shots=shots,
+                    meta=dict(nx=nx, nz=nz, n_acoustic_layers=nac))
 
 
 def gaussian_bump(db, center=None, sigma=None) -> np.ndarray:

@@ -67,3 +67,76 @@ def test_morton_launch_order_is_a_local_permutation():
     n, col = color(db, mo)
     assert n == 8
     check_proper(db, mo, col)
+
+
+def point_flags(db, elems, col, forced):
+    lib = abi.load()
+    ib = np.ascontiguousarray(db.ibool.reshape(db.nspec, 125).astype(np.int32))
+    order = np.argsort(col, kind="stable")
+    el = np.ascontiguousarray(np.asarray(elems, np.int32)[order])
+    cl = np.ascontiguousarray(np.asarray(col, np.int32)[order])
+    fl = np.zeros(db.nspec * 125, np.uint8)
+    rc = lib.sf_batch_point_flags(C.c_int(db.nglob), C.c_int(el.size), abi.ptr(ib), abi.ptr(el), abi.ptr(cl),
+                                  abi.ptr(np.ascontiguousarray(forced, np.uint8)), abi.ptr(fl))
+    return rc, el, cl, fl.reshape(db.nspec, 125)
+
+
+def fused_assembly(db, el, cl, fl, forced, ext, contrib):
+    """The fused batched mode's accumulation on the CPU: `a` starts as garbage (NaN) except where the
+    Stacey/source pass added `ext`; per colour, first writers store, others add; the finisher reads
+    the total. Returns the totals seen by the finishers and how many finishers each point had."""
+    ib = db.ibool.reshape(db.nspec, 125)
+    a = np.full(db.nglob, np.nan, np.float32)
+    a[forced] = np.float32(0) + ext[forced]
+    total = np.full(db.nglob, np.nan, np.float32)
+    nfin = np.zeros(db.nglob, int)
+    for c in np.unique(cl):
+        for e in el[cl == c]:
+            for p in range(125):
+                g, f = ib[e, p], fl[e, p]
+                a[g] = (np.float32(0) if f & 1 else a[g]) + contrib[e, p]
+                if f & 2:
+                    total[g] = a[g]
+                    nfin[g] += 1
+                    if f & 4:
+                        a[g] = 0
+    return total, nfin
+
+
+def test_fused_mode_point_flags_reproduce_the_plain_sum():
+    """Flags of the fused batched assembly (sf_batch_point_flags): with `a` never cleared except at
+    points Stacey/sources add to, first-writer/finisher flags must give every point exactly one
+    finisher and the same float32 total as zero-initialised accumulation in colour order."""
+    db = build_box(5, 4, 3, (0, 5000, 0, 4000, -3000, 0), material=homogeneous(2300, 2800, 1500))
+    rng = np.random.default_rng(7)
+    elems = rng.permutation(db.nspec).astype(np.int32)
+    n, col = color(db, elems)
+    forced = np.zeros(db.nglob, bool)
+    forced[np.unique(db.ibool[db.abs_faces.ispec].ravel())[::3]] = True     # some Stacey points
+    forced[np.unique(db.ibool[elems[5]])] = True                               # a source element
+    rc, el, cl, fl = point_flags(db, elems, col, forced)
+    assert rc == 0
+    ext = rng.normal(size=db.nglob).astype(np.float32)
+    contrib = rng.normal(size=(db.nspec, 125)).astype(np.float32)
+    total, nfin = fused_assembly(db, el, cl, fl, forced, ext, contrib)
+    ref = np.where(forced, ext, np.float32(0)).astype(np.float32)
+    ib = db.ibool.reshape(db.nspec, 125)
+    for c in np.unique(cl):
+        for e in el[cl == c]:
+            for p in range(125):
+                ref[ib[e, p]] = ref[ib[e, p]] + contrib[e, p]
+    assert (nfin == 1).all()
+    assert np.array_equal(total, ref)
+    # test of the test: a first-writer flag on a forced point drops its Stacey/source term
+    bad = fl.copy()
+    e, p = next((e, p) for e in el for p in range(125) if forced[ib[e, p]] and not fl[e, p] & 1)
+    bad[ib == ib[e, p]] &= ~np.uint8(1)
+    bad[e, p] |= 1
+    t2, _ = fused_assembly(db, el, cl, bad, forced, ext, contrib)
+    assert not np.array_equal(t2, ref)
+
+
+def test_point_flags_refuse_a_colour_that_shares_points():
+    db = build_box(2, 1, 1, (0, 2000, 0, 1000, -1000, 0), material=homogeneous(2300, 2800, 1500))
+    rc, *_ = point_flags(db, np.arange(2), np.zeros(2, np.int32), np.zeros(db.nglob, bool))
+    assert rc == -1
