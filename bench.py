@@ -246,6 +246,10 @@ def main():
     ap.add_argument("--exact", type=int, default=0, help="1: kernels without FMA (CPU rounding)")
     ap.add_argument("--assembly", type=int, default=0,
                     help="0 atomic scatter, 1 ordered (CPU element order), 2 coloured (no atomics)")
+    ap.add_argument("--element-order", default="morton", choices=["morton", "none"],
+                    help="launch order of the elastic elements of the single-shot run")
+    ap.add_argument("--skip-single", action="store_true",
+                    help="skip the single-shot 96x96x48 measurement (profiling runs of the other parts)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU oracle sample (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-shots", type=int, default=32, help="shots of the batched 64x64x32 workload (0: skip)")
@@ -284,6 +288,14 @@ def main():
     from specfem3d_amd import workloads as W
     from specfem3d_amd.solver import GPUSolver
 
+    if args.skip_single:
+        out = {"metric": "GLL-point updates/s/GPU (forward, single shot)", "value": None,
+               "batched": batched_measurement(args) if args.batch_shots > 0 else None,
+               "adjoint": adjoint_measurement(args) if args.adjoint_steps > 0 else None,
+               "large": large_measurement(args) if args.large_steps > 0 else None}
+        print(json.dumps(out), flush=True)
+        return
+
     t0 = time.time()
     nstep_total = args.warmup + args.steps
     w = W.config2(scale=args.scale, nstep=max(nstep_total, 1))
@@ -291,7 +303,8 @@ def main():
     log(f"[bench] built {w.name}: {db.summary()} in {time.time() - t0:.1f}s")
 
     g = GPUSolver(db, w.sources, w.receivers, nstep=nstep_total, attenuation=w.attenuation,
-                  exact=bool(args.exact), assembly=args.assembly)
+                  exact=bool(args.exact), assembly=args.assembly,
+                  element_order=None if args.element_order == "none" else args.element_order)
     log("[bench] prepared device arrays")
     for _ in range(args.warmup):
         g.step()
@@ -357,7 +370,8 @@ def main():
         "data": "synthetic (seeded gradient + Gaussian random field model, random CMT source)",
         "config": {"workload": w.name, "nglob": nglob, "nspec": db.nspec, "nspec_acoustic": db.nspec_acoustic,
                    "attenuation": "3 SLS, Q_mu=100", "receivers": w.receivers.n, "exact": args.exact,
-                   "assembly": ["atomic", "ordered", "coloured"][args.assembly]},
+                   "assembly": ["atomic", "ordered", "coloured"][args.assembly],
+                   "element_order": args.element_order},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -154,6 +154,7 @@ class OracleSolver:
         self.plant_flip_ac_el = 0
         self.plant_flip_el_ac = 0
         self.stf_table = None
+        self.absorb_record = None   # SAVE_FORWARD: Stacey terms of the step (3*25 per face), or None
 
     # -- pieces ---------------------------------------------------------------------
     def _call(self, name, *args):
@@ -232,7 +233,8 @@ class OracleSolver:
             if self.stacey:
                 n, isp, ijk, nrm, jw = self.abs
                 self._call("orc_stacey_elastic", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
-                           _p(self.is_el), _p(self.rho_vp), _p(self.rho_vs), _p(self.veloc), _p(self.accel), None)
+                           _p(self.is_el), _p(self.rho_vp), _p(self.rho_vs), _p(self.veloc), _p(self.accel),
+                           None if self.absorb_record is None else _p(self.absorb_record))
             if self.acoustic and self.cpl[0]:
                 n, isp, ijk, nrm, jw = self.cpl
                 self._call("orc_coupling_el_ac", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
@@ -248,6 +250,29 @@ class OracleSolver:
             self.forces_elastic()
             self._call("orc_kernel3_elastic", C.c_int64(self.db.nglob), _p(self.veloc), _p(self.accel),
                        _p(self.rmassx), _p(self.rmassy), _p(self.rmassz), self._r(self.deltatover2))
+
+    def backward_step(self, record: np.ndarray, stf_index: int):
+        """One step of the time-reversed reconstruction (elastic domains), iterate_time.F90:233-262 with
+        b_deltat = -DT (prepare_timerun.F90:424-429; set by ``reversed_solver``): predictor, then the
+        Stacey record ``record`` subtracted in place of Stacey (compute_stacey_viscoelastic.F90:212-287),
+        the sources at stf_table[stf_index], stiffness, mass and corrector. The reference passes the
+        record of forward step NSTEP-it+1 and the source time (NSTEP-it) DT - t0, i.e. stf_index =
+        NSTEP-it (compute_add_sources_viscoelastic.F90:650-690)."""
+        self.it += 1
+        self.predictor()
+        n, isp, ijk, nrm, jw = self.abs
+        if self.stacey:
+            rec = np.ascontiguousarray(record, self.dt_np)
+            self._call("orc_stacey_elastic_replay", C.c_int(n), _p(isp), _p(ijk), _p(self.ibool), _p(self.is_el),
+                       _p(rec), _p(self.accel))
+        if self.sources is not None and self.sources.n:
+            stf = np.ascontiguousarray(
+                np.asarray(self.stf_table[stf_index], np.float64).astype(np.float32).astype(self.dt_np))
+            self._call("orc_add_sources_el", C.c_int(self.sources.n), _p(self.src_ispec), _p(self.src_arr),
+                       _p(stf), _p(self.ibool), _p(self.is_el), _p(self.accel))
+        self.forces_elastic()
+        self._call("orc_kernel3_elastic", C.c_int64(self.db.nglob), _p(self.veloc), _p(self.accel),
+                   _p(self.rmassx), _p(self.rmassy), _p(self.rmassz), self._r(self.deltatover2))
 
     def seismogram(self, field: str = "displ") -> np.ndarray:
         out = np.zeros(3 * self.receivers.n, self.dt_np)
@@ -277,6 +302,21 @@ class OracleSolver:
         fn = getattr(self.L, name + self.sfx)
         fn.restype = C.c_float if self.prec == "f32" else C.c_double
         return fn(C.c_int64(self.db.nglob), _p(f))
+
+
+def reversed_solver(db, sources, receivers, nstep: int, last, precision: str = "f32") -> OracleSolver:
+    """The backward field of the classic kernel run: starts from the forward run's last frame
+    ``last`` = (displ, veloc, accel) (read_forward_arrays.f90:29-126) and steps with b_deltat = -DT,
+    b_deltatover2 = b_deltat/2, b_deltatsqover2 = b_deltat^2/2 (prepare_timerun.F90:424-429)."""
+    b = OracleSolver(db, sources, receivers, precision)
+    R = b.dt_np
+    b.deltat = R(-b.deltat)
+    b.deltatover2 = R(b.deltat / R(2))
+    b.deltatsqover2 = R(b.deltat * b.deltat / R(2))
+    b.stf_table = sources.stf_table(nstep) if sources is not None and sources.n else None
+    for dst, src in zip((b.displ, b.veloc, b.accel), last):
+        dst[:] = src
+    return b
 
 
 def element_strain(db, displ, precision="f32"):

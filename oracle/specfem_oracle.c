@@ -650,6 +650,23 @@ void FN(orc_compute_kernels_el)(int nelem, const int *elems, const int *ibool, c
     }
 }
 
+/* backward Stacey of the time-reversed reconstruction: b_accel -= b_absorb_field, the record the
+ * forward run stored at this step (compute_stacey_viscoelastic.F90:212-287, face order) */
+void FN(orc_stacey_elastic_replay)(int nfaces, const int *face_ispec, const int *face_ijk, const int *ibool,
+                                   const unsigned char *is_elastic, const R *b_absorb, R *accel)
+{
+    for (int f = 0; f < nfaces; f++) {
+        int e = face_ispec[f];
+        if (!is_elastic[e]) continue;
+        for (int a = 0; a < 25; a++) {
+            const int *ijk = face_ijk + ((int64_t)f * 25 + a) * 3;
+            int p = ijk[0] + 5 * ijk[1] + 25 * ijk[2];
+            int64_t g = ibool[(int64_t)e * 125 + p];
+            for (int c = 0; c < 3; c++) accel[3 * g + c] = accel[3 * g + c] - b_absorb[((int64_t)f * 25 + a) * 3 + c];
+        }
+    }
+}
+
 /* acoustic sensitivity kernels: compute_kernels.f90 compute_kernels_ac (rho_ac_kl, kappa_ac_kl) with
  * the gradient of compute_gradient_in_acoustic_fast_Deville (compute_gradient_in_acoustic.f90):
  * grad chi / rho from the three contractions, mapped by the inverse Jacobian (irregular) or

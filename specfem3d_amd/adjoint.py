@@ -367,11 +367,17 @@ class ReversalKernelSolver(GPUSolver):
         self.L.transfer_b_fields_to_device_(C.byref(C.c_int(n3)), *(abi.ptr(a) for a in self._last), self.MP)
         self.it = 0
 
-    def backward_step(self, records: np.ndarray):
-        """One reversed step of the backward field (FORWARD_OR_ADJOINT = 3)."""
+    def backward_step(self, records: np.ndarray, exact_reversal: bool = False):
+        """One reversed step of the backward field (FORWARD_OR_ADJOINT = 3).
+
+        As the reference indexes them, step ``it`` replays the record of forward step NSTEP-it+1
+        (read_abs) and adds the sources at (NSTEP-it) DT - t0. Undoing forward step NSTEP-it of the
+        Newmark scheme exactly needs that step's own record and source time, one step earlier:
+        ``exact_reversal`` uses those (the reconstruction is then exact to rounding)."""
         L, MP = self.L, self.MP
         self.it += 1
         it = self.it
+        k = self.nstep - it - (1 if exact_reversal else 0)    # 0-based forward step of the record/STF
         three, zero = self._ref(3), self._ref(0)
         bdt, bdt2, bdtsq2 = (C.byref(self.b_c[k]) for k in ("dt", "dt2", "dtsq2"))
         L.update_displacement_cuda_(MP, bdt, bdt2, bdtsq2, three)
@@ -380,10 +386,10 @@ class ReversalKernelSolver(GPUSolver):
             ip = C.c_int(iphase)
             L.compute_forces_viscoelastic_cuda_(MP, C.byref(ip), bdt, zero, zero, C.byref(strain), C.byref(att), three)
             if iphase == 1:
-                self._rec = np.ascontiguousarray(records[self.nstep - it], np.float32)
+                self._rec = np.ascontiguousarray(records[k], np.float32)
                 L.compute_stacey_viscoelastic_cuda_(MP, C.byref(ip), abi.ptr(self._rec), three)
                 if self.stf_table is not None:
-                    stf = np.ascontiguousarray(self.stf_table[self.nstep - it], np.float64)
+                    stf = np.ascontiguousarray(self.stf_table[k], np.float64)
                     L.compute_add_sources_el_s3_cuda_(MP, stf.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(stf))))
         L.kernel_3_a_cuda_(MP, bdt2, bdt2, zero, three)
 
