@@ -279,6 +279,10 @@ void sf_batch_finalize(long batch);
    at step 1 (a new batch or after sf_batch_reset) */
 void sf_batch_set_options(long batch, int shots_per_group, int assembly);
 void sf_batch_get_options(long batch, int* shots_per_group, int* assembly);
+/* kernel variants with the same results: "lds_overlay" 0/1 (displacement tile and stress products
+   share LDS); unknown names are fatal */
+void sf_batch_set_tuning(long batch, const char* name, int value);
+int sf_batch_get_tuning(long batch, const char* name);
 double sf_batch_autotune(long batch, int trial_steps);
 void sf_batch_run(long batch, int nsteps, int graph_steps);
 int sf_batch_current_step(long batch);

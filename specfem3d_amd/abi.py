@@ -49,7 +49,7 @@ EXPORTED = [
     "sf_set_options", "sf_set_tuning", "sf_set_element_order", "sf_color_elements", "sf_batch_point_flags", "sf_snapshot_save", "sf_snapshot_load",
     "sf_history_reserve", "sf_history_store", "sf_history_load", "sf_checkpoints_free", "sf_zero_kernels",
     "sf_batch_create", "sf_batch_add_source", "sf_batch_add_receiver", "sf_batch_finalize", "sf_batch_set_options",
-    "sf_batch_get_options", "sf_batch_autotune", "sf_batch_run", "sf_batch_current_step", "sf_batch_seismograms",
+    "sf_batch_get_options", "sf_batch_set_tuning", "sf_batch_get_tuning", "sf_batch_autotune", "sf_batch_run", "sf_batch_current_step", "sf_batch_seismograms",
     "sf_batch_displ", "sf_batch_reset", "sf_batch_time", "sf_batch_destroy", "sf_set_jacobian", "sf_synchronize", "sf_last_error",
 ]
 

@@ -29,7 +29,9 @@ class BatchSolver:
         self.db = db
         self.shots = list(shots)
         self.nstep = [int(n) for _, _, n in self.shots]
-        self.base = GPUSolver(db, None, None, nstep=max(self.nstep), exact=exact, assembly=0, device=device)
+        # Morton launch order: elements of a colour that sit close together share L2 lines of rows
+        self.base = GPUSolver(db, None, None, nstep=max(self.nstep), exact=exact, assembly=0, device=device,
+                              element_order="morton")
         L = self.L = self.base.L
         L.sf_batch_create.restype = C.c_long
         L.sf_batch_autotune.restype = C.c_double
@@ -66,7 +68,12 @@ class BatchSolver:
     def options(self) -> dict:
         sc, asm = C.c_int(0), C.c_int(0)
         self.L.sf_batch_get_options(self.h, C.byref(sc), C.byref(asm))
-        return {"shots_per_group": sc.value, "assembly": asm.value}
+        self.L.sf_batch_get_tuning.restype = C.c_int
+        return {"shots_per_group": sc.value, "assembly": asm.value,
+                "lds_overlay": int(self.L.sf_batch_get_tuning(self.h, b"lds_overlay"))}
+
+    def set_tuning(self, name: str, value: int):
+        self.L.sf_batch_set_tuning(self.h, name.encode(), C.c_int(int(value)))
 
     def set_options(self, shots_per_group: int, assembly: int):
         self.L.sf_batch_set_options(self.h, C.c_int(shots_per_group), C.c_int(assembly))

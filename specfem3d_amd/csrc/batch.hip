@@ -32,6 +32,7 @@ struct Batch {
   Mesh* m = nullptr;
   int nshots = 0, S = 0;
   int sc = 4, mode = 2;          // shots per workgroup, assembly (0 atomic, 2 coloured, 3 fused)
+  int overlay = 0;               // 1: displacement tile and stress products share LDS
   int max_nstep = 0;
   realw dt = 0.f, dt2 = 0.f, dtsq2 = 0.f;  // deltat, deltatover2, deltatsqover2
   std::vector<int> nstep_user;   // per user shot
@@ -331,12 +332,12 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
     for (int c = 0; c < m->el_ncolor; c++) {
       A.elems = m->el_color_elems + m->el_color_off[c];
       A.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
-      launch_elastic_batch(st, A, b->sc, b->mode, m->exact != 0);
+      launch_elastic_batch(st, A, b->sc, b->mode, b->overlay, m->exact != 0);
     }
   } else {
     A.elems = m->elastic_elems;
     A.nelem = m->nspec_elastic;
-    launch_elastic_batch(st, A, b->sc, 0, m->exact != 0);
+    launch_elastic_batch(st, A, b->sc, 0, b->overlay, m->exact != 0);
   }
 }
 
@@ -592,6 +593,27 @@ extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) 
   b->mode = assembly;
 }
 
+// Kernel variants that give the same results: "lds_overlay" 0/1 (the displacement tile shares LDS
+// with the stress products). Unknown names are fatal.
+extern "C" void sf_batch_set_tuning(long h, const char* name, int value) {
+  Batch* b = batch_of(h);
+  if (std::strcmp(name, "lds_overlay") == 0) {
+    if (value < 0 || value > 1) fatal("sf_batch_set_tuning", "lds_overlay must be 0 or 1");
+    SF_CHECK(hipStreamSynchronize(b->m->stream));
+    drop_graph(b);
+    b->overlay = value;
+  } else {
+    fatal("sf_batch_set_tuning", "unknown tuning switch");
+  }
+}
+
+extern "C" int sf_batch_get_tuning(long h, const char* name) {
+  Batch* b = batch_of(h);
+  if (std::strcmp(name, "lds_overlay") == 0) return b->overlay;
+  fatal("sf_batch_get_tuning", "unknown tuning switch");
+  return 0;
+}
+
 extern "C" void sf_batch_get_options(long h, int* shots_per_group, int* assembly) {
   Batch* b = batch_of(h);
   *shots_per_group = b->sc;
@@ -687,12 +709,14 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
   Batch* b = batch_of(h);
   if (!b->ready) fatal("sf_batch_autotune", "sf_batch_finalize first");
   double best = 1e300;
-  int best_sc = b->sc, best_mode = b->mode;
+  int best_sc = b->sc, best_mode = b->mode, best_ov = b->overlay;
   for (int mode : {3, 2, 0})
-    for (int sc : {4, 8}) {
+    for (int sc : {4, 8})
+    for (int ov : {1, 0}) {
       if (b->S % sc) continue;
       b->sc = sc;
       b->mode = mode;
+      b->overlay = ov;
       drop_graph(b);
       zero_state(b);
       run_steps(b, 2, 1);  // warm-up (first step runs the plain predictor)
@@ -702,10 +726,12 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
         best = st;
         best_sc = sc;
         best_mode = mode;
+        best_ov = ov;
       }
     }
   b->sc = best_sc;
   b->mode = best_mode;
+  b->overlay = best_ov;
   drop_graph(b);
   zero_state(b);
   return best;

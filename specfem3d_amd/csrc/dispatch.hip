@@ -19,8 +19,8 @@ void launch_elastic_exact(hipStream_t, const ElasticArgs&, bool, bool, int, int,
 void launch_elastic_fast(hipStream_t, const ElasticArgs&, bool, bool, int, int, int);
 void launch_element_strain_exact(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
 void launch_element_strain_fast(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
-void launch_elastic_batch_exact(hipStream_t, const BatchArgs&, int, int);
-void launch_elastic_batch_fast(hipStream_t, const BatchArgs&, int, int);
+void launch_elastic_batch_exact(hipStream_t, const BatchArgs&, int, int, int);
+void launch_elastic_batch_fast(hipStream_t, const BatchArgs&, int, int, int);
 void launch_acoustic_exact(hipStream_t, const AcousticArgs&, int);
 void launch_acoustic_fast(hipStream_t, const AcousticArgs&, int);
 
@@ -40,11 +40,11 @@ void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, real
     launch_element_strain_fast(s, a, eps, eps_tr3, with_trace);
 }
 
-void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact) {
+void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact) {
   if (exact)
-    launch_elastic_batch_exact(s, a, sc, mode);
+    launch_elastic_batch_exact(s, a, sc, mode, overlay);
   else
-    launch_elastic_batch_fast(s, a, sc, mode);
+    launch_elastic_batch_fast(s, a, sc, mode, overlay);
 }
 
 void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, int mode) {

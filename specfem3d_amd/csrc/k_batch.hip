@@ -57,22 +57,27 @@ __device__ __forceinline__ void newmark_finish(realw& u, realw& v, realw& a, rea
 }
 
 __device__ __forceinline__ realw& q4(float4& x, int r) { return r == 0 ? x.x : r == 1 ? x.y : r == 2 ? x.z : x.w; }
+__device__ __forceinline__ realw c4(const float4& x, int r) { return r == 0 ? x.x : r == 1 ? x.y : r == 2 ? x.z : x.w; }
 
 __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
   const int q = nb >> 3, r = nb & 7, x = b & 7;
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// NQ quads of 4 shots per workgroup (SC = 4 NQ shots): lane = (point slot p in 0..127, quad q),
-// threads = 128 NQ. Each lane carries its point for 4 consecutive shots: the row segment of a
-// point is one 16-byte load/store per component, the element tile reads are 16-byte LDS reads
-// (the 4 shots of one point are adjacent), and each GLL coefficient read serves 4 shots.
-template <int NQ, int MODE>
+// OV = 1 overlays the displacement tile on the stress-product planes (9 instead of 12 planes of
+// LDS, one more barrier; the lane keeps its own displacement in registers for the fused update),
+// which lets more workgroups share a CU.
+template <int NQ, int MODE, int OV>
 __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G) {
   constexpr int SC = 4 * NQ;
-  constexpr int PL = 125 * SC;                    // one plane of the tile
-  __shared__ __attribute__((aligned(16))) realw sU[3 * PL];  // displacement tile [c][p][s]
-  __shared__ __attribute__((aligned(16))) realw sT[9 * PL];  // stress products   [m][p][s]
+  // LDS tile rows (component or stress product, shot) of PST floats, point fastest. PST = 132:
+  // the rows of a lane's two quads start 16 banks apart (4 * 132 = 16 mod 32), so the 32 lanes of
+  // an LDS pass (16 consecutive points, two quads) read 32 different banks.
+  constexpr int PST = 132;
+  constexpr int CPL = SC * PST;                   // one component / product plane
+  __shared__ realw sBuf[(OV ? 9 : 12) * CPL];
+  realw* const sT = sBuf;                         // stress products   [m][s][p]
+  realw* const sU = OV ? sBuf : sBuf + 9 * CPL;   // displacement tile [c][s][p]
   __shared__ realw sG[75];                        // hprime, hprimewgll, wgllwgll
   stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -94,17 +99,20 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   const int g = B.ibool[(int64_t)e * 125 + pp];
   const int64_t row = (int64_t)g * 3 * S + sq;
   const int ts = 4 * q;                           // tile slot of the quad
+  float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;
   {
-    float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;
     if (live0) {
       u0 = *reinterpret_cast<const float4*>(B.u + row);
       u1 = *reinterpret_cast<const float4*>(B.u + row + S);
       u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
     }
     if (active) {
-      *reinterpret_cast<float4*>(sU + (0 * 125 + p) * SC + ts) = u0;
-      *reinterpret_cast<float4*>(sU + (1 * 125 + p) * SC + ts) = u1;
-      *reinterpret_cast<float4*>(sU + (2 * 125 + p) * SC + ts) = u2;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        sU[(0 * SC + ts + r) * PST + p] = c4(u0, r);
+        sU[(1 * SC + ts + r) * PST + p] = c4(u1, r);
+        sU[(2 * SC + ts + r) * PST + p] = c4(u2, r);
+      }
     }
   }
   JacPt J = {};
@@ -120,15 +128,18 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   realw t[4][9];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const Grad9 tg = tile_gradient<SC>(sU + ts + r, sU + PL + ts + r, sU + 2 * PL + ts + r, sG, i, j, k);
+    const realw* su = sU + (ts + r) * PST;
+    const Grad9 tg = tile_gradient<1>(su, su + SC * PST, su + 2 * SC * PST, sG, i, j, k);
     const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
     const Sigma6 sg = iso_stress(du, km.x, km.y);
     stress_products(sg, irr != 0, J, jacl, B.xix_regular, t[r]);
   }
+  if (OV) __syncthreads();                        // every lane is done with the displacement tile
   if (active) {
 #pragma unroll
     for (int m = 0; m < 9; m++)
-      *reinterpret_cast<float4*>(sT + (m * 125 + p) * SC + ts) = make_float4(t[0][m], t[1][m], t[2][m], t[3][m]);
+#pragma unroll
+      for (int r = 0; r < 4; r++) sT[(m * SC + ts + r) * PST + p] = t[r][m];
   }
   __syncthreads();
   if (!active || !live0) return;
@@ -151,7 +162,7 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   for (int c = 0; c < 3; c++) {
     realw f[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) f[r] = tile_divergence_c<SC>(sT + ts + r, PL, hpw, wxy, i, j, k, c);
+    for (int r = 0; r < 4; r++) f[r] = tile_divergence_c<1>(sT + (ts + r) * PST, CPL, hpw, wxy, i, j, k, c);
     if (MODE == 0) {
 #pragma unroll
       for (int r = 0; r < 4; r++)
@@ -170,7 +181,11 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
     // fused mode, last colour of this point: finish the Newmark update of component c
     const realw M = B.rmass[(int64_t)g * 3 + c];
     float4 V = *reinterpret_cast<const float4*>(B.v + row + c * S);
-    float4 U = *reinterpret_cast<const float4*>(sU + (c * 125 + p) * SC + ts);
+    float4 U = c == 0 ? u0 : c == 1 ? u1 : u2;
+    if (!OV) {
+      const realw* su = sU + (c * SC + ts) * PST + p;
+      U = make_float4(su[0], su[PST], su[2 * PST], su[3 * PST]);
+    }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       if (!live[r]) continue;
@@ -186,13 +201,15 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
 }  // namespace SF_VNS
 using namespace SF_VNS;
 
-void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int mode) {
+void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int mode, int overlay) {
   if (a.nelem <= 0 || a.ngroups <= 0) return;
   const int64_t nb = (int64_t)a.nelem * a.ngroups;
   if (nb > 0x7fffffff) fatal("launch_elastic_batch", "grid too large");
   dim3 grid((unsigned)nb);
   const GLLk G = g_gll_host;
-#define SF_KB(NQ, M) hipLaunchKernelGGL((k_elastic_batch<NQ, M>), grid, dim3(128 * NQ), 0, st, a, G)
+#define SF_KB(NQ, M)                                                                           \
+  if (overlay) hipLaunchKernelGGL((k_elastic_batch<NQ, M, 1>), grid, dim3(128 * NQ), 0, st, a, G); \
+  else hipLaunchKernelGGL((k_elastic_batch<NQ, M, 0>), grid, dim3(128 * NQ), 0, st, a, G)
 #define SF_KBM(M)                  \
   if (sc == 4) SF_KB(1, M);        \
   else if (sc == 8) SF_KB(2, M);   \

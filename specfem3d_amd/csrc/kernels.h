@@ -83,8 +83,9 @@ struct BatchArgs {
 };
 enum : unsigned char { BF_FIRST = 1, BF_LAST = 2, BF_ZERO = 4 };
 // sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour),
-// 3 one colour with the Newmark update fused (see k_batch.hip)
-void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
+// 3 one colour with the Newmark update fused (see k_batch.hip); overlay 1 shares the LDS of the
+// displacement tile and the stress products
+void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact);
 
 struct AcousticArgs {
   int nelem;

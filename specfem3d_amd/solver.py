@@ -58,7 +58,7 @@ class GPUSolver:
                  attenuation=None, exact: bool = True, assembly: int = 0, simulation_type: int = 1,
                  save_forward: bool = False, save_seismograms=("d",), nlength: int | None = None,
                  device: int = 0, tuning: dict | None = None, undo_attenuation: bool = False,
-                 element_order: str | None = "morton"):
+                 element_order: str | None = None):
         self.L = abi.load()
         self.db = db
         self.sources = sources

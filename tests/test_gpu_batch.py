@@ -26,7 +26,7 @@ def small_batch():
 def single_shot(w, i, exact, assembly):
     from specfem3d_amd.solver import GPUSolver
     src, rec, n = w.shots[i]
-    g = GPUSolver(w.db, src, rec, nstep=n, exact=exact, assembly=assembly)
+    g = GPUSolver(w.db, src, rec, nstep=n, exact=exact, assembly=assembly, element_order="morton")
     s = g.run().copy()
     u, _, _ = g.fields()
     g.close()
@@ -38,13 +38,14 @@ def single_runs(small_batch):
     return [single_shot(small_batch, i, True, 2) for i in range(len(small_batch.shots))]
 
 
-@pytest.mark.parametrize("assembly,sc", [(2, 4), (3, 4), (3, 8)])
-def test_batch_shots_bitwise_equal_single_shot_runs(small_batch, single_runs, assembly, sc):
+@pytest.mark.parametrize("assembly,sc,overlay", [(2, 4, 0), (3, 4, 0), (3, 8, 0), (3, 4, 1), (3, 8, 1)])
+def test_batch_shots_bitwise_equal_single_shot_runs(small_batch, single_runs, assembly, sc, overlay):
     from specfem3d_amd.batch import BatchSolver
     w = small_batch
     print(f"[gpu] batch of {len(w.shots)} shots, nstep {[s[2] for s in w.shots]}, assembly {assembly}, "
-          f"shots per group {sc}", flush=True)
+          f"shots per group {sc}, LDS overlay {overlay}", flush=True)
     b = BatchSolver(w.db, w.shots, exact=True, shots_per_group=sc, assembly=assembly)
+    b.set_tuning("lds_overlay", overlay)
     b.run(graph_steps=1)
     for i in range(len(w.shots)):
         ref, uref = single_runs[i]
@@ -81,7 +82,7 @@ def test_autotuned_batch_matches_oracle(small_batch, oracle_mod):
     b = BatchSolver(w.db, w.shots, exact=False)
     choice = b.autotune(trial_steps=3)
     print(f"[gpu] autotuner choice {choice}", flush=True)
-    assert choice["shots_per_group"] in (4, 8) and choice["assembly"] in (0, 2, 3)
+    assert choice["shots_per_group"] in (4, 8) and choice["assembly"] in (0, 2, 3) and choice["lds_overlay"] in (0, 1)
     b.run(graph_steps=10)
     for i, (src, rec, n) in enumerate(w.shots):
         ref = oracle_mod.OracleSolver(w.db, src, rec, "f32").run(n)
