@@ -703,8 +703,9 @@ extern "C" void sf_batch_time(long h, int nsteps, double* stiffness_ms, double* 
   b->ev_used = 0;
 }
 
-// Autotuner: times every (shots per workgroup, assembly) candidate for `trial_steps` eager steps
-// from rest, keeps the fastest and rewinds the batch. Returns the chosen step time in ms.
+// Autotuner: times every (assembly, shots per workgroup, LDS overlay) candidate for `trial_steps`
+// eager steps from rest (after a warm-up; a candidate whose first step takes over twice the best
+// so far is dropped), keeps the fastest and rewinds the batch. Returns the chosen step time in ms.
 extern "C" double sf_batch_autotune(long h, int trial_steps) {
   Batch* b = batch_of(h);
   if (!b->ready) fatal("sf_batch_autotune", "sf_batch_finalize first");
@@ -721,6 +722,8 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
       zero_state(b);
       run_steps(b, 2, 1);  // warm-up (first step runs the plain predictor)
       double ks = 0.0, st = 0.0;
+      sf_batch_time(h, 1, &ks, &st);
+      if (st > 2.0 * best) continue;  // far behind the best so far: not worth the full trial
       sf_batch_time(h, trial_steps, &ks, &st);
       if (st < best) {
         best = st;
