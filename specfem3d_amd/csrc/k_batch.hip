@@ -68,7 +68,8 @@ __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
 // LDS, one more barrier; the lane keeps its own displacement in registers for the fused update),
 // which lets more workgroups share a CU.
 template <int NQ, int MODE, int OV>
-__global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G) {
+__global__ void __launch_bounds__(128 * NQ) __attribute__((amdgpu_waves_per_eu(4)))
+k_elastic_batch(BatchArgs B, GLLk G) {
   constexpr int SC = 4 * NQ;
   // LDS tile rows (component or stress product, shot) of PST floats, point fastest. PST = 132:
   // the rows of a lane's two quads start 16 banks apart (4 * 132 = 16 mod 32), so the 32 lanes of
@@ -77,7 +78,9 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   constexpr int CPL = SC * PST;                   // one component / product plane
   __shared__ realw sBuf[(OV ? 9 : 12) * CPL];
   realw* const sT = sBuf;                         // stress products   [m][s][p]
-  realw* const sU = OV ? sBuf : sBuf + 9 * CPL;   // displacement tile [c][s][p]
+  // displacement tile [c][s][p]; with OV it shares the planes of the products t3 (m = 6..8),
+  // which are written last
+  realw* const sU = OV ? sBuf + 6 * CPL : sBuf + 9 * CPL;
   __shared__ realw sG[75];                        // hprime, hprimewgll, wgllwgll
   stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -99,20 +102,32 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   const int g = B.ibool[(int64_t)e * 125 + pp];
   const int64_t row = (int64_t)g * 3 * S + sq;
   const int ts = 4 * q;                           // tile slot of the quad
-  float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;
-  {
-    if (live0) {
-      u0 = *reinterpret_cast<const float4*>(B.u + row);
-      u1 = *reinterpret_cast<const float4*>(B.u + row + S);
-      u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
-    }
-    if (active) {
+  float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;  // staged into LDS below
+  if (live0) {
+    u0 = *reinterpret_cast<const float4*>(B.u + row);
+    u1 = *reinterpret_cast<const float4*>(B.u + row + S);
+    u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
+  }
+  // the lane's accumulator and velocity rows are loaded now, so they are in flight during the
+  // element computation (a: unless this lane writes first; v: only where it finishes the update)
+  const unsigned fl = MODE == 3 ? (unsigned)B.flags[(int64_t)e * 125 + pp] : 0u;
+  const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
+  const bool last = MODE == 3 && (fl & BF_LAST) != 0;
+  const bool mine = active && live0;
+  float4 A[3], V[3];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        sU[(0 * SC + ts + r) * PST + p] = c4(u0, r);
-        sU[(1 * SC + ts + r) * PST + p] = c4(u1, r);
-        sU[(2 * SC + ts + r) * PST + p] = c4(u2, r);
-      }
+  for (int c = 0; c < 3; c++) {
+    A[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    V[c] = A[c];
+    if (MODE != 0 && mine && (!first || !live[3])) A[c] = *reinterpret_cast<const float4*>(B.a + row + c * S);
+    if (last && mine) V[c] = *reinterpret_cast<const float4*>(B.v + row + c * S);
+  }
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      sU[(0 * SC + ts + r) * PST + p] = c4(u0, r);
+      sU[(1 * SC + ts + r) * PST + p] = c4(u1, r);
+      sU[(2 * SC + ts + r) * PST + p] = c4(u2, r);
     }
   }
   JacPt J = {};
@@ -125,38 +140,42 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
   __syncthreads();
 
   const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
-  realw t[4][9];
+  realw t3[4][3];                                 // products t3 of each shot (written last with OV)
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const realw* su = sU + (ts + r) * PST;
     const Grad9 tg = tile_gradient<1>(su, su + SC * PST, su + 2 * SC * PST, sG, i, j, k);
     const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
     const Sigma6 sg = iso_stress(du, km.x, km.y);
-    stress_products(sg, irr != 0, J, jacl, B.xix_regular, t[r]);
+    realw t[9];
+    stress_products(sg, irr != 0, J, jacl, B.xix_regular, t);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < (OV ? 6 : 9); m++) sT[(m * SC + ts + r) * PST + p] = t[m];
+    }
+    t3[r][0] = t[6]; t3[r][1] = t[7]; t3[r][2] = t[8];
   }
-  if (OV) __syncthreads();                        // every lane is done with the displacement tile
-  if (active) {
+  if (OV) {
+    __syncthreads();                              // every lane is done with the displacement tile
+    if (active) {
 #pragma unroll
-    for (int m = 0; m < 9; m++)
+      for (int r = 0; r < 4; r++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) sT[(m * SC + ts + r) * PST + p] = t[r][m];
+        for (int m = 6; m < 9; m++) sT[(m * SC + ts + r) * PST + p] = t3[r][m - 6];
+    }
   }
   __syncthreads();
-  if (!active || !live0) return;
+  if (!mine) return;
 
   // one force component at a time for the lane's four shots (keeps few values live)
   realw* a = B.a + row;
   const realw* hpw = sG + 25;
   const realw* wxy = sG + 50;
-  unsigned fl = 0;
   bool cont[4], ends = false;
-  if (MODE == 3) {
-    fl = B.flags[(int64_t)e * 125 + p];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      cont[r] = live[r] && B.nstep[sq + r] > it;
-      ends = ends || (live[r] && !cont[r]);
-    }
+  for (int r = 0; r < 4; r++) {
+    cont[r] = live[r] && B.nstep[sq + r] > it;
+    ends = ends || (live[r] && !cont[r]);
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -169,32 +188,34 @@ __global__ void __launch_bounds__(128 * NQ) k_elastic_batch(BatchArgs B, GLLk G)
         if (live[r]) atomicAdd(a + c * S + r, -f[r]);
       continue;
     }
-    const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
-    float4 A = (!first || !live[3]) ? *reinterpret_cast<const float4*>(a + c * S) : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 Ac = A[c];
 #pragma unroll
     for (int r = 0; r < 4; r++)
-      if (live[r]) q4(A, r) = (first ? 0.f : q4(A, r)) + (-f[r]);
-    if (MODE == 2 || !(fl & BF_LAST)) {
-      *reinterpret_cast<float4*>(a + c * S) = A;
+      if (live[r]) q4(Ac, r) = (first ? 0.f : q4(Ac, r)) + (-f[r]);
+    if (MODE == 2 || !last) {
+      *reinterpret_cast<float4*>(a + c * S) = Ac;
       continue;
     }
     // fused mode, last colour of this point: finish the Newmark update of component c
     const realw M = B.rmass[(int64_t)g * 3 + c];
-    float4 V = *reinterpret_cast<const float4*>(B.v + row + c * S);
-    float4 U = c == 0 ? u0 : c == 1 ? u1 : u2;
-    if (!OV) {
+    float4 Vc = V[c];
+    // this step's displacement: the LDS tile without overlay, else the row again (still in cache)
+    float4 U;
+    if (OV) {
+      U = *reinterpret_cast<const float4*>(B.u + row + c * S);
+    } else {
       const realw* su = sU + (c * SC + ts) * PST + p;
       U = make_float4(su[0], su[PST], su[2 * PST], su[3 * PST]);
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       if (!live[r]) continue;
-      newmark_finish(q4(U, r), q4(V, r), q4(A, r), M, B.dt, B.dt2, B.dtsq2, cont[r]);
-      if (cont[r]) q4(A, r) = 0.f;
+      newmark_finish(q4(U, r), q4(Vc, r), q4(Ac, r), M, B.dt, B.dt2, B.dtsq2, cont[r]);
+      if (cont[r]) q4(Ac, r) = 0.f;
     }
     if (cont[0]) *reinterpret_cast<float4*>(B.u + row + c * S) = U;
-    *reinterpret_cast<float4*>(B.v + row + c * S) = V;
-    if ((fl & BF_ZERO) || ends) *reinterpret_cast<float4*>(a + c * S) = A;
+    *reinterpret_cast<float4*>(B.v + row + c * S) = Vc;
+    if ((fl & BF_ZERO) || ends) *reinterpret_cast<float4*>(a + c * S) = Ac;
   }
 }
 
