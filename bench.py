@@ -246,7 +246,8 @@ def main():
     ap.add_argument("--exact", type=int, default=0, help="1: kernels without FMA (CPU rounding)")
     ap.add_argument("--assembly", type=int, default=0,
                     help="0 atomic scatter, 1 ordered (CPU element order), 2 coloured (no atomics)")
-    ap.add_argument("--element-order", default="none", choices=["morton", "none"],
+    ap.add_argument("--element-order", default="none",
+                    choices=["morton", "none", "brick1x2", "brick2x1", "brick2x2", "brick1x4", "brick4x4", "brick2x4"],
                     help="launch order of the elastic elements of the single-shot run")
     ap.add_argument("--skip-single", action="store_true",
                     help="skip the single-shot 96x96x48 measurement (profiling runs of the other parts)")

@@ -43,6 +43,26 @@ def morton_order(db, elems: np.ndarray) -> np.ndarray:
     return elems[np.argsort(key, kind="stable")].astype(np.int32)
 
 
+def brick_order(db, elems: np.ndarray, by: int = 2, bz: int = 2) -> np.ndarray:
+    """``elems`` in slab order: (z // bz, y // by, x, y % by, z % bz) of their centres on the element
+    grid. Consecutive launches then walk by * bz rows of the natural numbering side by side, so the
+    per-element state arrays are still read as a few sequential streams, while the neighbours across
+    y and z faces run within a few launches of each other (their shared accumulator lines meet in
+    cache)."""
+    elems = np.asarray(elems, np.int64)
+    if elems.size == 0:
+        return elems.astype(np.int32)
+    ib = db.ibool.reshape(db.nspec, 125)[elems]
+    c = np.stack([db.xstore[ib].mean(1), db.ystore[ib].mean(1), db.zstore[ib].mean(1)], 1).astype(np.float64)
+    # element grid indices from the centres and the smallest element extent along each axis
+    ext = np.stack([db.xstore[ib].max(1) - db.xstore[ib].min(1), db.ystore[ib].max(1) - db.ystore[ib].min(1),
+                    db.zstore[ib].max(1) - db.zstore[ib].min(1)], 1).astype(np.float64)
+    h = np.maximum(ext.min(0), 1e-30)
+    ix, iy, iz = (np.floor((c[:, d] - c[:, d].min()) / h[d] + 0.5).astype(np.int64) for d in range(3))
+    key = np.lexsort((iz % bz, iy % by, ix, iy // by, iz // bz))
+    return elems[key].astype(np.int32)
+
+
 def _fortran_phase_list(elems: np.ndarray) -> tuple[int, np.ndarray]:
     """phase_ispec_inner_*(num, 2): column 1 outer (none at NPROC=1), column 2 inner, 1-based."""
     n = max(len(elems), 1)
@@ -187,8 +207,14 @@ class GPUSolver:
                         self.MP, A.i(nrec), A.i(self.nstep), A.fa(R.hxir.astype(np.float32).ravel()),
                         A.fa(R.hetar.astype(np.float32).ravel()), A.fa(R.hgammar.astype(np.float32).ravel()),
                         A.i(nrec), A.ia(np.zeros(nrec, np.int32)), A.ia(rec_ispec))
-        if self.elastic and self.element_order == "morton":
-            order = morton_order(db, db.elastic_elements())
+        if self.elastic and self.element_order:
+            if self.element_order == "morton":
+                order = morton_order(db, db.elastic_elements())
+            elif self.element_order.startswith("brick"):
+                by, bz = (int(v) for v in self.element_order[5:].split("x"))
+                order = brick_order(db, db.elastic_elements(), by, bz)
+            else:
+                raise ValueError(f"unknown element order {self.element_order!r}")
             L.sf_set_element_order(self.MP, C.c_int(len(order)), abi.ptr(order))
         n3 = 3 * db.nglob
         self.displ = np.zeros(n3, np.float32)

@@ -69,6 +69,31 @@ def test_morton_launch_order_is_a_local_permutation():
     check_proper(db, mo, col)
 
 
+def test_brick_launch_order_interleaves_few_sequential_rows():
+    """Slab ("brick") launch order (z // bz, y // by, x, y % by, z % bz): a permutation in which every
+    y and z face neighbour within a slab is at most by * bz * nx launches away, and the element ids
+    visited form by * bz increasing sequences (sequential streams of the per-element arrays)."""
+    from specfem3d_amd.solver import brick_order
+    nx = 6
+    db = build_box(nx, 4, 4, (0, 6000, 0, 4000, -4000, 0), material=homogeneous(2300, 2800, 1500))
+    ref = np.arange(db.nspec)
+    for by, bz in ((1, 2), (2, 2), (2, 4)):
+        o = brick_order(db, ref, by, bz)
+        assert sorted(o.tolist()) == ref.tolist()
+        ex, ey, ez = o % nx, (o // nx) % 4, o // (nx * 4)
+        streams = {}
+        for e, y, z in zip(o, ey, ez):
+            streams.setdefault((y % by, z % bz), []).append(e)
+        assert len(streams) == by * bz
+        assert all(np.all(np.diff(v) > 0) for v in streams.values())
+        pos = np.empty(db.nspec, np.int64)
+        pos[o] = np.arange(db.nspec)
+        e = ref
+        x, y, z = e % nx, (e // nx) % 4, e // (nx * 4)
+        inner_z = (z % bz) < bz - 1
+        assert np.all(np.abs(pos[e[inner_z] + nx * 4] - pos[e[inner_z]]) <= by * bz)
+
+
 def point_flags(db, elems, col, forced):
     lib = abi.load()
     ib = np.ascontiguousarray(db.ibool.reshape(db.nspec, 125).astype(np.int32))
