@@ -476,6 +476,27 @@ static void build_factor_table(Mesh* m, const realw* fc, const realw* fck, int64
 static void build_colors(Mesh* m, const std::vector<int>& elems, int*& d_list, int* off, int& ncolor);
 }  // namespace sf
 
+namespace sf {
+// Host arrays in the reference's layouts -> device per-element blocks (r_idx / eps_idx):
+// epsilondev_*(NGLL3, NSPEC), one array per component (a null component stays zero), and
+// R_*(N_SLS, NGLL3, NSPEC), SLS fastest.
+static realw* upload_strain_blocks(const realw* const src[6], int64_t nloc) {
+  std::vector<realw> E(6 * (size_t)nloc, 0.f);
+  for (int c = 0; c < 6; c++) {
+    if (!src[c]) continue;
+    for (int64_t q = 0; q < nloc; q++) E[(size_t)eps_idx(q / 125, c, (int)(q % 125))] = src[c][q];
+  }
+  return dupload(E.data(), 6 * nloc);
+}
+static realw* upload_memory_blocks(const realw* const rs[6], int64_t nloc) {
+  std::vector<realw> R(18 * (size_t)nloc);
+  for (int c = 0; c < 6; c++)
+    for (int64_t q = 0; q < nloc; q++)
+      for (int s = 0; s < 3; s++) R[(size_t)r_idx(q / 125, c * 3 + s, (int)(q % 125))] = rs[c][3 * q + s];
+  return dupload(R.data(), 18 * nloc);
+}
+}  // namespace sf
+
 // prepare_mesh_constants_cuda.cu:657
 extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_DEVICE)(
     long* Mesh_pointer, realw* rmassx, realw* rmassy, realw* rmassz, realw* rho_vp, realw* rho_vs, realw* h_kappav,
@@ -566,20 +587,14 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
   m->compute_and_store_strain = *COMPUTE_AND_STORE_STRAIN;
   m->attenuation = *ATTENUATION;
   if (m->compute_and_store_strain || m->attenuation) {
-    std::vector<realw> E(6 * (size_t)nloc, 0.f);
-    const realw* src[5] = {epsilondev_xx, epsilondev_yy, epsilondev_xy, epsilondev_xz, epsilondev_yz};
-    for (int c = 0; c < 5; c++) std::memcpy(&E[(size_t)c * nloc], src[c], sizeof(realw) * (size_t)nloc);
-    if (m->attenuation) std::memcpy(&E[5 * (size_t)nloc], epsilondev_trace, sizeof(realw) * (size_t)nloc);
-    m->att[0].eps = dupload(E.data(), 6 * nloc);
+    const realw* src[6] = {epsilondev_xx, epsilondev_yy, epsilondev_xy, epsilondev_xz, epsilondev_yz,
+                           m->attenuation ? epsilondev_trace : nullptr};
+    m->att[0].eps = upload_strain_blocks(src, nloc);
   }
   if (m->attenuation) {
     if (*R_size != 3 * nloc) fatal("prepare_fields_elastic_device", "R_size must be N_SLS*125*NSPEC");
-    std::vector<realw> R(18 * (size_t)nloc);
     const realw* rs[6] = {R_xx, R_yy, R_xy, R_xz, R_yz, R_trace};
-    for (int c = 0; c < 6; c++)
-      for (int64_t q = 0; q < nloc; q++)
-        for (int s = 0; s < 3; s++) R[((size_t)c * 3 + s) * nloc + q] = rs[c][3 * q + s];
-    m->att[0].R = dupload(R.data(), 18 * nloc);
+    m->att[0].R = upload_memory_blocks(rs, nloc);
     std::vector<realw> F(3 * (size_t)nloc), Fk(3 * (size_t)nloc);
     for (int64_t q = 0; q < nloc; q++)
       for (int s = 0; s < 3; s++) {
@@ -789,11 +804,9 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_adj_dev, PREPARE_FIELDS_ELASTIC_
   m->eps_trace_over_3[0] = dupload(epsilon_trace_over_3, nloc);
   m->eps_trace_over_3[1] = dupload(b_epsilon_trace_over_3, nloc);
   {
-    std::vector<realw> E(6 * (size_t)nloc, 0.f);
-    const realw* src[5] = {b_epsilondev_xx, b_epsilondev_yy, b_epsilondev_xy, b_epsilondev_xz, b_epsilondev_yz};
-    for (int c = 0; c < 5; c++) std::memcpy(&E[(size_t)c * nloc], src[c], sizeof(realw) * (size_t)nloc);
-    if (m->attenuation) std::memcpy(&E[5 * (size_t)nloc], b_epsilondev_trace, sizeof(realw) * (size_t)nloc);
-    m->att[1].eps = dupload(E.data(), 6 * nloc);
+    const realw* src[6] = {b_epsilondev_xx, b_epsilondev_yy, b_epsilondev_xy, b_epsilondev_xz, b_epsilondev_yz,
+                           m->attenuation ? b_epsilondev_trace : nullptr};
+    m->att[1].eps = upload_strain_blocks(src, nloc);
   }
   if (!m->att[0].eps) {
     m->att[0].eps = dmalloc<realw>(6 * nloc);
@@ -801,12 +814,8 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_adj_dev, PREPARE_FIELDS_ELASTIC_
   }
   if (m->attenuation) {
     if (*R_size != 3 * nloc) fatal("prepare_fields_elastic_adj_dev", "R_size must be N_SLS*125*NSPEC");
-    std::vector<realw> R(18 * (size_t)nloc);
     const realw* rs[6] = {b_R_xx, b_R_yy, b_R_xy, b_R_xz, b_R_yz, b_R_trace};
-    for (int c = 0; c < 6; c++)
-      for (int64_t q = 0; q < nloc; q++)
-        for (int s = 0; s < 3; s++) R[((size_t)c * 3 + s) * nloc + q] = rs[c][3 * q + s];
-    m->att[1].R = dupload(R.data(), 18 * nloc);
+    m->att[1].R = upload_memory_blocks(rs, nloc);
     for (int s = 0; s < 3; s++) {
       m->b_alphaval[s] = b_alphaval[s];
       m->b_betaval[s] = b_betaval[s];

@@ -658,10 +658,13 @@ extern "C" void FC_FUNC_(transfer_strain_from_device, TRANSFER_STRAIN_FROM_DEVIC
     realw* epsilondev_yz, realw* epsilondev_trace, int* size_epsilondev) {
   Mesh* m = from_handle(Mesh_pointer);
   if (!m->att[0].eps) fatal("transfer_strain_from_device", "strain arrays not prepared");
-  const int64_t nl = (int64_t)m->nspec * 125, n = *size_epsilondev;
+  const int64_t nl = (int64_t)m->nspec * 125, n = std::min<int64_t>(*size_epsilondev, nl);
   realw* dst[6] = {epsilondev_xx, epsilondev_yy, epsilondev_xy, epsilondev_xz, epsilondev_yz, epsilondev_trace};
+  std::vector<realw> E(6 * (size_t)nl);
+  d2h(m, E.data(), m->att[0].eps, 6 * nl);
   for (int c = 0; c < 6; c++)
-    if (dst[c]) d2h(m, dst[c], m->att[0].eps + c * nl, n);
+    if (dst[c])
+      for (int64_t q = 0; q < n; q++) dst[c][q] = E[(size_t)eps_idx(q / 125, c, (int)(q % 125))];
 }
 
 extern "C" void FC_FUNC_(transfer_b_strain_to_device, TRANSFER_B_strain_TO_DEVICE)(
@@ -669,11 +672,15 @@ extern "C" void FC_FUNC_(transfer_b_strain_to_device, TRANSFER_B_strain_TO_DEVIC
     realw* b_epsilondev_xz, realw* b_epsilondev_yz, realw* b_epsilondev_trace, int* size_epsilondev) {
   Mesh* m = from_handle(Mesh_pointer);
   if (!m->att[1].eps) fatal("transfer_b_strain_to_device", "backward strain arrays not prepared");
-  const int64_t nl = (int64_t)m->nspec * 125, n = *size_epsilondev;
+  const int64_t nl = (int64_t)m->nspec * 125, n = std::min<int64_t>(*size_epsilondev, nl);
   realw* src[6] = {b_epsilondev_xx, b_epsilondev_yy, b_epsilondev_xy, b_epsilondev_xz, b_epsilondev_yz,
                    b_epsilondev_trace};
+  std::vector<realw> E(6 * (size_t)nl);
+  d2h(m, E.data(), m->att[1].eps, 6 * nl);   // components not passed keep their device values
   for (int c = 0; c < 6; c++)
-    if (src[c]) h2d(m, m->att[1].eps + c * nl, src[c], n);
+    if (src[c])
+      for (int64_t q = 0; q < n; q++) E[(size_t)eps_idx(q / 125, c, (int)(q % 125))] = src[c][q];
+  h2d(m, m->att[1].eps, E.data(), 6 * nl);
 }
 
 static void rmem_copy(Mesh* m, int f, realw* const* arrs, int64_t nR, bool to_device) {
@@ -685,7 +692,7 @@ static void rmem_copy(Mesh* m, int f, realw* const* arrs, int64_t nR, bool to_de
   for (int c = 0; c < 6; c++)
     for (int64_t q = 0; q < nl; q++)
       for (int s = 0; s < 3; s++) {
-        realw& pv = plane[((size_t)c * 3 + s) * nl + q];
+        realw& pv = plane[(size_t)r_idx(q / 125, c * 3 + s, (int)(q % 125))];
         if (to_device) pv = arrs[c][3 * q + s];
         else arrs[c][3 * q + s] = pv;
       }

@@ -68,7 +68,7 @@ __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
 // LDS, one more barrier; the lane keeps its own displacement in registers for the fused update),
 // which lets more workgroups share a CU.
 template <int NQ, int MODE, int OV>
-__global__ void __launch_bounds__(128 * NQ) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(128 * NQ) __attribute__((amdgpu_waves_per_eu(OV ? 4 : 3)))
 k_elastic_batch(BatchArgs B, GLLk G) {
   constexpr int SC = 4 * NQ;
   // LDS tile rows (component or stress product, shot) of PST floats, point fastest. PST = 132:

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
 #pragma unroll
       for (int cc = 0; cc < 6; cc++)
 #pragma unroll
-        for (int s = 0; s < 3; s++) Rv[cc][s] = R[(cc * 3 + s) * nl + ep];
+        for (int s = 0; s < 3; s++) Rv[cc][s] = R[r_idx(e, cc * 3 + s, p)];
       const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
       const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
       const realw rtr = (Rv[5][0] + Rv[5][1]) + Rv[5][2];
@@ -215,25 +215,24 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
       sigma_yz = sigma_yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
       // memory-variable update (second-order RK) with old strain Sn and new strain Snp1
       realw* Ep = A.eps;
-      const realw Sn[6] = {Ep[0 * nl + ep], Ep[1 * nl + ep], Ep[2 * nl + ep], Ep[3 * nl + ep], Ep[4 * nl + ep],
-                           Ep[5 * nl + ep]};
+      const realw Sn[6] = {Ep[eps_idx(e, 0, p)], Ep[eps_idx(e, 1, p)], Ep[eps_idx(e, 2, p)], Ep[eps_idx(e, 3, p)],
+                           Ep[eps_idx(e, 4, p)], Ep[eps_idx(e, 5, p)]};
       const realw Snp1[6] = {e_xx, e_yy, e_xy, e_xz, e_yz, e_tr};
 #pragma unroll
       for (int s = 0; s < 3; s++) {
         const realw flk = kappal * A.fck[s * nl + ep];
-        R[(5 * 3 + s) * nl + ep] = A.alpha[s] * Rv[5][s] + flk * (A.beta[s] * Sn[5] + A.gamma[s] * Snp1[5]);
+        R[r_idx(e, 5 * 3 + s, p)] = A.alpha[s] * Rv[5][s] + flk * (A.beta[s] * Sn[5] + A.gamma[s] * Snp1[5]);
         const realw flm = mul * A.fc[s * nl + ep];
 #pragma unroll
         for (int cc = 0; cc < 5; cc++)
-          R[(cc * 3 + s) * nl + ep] = A.alpha[s] * Rv[cc][s] + flm * (A.beta[s] * Sn[cc] + A.gamma[s] * Snp1[cc]);
+          R[r_idx(e, cc * 3 + s, p)] = A.alpha[s] * Rv[cc][s] + flm * (A.beta[s] * Sn[cc] + A.gamma[s] * Snp1[cc]);
       }
     }
     if (STRAIN) {
       realw* Ep = A.eps;
-      const int64_t nl = A.nloc;
-      Ep[0 * nl + ep] = e_xx; Ep[1 * nl + ep] = e_yy; Ep[2 * nl + ep] = e_xy;
-      Ep[3 * nl + ep] = e_xz; Ep[4 * nl + ep] = e_yz;
-      if (ATT) Ep[5 * nl + ep] = e_tr;
+      Ep[eps_idx(e, 0, p)] = e_xx; Ep[eps_idx(e, 1, p)] = e_yy; Ep[eps_idx(e, 2, p)] = e_xy;
+      Ep[eps_idx(e, 3, p)] = e_xz; Ep[eps_idx(e, 4, p)] = e_yz;
+      if (ATT) Ep[eps_idx(e, 5, p)] = e_tr;
     }
 
     const realw syx = sigma_xy, szx = sigma_xz, szy = sigma_yz;
@@ -314,10 +313,10 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
 }
 
 // stored strain (6 planes) and SLS factors (table row or planes) of one point
-__device__ __forceinline__ void load_att_state(const ElasticArgs& A, int64_t ep, int64_t nl, realw Sn[6],
+__device__ __forceinline__ void load_att_state(const ElasticArgs& A, int e, int q, int64_t ep, int64_t nl, realw Sn[6],
                                                realw fcv[3], realw fckv[3]) {
 #pragma unroll
-  for (int cc = 0; cc < 6; cc++) Sn[cc] = A.eps[cc * nl + ep];
+  for (int cc = 0; cc < 6; cc++) Sn[cc] = A.eps[eps_idx(e, cc, q)];
   if (A.fc_index) {
     const int r = A.fc_index[ep];
 #pragma unroll
@@ -383,8 +382,8 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
 #pragma unroll
     for (int cc = 0; cc < 6; cc++)
 #pragma unroll
-      for (int s = 0; s < 3; s++) Rv[cc][s] = A.R[(cc * 3 + s) * nl + ep];
-    if (!LATE) load_att_state(A, ep, nl, Sn, fcv, fckv);
+      for (int s = 0; s < 3; s++) Rv[cc][s] = A.R[r_idx(e, cc * 3 + s, q)];
+    if (!LATE) load_att_state(A, e, q, ep, nl, Sn, fcv, fckv);
   }
   __syncthreads();
 
@@ -422,24 +421,24 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
     sg.yz = sg.yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
     if (active) {
       // memory-variable update (second-order RK) with old strain Sn and new strain Snp1
-      if (LATE) load_att_state(A, ep, nl, Sn, fcv, fckv);
+      if (LATE) load_att_state(A, e, q, ep, nl, Sn, fcv, fckv);
       const realw Snp1[6] = {e_xx, e_yy, e_xy, e_xz, e_yz, e_tr};
 #pragma unroll
       for (int s = 0; s < 3; s++) {
         const realw flk = kappal * fckv[s];
-        A.R[(5 * 3 + s) * nl + ep] = A.alpha[s] * Rv[5][s] + flk * (A.beta[s] * Sn[5] + A.gamma[s] * Snp1[5]);
+        A.R[r_idx(e, 5 * 3 + s, q)] = A.alpha[s] * Rv[5][s] + flk * (A.beta[s] * Sn[5] + A.gamma[s] * Snp1[5]);
         const realw flm = mul * fcv[s];
 #pragma unroll
         for (int cc = 0; cc < 5; cc++)
-          A.R[(cc * 3 + s) * nl + ep] = A.alpha[s] * Rv[cc][s] + flm * (A.beta[s] * Sn[cc] + A.gamma[s] * Snp1[cc]);
+          A.R[r_idx(e, cc * 3 + s, q)] = A.alpha[s] * Rv[cc][s] + flm * (A.beta[s] * Sn[cc] + A.gamma[s] * Snp1[cc]);
       }
     }
   }
   if (STRAIN && active) {
     realw* Ep = A.eps;
-    Ep[0 * nl + ep] = e_xx; Ep[1 * nl + ep] = e_yy; Ep[2 * nl + ep] = e_xy;
-    Ep[3 * nl + ep] = e_xz; Ep[4 * nl + ep] = e_yz;
-    if (ATT) Ep[5 * nl + ep] = e_tr;
+    Ep[eps_idx(e, 0, q)] = e_xx; Ep[eps_idx(e, 1, q)] = e_yy; Ep[eps_idx(e, 2, q)] = e_xy;
+    Ep[eps_idx(e, 3, q)] = e_xz; Ep[eps_idx(e, 4, q)] = e_yz;
+    if (ATT) Ep[eps_idx(e, 5, q)] = e_tr;
   }
 
   {
@@ -538,16 +537,15 @@ __global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, r
     }
     const realw templ = ONE_THIRD * (duxdx + duydy + duzdz);
     const int64_t ep = (int64_t)e * 125 + p;
-    const int64_t nl = A.nloc;
     eps_tr3[ep] = templ;
-    eps[0 * nl + ep] = duxdx - templ;
-    eps[1 * nl + ep] = duydy - templ;
-    eps[2 * nl + ep] = (realw)0.5 * (duxdy + duydx);
-    eps[3 * nl + ep] = (realw)0.5 * (duzdx + duxdz);
-    eps[4 * nl + ep] = (realw)0.5 * (duzdy + duydz);
+    eps[eps_idx(e, 0, p)] = duxdx - templ;
+    eps[eps_idx(e, 1, p)] = duydy - templ;
+    eps[eps_idx(e, 2, p)] = (realw)0.5 * (duxdy + duydx);
+    eps[eps_idx(e, 3, p)] = (realw)0.5 * (duzdx + duxdz);
+    eps[eps_idx(e, 4, p)] = (realw)0.5 * (duzdy + duydz);
     // with attenuation the trace plane feeds the bulk memory variables (compute_strain_att,
     // compute_element_strain.F90:196: epsilondev_trace = 3 * epsilon_trace_over_3)
-    if (with_trace) eps[5 * nl + ep] = (realw)3 * templ;
+    if (with_trace) eps[eps_idx(e, 5, p)] = (realw)3 * templ;
   }
 }
 

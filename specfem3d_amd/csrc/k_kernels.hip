@@ -22,9 +22,10 @@ __global__ void k_compute_kernels_el(int nelem, const int* __restrict__ elems, c
   const realw dot = accel[3 * g] * b_displ[3 * g] + accel[3 * g + 1] * b_displ[3 * g + 1] +
                     accel[3 * g + 2] * b_displ[3 * g + 2];
   rho_kl[ep] = rho_kl[ep] + deltat * dot;
-  const realw e1 = eps[ep], e2 = eps[nl + ep], e3 = eps[2 * nl + ep], e4 = eps[3 * nl + ep], e5 = eps[4 * nl + ep];
-  const realw b1 = beps[ep], b2 = beps[nl + ep], b3 = beps[2 * nl + ep], b4 = beps[3 * nl + ep],
-              b5 = beps[4 * nl + ep];
+  const realw e1 = eps[eps_idx(e, 0, p)], e2 = eps[eps_idx(e, 1, p)], e3 = eps[eps_idx(e, 2, p)],
+              e4 = eps[eps_idx(e, 3, p)], e5 = eps[eps_idx(e, 4, p)];
+  const realw b1 = beps[eps_idx(e, 0, p)], b2 = beps[eps_idx(e, 1, p)], b3 = beps[eps_idx(e, 2, p)],
+              b4 = beps[eps_idx(e, 3, p)], b5 = beps[eps_idx(e, 4, p)];
   mu_kl[ep] = mu_kl[ep] + deltat * (e1 * b1 + e2 * b2 + (e1 + e2) * (b1 + b2) + (realw)2 * (e3 * b3 + e4 * b4 + e5 * b5));
   kappa_kl[ep] = kappa_kl[ep] + deltat * ((realw)9 * etr3[ep] * btr3[ep]);
 }

@@ -7,6 +7,13 @@ namespace sf {
 
 // GLL constants passed to the element kernels by value (kernel-argument segment, read through
 // the scalar cache): hp[i*5+l] = l'_l(x_i), hpw[i*5+l] = hp[i*5+l] w_i, wxy[a*5+b] = w_a w_b.
+// Attenuation state of the elastic elements, in per-element blocks: the 18 memory-variable planes
+// (component-major, SLS second) and the 6 strain planes of one element are contiguous, 125 points
+// per plane. A stiffness launch in element order then reads and writes the state as a few
+// sequential streams, and each plane of an element is one coalesced 500-byte run.
+__host__ __device__ __forceinline__ int64_t r_idx(int64_t e, int plane, int p) { return (e * 18 + plane) * 125 + p; }
+__host__ __device__ __forceinline__ int64_t eps_idx(int64_t e, int c, int p) { return (e * 6 + c) * 125 + p; }
+
 struct GLLk {
   realw hp[25];
   realw hpw[25];
