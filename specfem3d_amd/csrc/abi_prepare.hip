@@ -961,7 +961,7 @@ extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
     if (value < 0 || value > 1) fatal("sf_set_tuning", "elastic_shape must be 0 or 1");
     m->elastic_shape = value;
   } else if (std::strcmp(name, "att_variant") == 0) {
-    if (value < 0 || value > 2) fatal("sf_set_tuning", "att_variant must be 0, 1 or 2");
+    if (value < 0 || value > 4) fatal("sf_set_tuning", "att_variant must be 0..4");
     m->att_variant = value;
   } else if (std::strcmp(name, "acoustic_kernel_field") == 0) {
     // which adjoint array compute_kernels_acoustic_cuda pairs with the reconstructed field:

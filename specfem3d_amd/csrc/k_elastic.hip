@@ -313,6 +313,24 @@ __global__ void __launch_bounds__(128) k_elastic(ElasticArgs A, GLLk G) {
 }
 
 // stored strain (6 planes) and SLS factors (table row or planes) of one point
+__device__ __forceinline__ void load_factors(const ElasticArgs& A, int64_t ep, int64_t nl, realw fcv[3],
+                                             realw fckv[3]) {
+  if (A.fc_index) {
+    const int r = A.fc_index[ep];
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = A.fc_table[s * A.fc_rows + r];
+      fckv[s] = A.fc_table[(3 + s) * A.fc_rows + r];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = A.fc[s * nl + ep];
+      fckv[s] = A.fck[s * nl + ep];
+    }
+  }
+}
+
 __device__ __forceinline__ void load_att_state(const ElasticArgs& A, int e, int q, int64_t ep, int64_t nl, realw Sn[6],
                                                realw fcv[3], realw fckv[3]) {
 #pragma unroll
@@ -342,14 +360,22 @@ __device__ __forceinline__ void load_att_state(const ElasticArgs& A, int e, int 
 // AV selects how the attenuation state is scheduled (the autotuner's choice; same results):
 // 0 every per-point array is loaded before the first barrier; 1 only the memory variables are,
 // the stored strain and SLS factors are loaded where the memory update needs them (fewer live
-// registers); 2 as 1 with the register budget capped for 5 waves per SIMD.
+// registers); 2 as 1 with the register budget capped for 5 waves per SIMD; 3 as 0 with that cap;
+// 4 loads the memory variables and stored strain straight into LDS (global_load_lds, no
+// registers held while they are in flight; each lane reads back its own point), which frees
+// registers for 5 waves per SIMD; the stress products then reuse those LDS planes.
+typedef __attribute__((address_space(3))) void* sf_lds_ptr;
+typedef __attribute__((address_space(1))) void* sf_global_ptr;
+
 template <bool STRAIN, bool ATT, int MODE, int AV>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(AV == 2 ? 5 : 1)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(AV >= 2 ? 5 : 1)))
 k_elastic_pt(ElasticArgs A, GLLk G) {
   constexpr bool LATE = ATT && (AV == 1 || AV == 2);
-  __shared__ realw sU[6][128];  // u (0..2) and dt*v (3..5)
-  __shared__ realw sT[9][128];  // stress products t1 (0..2), t2 (3..5), t3 (6..8)
-  __shared__ realw sG[75];      // hprime, hprimewgll, wgllwgll
+  constexpr bool DL = ATT && AV == 4;
+  __shared__ realw sU[6][128];            // u (0..2) and dt*v (3..5)
+  __shared__ realw sX[DL ? 24 : 9][128];  // stress products (planes 0..8); with DL first the state
+  realw(*const sT)[128] = sX;             // t1 (0..2), t2 (3..5), t3 (6..8)
+  __shared__ realw sG[75];                // hprime, hprimewgll, wgllwgll
   const int p = threadIdx.x;
   const bool active = p < 125;
   stage_gll(sG, G, p);
@@ -360,15 +386,6 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
   const int64_t ep = (int64_t)e * 125 + q;
   const int64_t nl = A.nloc;
   const int g = A.ibool[ep];
-  {
-    const realw* d = A.displ + 3 * (int64_t)g;
-    const realw u0 = d[0], u1 = d[1], u2 = d[2];
-    sU[0][p] = u0; sU[1][p] = u1; sU[2][p] = u2;
-    if (ATT) {
-      const realw* v = A.veloc + 3 * (int64_t)g;
-      sU[3][p] = A.deltat * v[0]; sU[4][p] = A.deltat * v[1]; sU[5][p] = A.deltat * v[2];
-    }
-  }
   const int irr = A.irregular[e];
   JacPt J = {};
   realw jacl = A.jacobian_regular;
@@ -378,14 +395,43 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
   }
   const float2 km = A.kappamu[ep];
   realw Rv[6][3], Sn[6], fcv[3], fckv[3];
-  if (ATT) {
+  if (ATT && !DL) {
 #pragma unroll
     for (int cc = 0; cc < 6; cc++)
 #pragma unroll
       for (int s = 0; s < 3; s++) Rv[cc][s] = A.R[r_idx(e, cc * 3 + s, q)];
     if (!LATE) load_att_state(A, e, q, ep, nl, Sn, fcv, fckv);
   }
+  if (DL) load_factors(A, ep, nl, fcv, fckv);
+  {
+    const realw* d = A.displ + 3 * (int64_t)g;
+    const realw u0 = d[0], u1 = d[1], u2 = d[2];
+    realw v0 = 0, v1 = 0, v2 = 0;
+    if (ATT) {
+      const realw* v = A.veloc + 3 * (int64_t)g;
+      v0 = v[0]; v1 = v[1]; v2 = v[2];
+    }
+    sU[0][p] = u0; sU[1][p] = u1; sU[2][p] = u2;
+    if (ATT) {
+      sU[3][p] = A.deltat * v0; sU[4][p] = A.deltat * v1; sU[5][p] = A.deltat * v2;
+    }
+  }
   __syncthreads();
+  if (DL) {
+    // after the barrier (whose release would wait for them, as they write LDS): R planes 0..17 and
+    // strain planes 0..5 of this element into sX[plane][lane], in flight during the gradient;
+    // each lane reads back only its own slots, after its own wait
+    __builtin_amdgcn_sched_barrier(0);
+    const int w64 = p & 64;
+#pragma unroll
+    for (int pl = 0; pl < 18; pl++)
+      __builtin_amdgcn_global_load_lds((sf_global_ptr)(A.R + r_idx(e, pl, q)), (sf_lds_ptr)&sX[pl][w64], 4, 0, 0);
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+      __builtin_amdgcn_global_load_lds((sf_global_ptr)(A.eps + eps_idx(e, c, q)), (sf_lds_ptr)&sX[18 + c][w64], 4,
+                                       0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+  }
 
   const realw ONE_THIRD = (realw)1 / (realw)3;
   const Grad9 tg = tile_gradient<1>(sU[0], sU[1], sU[2], sG, i, j, k);
@@ -409,6 +455,15 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
   const realw kappal = km.x, mul = km.y;
   Sigma6 sg = iso_stress(du, kappal, mul);
 
+  if (DL) {
+    __builtin_amdgcn_s_waitcnt(0);  // the LDS-direct loads of this wave have landed
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+      for (int s = 0; s < 3; s++) Rv[cc][s] = sX[cc * 3 + s][p];
+#pragma unroll
+    for (int c = 0; c < 6; c++) Sn[c] = sX[18 + c][p];
+  }
   if (ATT) {
     const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
     const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
@@ -563,6 +618,8 @@ void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool 
   if (att) {                                       \
     if (att_variant == 1) SF_K(true, true, O, 1);  \
     else if (att_variant == 2) SF_K(true, true, O, 2); \
+    else if (att_variant == 3) SF_K(true, true, O, 3); \
+    else if (att_variant == 4) SF_K(true, true, O, 4); \
     else SF_K(true, true, O, 0);                   \
   } else if (strain) SF_K(true, false, O, 0);      \
   else SF_K(false, false, O, 0);

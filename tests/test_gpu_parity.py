@@ -63,7 +63,7 @@ def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     assert rel_l2(fast, ref) < 1e-5
 
 
-@pytest.mark.parametrize("att_variant", [1, 2])
+@pytest.mark.parametrize("att_variant", [1, 2, 3, 4])
 def test_attenuation_schedules_bitwise(oracle_mod, att_variant):
     """The point kernel's attenuation-state schedules give the oracle's results bit for bit."""
     from specfem3d_amd import workloads as W
