@@ -246,8 +246,9 @@ void unregister_host_array_(realw* h_array);
    colouring with a plain read-modify-write scatter (deterministic, no atomics). */
 void sf_set_options(long* Mesh_pointer, int exact, int assembly);
 /* tuning switch by name: "elastic_shape" 0 = column-per-lane stiffness kernel (five elements per
-   workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0/1/2 =
-   scheduling of the attenuation state in the point kernel (same results); "acoustic_kernel_field"
+   workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0..4 =
+   scheduling of the attenuation state in the point kernel (same results; 4, the default, loads it
+   straight into LDS); "acoustic_kernel_field"
    0 = compute_kernels_acoustic_cuda pairs the adjoint potential with the reconstructed field as the
    reference does (compute_kernels.f90:269,325), 1 = it uses the adjoint potential_dot_dot instead
    (the misfit gradient when the adjoint field enters the fluid through the coupling);

@@ -952,8 +952,9 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
 }
 
 // Tuning switches read by the step calls: "elastic_shape" (0 column-per-lane stiffness kernel,
-// 1 point-per-lane, the default), "att_variant" (0, 1, 2: attenuation-state scheduling of the
-// point kernel, same results). Unknown names are a fatal error.
+// 1 point-per-lane, the default), "att_variant" (0..4: attenuation-state scheduling of the
+// point kernel, same results; 4 is the default), "acoustic_kernel_field" (below). Unknown names
+// are a fatal error.
 extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
