@@ -268,7 +268,8 @@ void sf_checkpoints_free(long* Mesh_pointer);
 void sf_zero_kernels(long* Mesh_pointer);
 /* batched many-shot forward engine (no counterpart in the reference, which runs one shot per
    process): shots with their own sources, receivers and record lengths stepped together on one
-   prepared elastic mesh in shot-interleaved wavefields; element ids 0-based */
+   prepared elastic mesh (isotropic, optionally 3-SLS viscoelastic: each shot then carries its own
+   memory variables) in shot-interleaved wavefields; element ids 0-based */
 long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep, realw deltat, realw deltatover2,
                      realw deltatsqover2);
 void sf_batch_add_source(long batch, int shot, int ispec, const realw* sourcearray, const realw* stf);

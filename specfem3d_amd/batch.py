@@ -20,10 +20,12 @@ from .solver import GPUSolver
 
 class BatchSolver:
     def __init__(self, db, shots, exact: bool = False, device: int = 0, shots_per_group: int | None = None,
-                 assembly: int | None = None):
+                 assembly: int | None = None, attenuation=None):
         """``shots``: list of (SourceSet, ReceiverSet, nstep), one per shot. ``assembly``: 0 atomic
         scatter, 2 one launch per element colour, 3 coloured with the Newmark update fused into the
-        colour launches (same bits as 2); None keeps the library default until ``autotune``."""
+        colour launches (same bits as 2); None keeps the library default until ``autotune``.
+        ``attenuation``: 3-SLS set-up of the mesh (each shot then carries its own memory variables;
+        coloured assembly only)."""
         if not shots:
             raise ValueError("no shots")
         self.db = db
@@ -31,7 +33,7 @@ class BatchSolver:
         self.nstep = [int(n) for _, _, n in self.shots]
         # Morton launch order: elements of a colour that sit close together share L2 lines of rows
         self.base = GPUSolver(db, None, None, nstep=max(self.nstep), exact=exact, assembly=0, device=device,
-                              element_order="morton")
+                              element_order="morton", attenuation=attenuation)
         L = self.L = self.base.L
         L.sf_batch_create.restype = C.c_long
         L.sf_batch_autotune.restype = C.c_double

@@ -41,6 +41,8 @@ struct Batch {
   int* d_it = nullptr;
   int h_it = 1;                  // host mirror of the step counter
   realw *u = nullptr, *v = nullptr, *a = nullptr;
+  bool att = false;              // 3-SLS attenuation: per-shot memory variables and strain
+  realw *R = nullptr, *eps = nullptr;
   unsigned char* flags = nullptr;  // fused mode: BF_* per local point (first/last colour, cleared)
   int flags_version = -1;        // Mesh::color_version the flags were built for (-1: stale)
   // sources: (source, point) entries sorted into rounds (the k-th source of each shot)
@@ -240,6 +242,11 @@ static void zero_state(Batch* b) {
   const int64_t n = 3 * (int64_t)m->nglob * b->S;
   for (realw* p : {b->u, b->v, b->a})
     hipLaunchKernelGGL(k_batch_fill, sf_grid(n, 256), dim3(256), 0, m->stream, n, p);
+  if (b->att) {
+    const int64_t nr = 18 * (int64_t)m->nspec * 125 * b->S, ne = 6 * (int64_t)m->nspec * 125 * b->S;
+    hipLaunchKernelGGL(k_batch_fill, sf_grid(nr, 256), dim3(256), 0, m->stream, nr, b->R);
+    hipLaunchKernelGGL(k_batch_fill, sf_grid(ne, 256), dim3(256), 0, m->stream, ne, b->eps);
+  }
   if (b->seis) {
     const int64_t ns = (int64_t)b->max_nstep * b->nrec * 3;
     hipLaunchKernelGGL(k_batch_fill, sf_grid(ns, 256), dim3(256), 0, m->stream, ns, b->seis);
@@ -328,7 +335,26 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
   A.dt = b->dt;
   A.dt2 = b->dt2;
   A.dtsq2 = b->dtsq2;
-  if (b->mode == 2 || b->mode == 3) {
+  if (b->att) {
+    A.R = b->R;
+    A.eps = b->eps;
+    A.fc_index = m->fc_index;
+    A.fc_table = m->fc_table;
+    A.fc_rows = m->fc_rows;
+    A.fc = m->factor_common;
+    A.fck = m->factor_common_kappa;
+    A.nloc = (int64_t)m->nspec * 125;
+    for (int s = 0; s < 3; s++) {
+      A.alpha[s] = m->alphaval[s];
+      A.beta[s] = m->betaval[s];
+      A.gamma[s] = m->gammaval[s];
+    }
+    for (int c = 0; c < m->el_ncolor; c++) {
+      A.elems = m->el_color_elems + m->el_color_off[c];
+      A.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
+      launch_visco_batch(st, A, b->sc, b->mode, m->exact != 0);
+    }
+  } else if (b->mode == 2 || b->mode == 3) {
     for (int c = 0; c < m->el_ncolor; c++) {
       A.elems = m->el_color_elems + m->el_color_off[c];
       A.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
@@ -440,7 +466,8 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   if (nshots <= 0) fatal("sf_batch_create", "need at least one shot");
   if (!m->rmass || !m->kappamu || m->nspec_elastic <= 0) fatal("sf_batch_create", "mesh has no prepared elastic domain");
   if (m->nspec_acoustic > 0) fatal("sf_batch_create", "batched shots cover elastic meshes only");
-  if (m->attenuation) fatal("sf_batch_create", "batched shots do not support attenuation");
+  if (m->attenuation && !m->fc_index && !m->factor_common)
+    fatal("sf_batch_create", "attenuation factors not prepared");
   Batch* b = new Batch();
   b->m = m;
   b->dt = deltat;
@@ -469,6 +496,12 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   b->u = balloc<realw>(n);
   b->v = balloc<realw>(n);
   b->a = balloc<realw>(n);
+  if (m->attenuation) {
+    b->att = true;
+    b->mode = 3;
+    b->R = balloc<realw>(18 * (int64_t)m->nspec * 125 * b->S);
+    b->eps = balloc<realw>(6 * (int64_t)m->nspec * 125 * b->S);
+  }
   return reinterpret_cast<long>(b);
 }
 
@@ -587,6 +620,7 @@ extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) 
   if (assembly != 0 && assembly != 2 && assembly != 3)
     fatal("sf_batch_set_options", "assembly must be 0 (atomic), 2 (coloured) or 3 (coloured, fused update)");
   if (b->h_it != 1) fatal("sf_batch_set_options", "options change only at step 1 (call sf_batch_reset first)");
+  if (b->att && assembly == 0) fatal("sf_batch_set_options", "viscoelastic batches use coloured assembly (2 or 3)");
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   drop_graph(b);
   b->sc = shots_per_group;
@@ -715,6 +749,7 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
     for (int sc : {4, 8})
     for (int ov : {1, 0}) {
       if (b->S % sc) continue;
+      if (b->att && (mode == 0 || ov == 0)) continue;  // coloured only; no overlay switch
       b->sc = sc;
       b->mode = mode;
       b->overlay = ov;
@@ -744,7 +779,7 @@ extern "C" void sf_batch_destroy(long h) {
   Batch* b = batch_of(h);
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   drop_graph(b);
-  bfree(b->d_nstep); bfree(b->d_it); bfree(b->u); bfree(b->v); bfree(b->a);
+  bfree(b->d_nstep); bfree(b->d_it); bfree(b->u); bfree(b->v); bfree(b->a); bfree(b->R); bfree(b->eps);
   bfree(b->flags);
   bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
   bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);

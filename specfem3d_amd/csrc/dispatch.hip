@@ -40,6 +40,15 @@ void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, real
     launch_element_strain_fast(s, a, eps, eps_tr3, with_trace);
 }
 
+void launch_visco_batch_exact(hipStream_t, const BatchArgs&, int, int);
+void launch_visco_batch_fast(hipStream_t, const BatchArgs&, int, int);
+void launch_visco_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact) {
+  if (exact)
+    launch_visco_batch_exact(s, a, sc, mode);
+  else
+    launch_visco_batch_fast(s, a, sc, mode);
+}
+
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact) {
   if (exact)
     launch_elastic_batch_exact(s, a, sc, mode, overlay);

@@ -219,6 +219,203 @@ k_elastic_batch(BatchArgs B, GLLk G) {
   }
 }
 
+// Viscoelastic (3-SLS) variant: each shot also carries its memory variables and stored strain
+// (per-element blocks, shots fastest). The tile holds u and dt*v (6 rows per shot; they share the
+// LDS of the stress products t2, t3, written after a barrier). Per shot, the arithmetic and its
+// order are the single-shot point kernel's (k_elastic_pt): strain at u + dt v, isotropic stress
+// minus the memory-variable sums, memory update, stress products. Assembly as k_elastic_batch
+// (MODE 2 coloured read-modify-write, 3 coloured with the Newmark update fused).
+__device__ __forceinline__ int64_t rb_idx(int64_t e, int plane, int p, int64_t S, int s) {
+  return ((e * 18 + plane) * 125 + p) * S + s;
+}
+__device__ __forceinline__ int64_t eb_idx(int64_t e, int c, int p, int64_t S, int s) {
+  return ((e * 6 + c) * 125 + p) * S + s;
+}
+
+template <int NQ, int MODE>
+__global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
+  constexpr int SC = 4 * NQ;
+  constexpr int PST = 132;                        // see k_elastic_batch
+  constexpr int CPL = SC * PST;
+  __shared__ realw sBuf[9 * CPL];
+  realw* const sT = sBuf;                         // stress products [m][s][p]
+  realw* const sU = sBuf + 3 * CPL;               // u (rows 0..2) and dt*v (rows 3..5), [c][s][p]
+  __shared__ realw sG[75];
+  stage_gll(sG, G, threadIdx.x);
+  const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int b = xcd_remap_b(blockIdx.x, gridDim.x);
+  const int ie = b / B.ngroups;
+  const int s0 = (b - ie * B.ngroups) * SC;
+  if (B.nstep[s0] < it) return;
+  const int q = threadIdx.x % NQ, p = threadIdx.x / NQ;
+  const int sq = s0 + 4 * q;
+  const bool live0 = B.nstep[sq] >= it;
+  bool live[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) live[r] = B.nstep[sq + r] >= it;
+  const bool active = p < 125;
+  const int pp = active ? p : 124;
+  const int e = B.elems[ie];
+  const int irr = B.irregular[e];
+  const int64_t S = B.S;
+  const int64_t ep = (int64_t)e * 125 + pp;
+  const int g = B.ibool[ep];
+  const int64_t row = (int64_t)g * 3 * S + sq;
+  const int ts = 4 * q;
+  float4 u[3], V[3];                              // staged into the tile below
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    u[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    V[c] = u[c];
+    if (live0) {
+      u[c] = *reinterpret_cast<const float4*>(B.u + row + c * S);
+      V[c] = *reinterpret_cast<const float4*>(B.v + row + c * S);
+    }
+  }
+  const unsigned fl = MODE == 3 ? (unsigned)B.flags[ep] : 0u;
+  const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
+  const bool last = MODE == 3 && (fl & BF_LAST) != 0;
+  const bool mine = active && live0;
+  JacPt J = {};
+  realw jacl = B.jacobian_regular;
+  if (irr) {
+    J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
+    jacl = J.jac;
+  }
+  const float2 km = B.kappamu[ep];
+  realw fcv[3], fckv[3];
+  if (B.fc_index) {
+    const int rrow = B.fc_index[ep];
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = B.fc_table[s * B.fc_rows + rrow];
+      fckv[s] = B.fc_table[(3 + s) * B.fc_rows + rrow];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      fcv[s] = B.fc[s * B.nloc + ep];
+      fckv[s] = B.fck[s * B.nloc + ep];
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        sU[(c * SC + ts + r) * PST + p] = c4(u[c], r);
+        sU[((3 + c) * SC + ts + r) * PST + p] = B.dt * c4(V[c], r);
+      }
+  }
+  __syncthreads();
+
+  const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
+  const realw ONE_THIRD = (realw)1 / (realw)3;
+  const realw kappal = km.x, mul = km.y;
+  realw t23[4][6];                                // products t2, t3 of each shot, written after a barrier
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int s = sq + r;
+    const realw* su = sU + (ts + r) * PST;
+    const Grad9 tg = tile_gradient<1>(su, su + CPL, su + 2 * CPL, sG, i, j, k);
+    const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
+    const Du9 st = map_gradient(tile_gradient_continue<1>(tg, su + 3 * CPL, su + 4 * CPL, su + 5 * CPL, sG, i, j, k),
+                                irr != 0, J, B.xix_regular);
+    const realw e_tr = (st.xx + st.yy + st.zz);
+    const realw templ = ONE_THIRD * (st.xx + st.yy + st.zz);
+    const realw Snp1[6] = {st.xx - templ, st.yy - templ, (realw)0.5 * (st.xy + st.yx), (realw)0.5 * (st.zx + st.xz),
+                           (realw)0.5 * (st.zy + st.yz), e_tr};
+    Sigma6 sg = iso_stress(du, kappal, mul);
+    realw Rv[6][3], Sn[6];
+#pragma unroll
+    for (int cc = 0; cc < 6; cc++) {
+#pragma unroll
+      for (int sl = 0; sl < 3; sl++) Rv[cc][sl] = B.R[rb_idx(e, cc * 3 + sl, pp, S, s)];
+      Sn[cc] = B.eps[eb_idx(e, cc, pp, S, s)];
+    }
+    const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
+    const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
+    const realw rtr = (Rv[5][0] + Rv[5][1]) + Rv[5][2];
+    sg.xx = sg.xx - rxx - rtr;
+    sg.yy = sg.yy - ryy - rtr;
+    sg.zz = sg.zz + rxx + ryy - rtr;
+    sg.xy = sg.xy - ((Rv[2][0] + Rv[2][1]) + Rv[2][2]);
+    sg.xz = sg.xz - ((Rv[3][0] + Rv[3][1]) + Rv[3][2]);
+    sg.yz = sg.yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
+    if (mine && live[r]) {
+#pragma unroll
+      for (int sl = 0; sl < 3; sl++) {
+        const realw flk = kappal * fckv[sl];
+        B.R[rb_idx(e, 5 * 3 + sl, pp, S, s)] =
+            B.alpha[sl] * Rv[5][sl] + flk * (B.beta[sl] * Sn[5] + B.gamma[sl] * Snp1[5]);
+        const realw flm = mul * fcv[sl];
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++)
+          B.R[rb_idx(e, cc * 3 + sl, pp, S, s)] =
+              B.alpha[sl] * Rv[cc][sl] + flm * (B.beta[sl] * Sn[cc] + B.gamma[sl] * Snp1[cc]);
+      }
+#pragma unroll
+      for (int cc = 0; cc < 6; cc++) B.eps[eb_idx(e, cc, pp, S, s)] = Snp1[cc];
+    }
+    realw t[9];
+    stress_products(sg, irr != 0, J, jacl, B.xix_regular, t);
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < 3; m++) sT[(m * SC + ts + r) * PST + p] = t[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; m++) t23[r][m] = t[3 + m];
+    __builtin_amdgcn_sched_barrier(0);            // one shot's state in registers at a time
+  }
+  __syncthreads();                                // every lane is done with the u / dt*v tile
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+      for (int m = 3; m < 9; m++) sT[(m * SC + ts + r) * PST + p] = t23[r][m - 3];
+  }
+  __syncthreads();
+  if (!mine) return;
+
+  realw* a = B.a + row;
+  const realw* hpw = sG + 25;
+  const realw* wxy = sG + 50;
+  bool cont[4], ends = false;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    cont[r] = live[r] && B.nstep[sq + r] > it;
+    ends = ends || (live[r] && !cont[r]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    realw f[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) f[r] = tile_divergence_c<1>(sT + (ts + r) * PST, CPL, hpw, wxy, i, j, k, c);
+    float4 Ac = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!first || !live[3]) Ac = *reinterpret_cast<const float4*>(a + c * S);
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (live[r]) q4(Ac, r) = (first ? 0.f : q4(Ac, r)) + (-f[r]);
+    if (MODE == 2 || !last) {
+      *reinterpret_cast<float4*>(a + c * S) = Ac;
+      continue;
+    }
+    const realw M = B.rmass[(int64_t)g * 3 + c];
+    // this step's u and v rows again (read at the start for the tile; kept out of registers)
+    float4 Vc = *reinterpret_cast<const float4*>(B.v + row + c * S);
+    float4 U = *reinterpret_cast<const float4*>(B.u + row + c * S);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (!live[r]) continue;
+      newmark_finish(q4(U, r), q4(Vc, r), q4(Ac, r), M, B.dt, B.dt2, B.dtsq2, cont[r]);
+      if (cont[r]) q4(Ac, r) = 0.f;
+    }
+    if (cont[0]) *reinterpret_cast<float4*>(B.u + row + c * S) = U;
+    *reinterpret_cast<float4*>(B.v + row + c * S) = Vc;
+    if ((fl & BF_ZERO) || ends) *reinterpret_cast<float4*>(a + c * S) = Ac;
+  }
+}
+
 }  // namespace SF_VNS
 using namespace SF_VNS;
 
@@ -246,6 +443,30 @@ void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int m
 #undef SF_KBM
 #undef SF_KB
   SF_LAUNCH_CHECK("k_elastic_batch");
+}
+
+void SFX(launch_visco_batch)(hipStream_t st, const BatchArgs& a, int sc, int mode) {
+  if (a.nelem <= 0 || a.ngroups <= 0) return;
+  const int64_t nb = (int64_t)a.nelem * a.ngroups;
+  if (nb > 0x7fffffff) fatal("launch_visco_batch", "grid too large");
+  if (!a.R || !a.eps || !a.v) fatal("launch_visco_batch", "memory variables, strain and velocity are needed");
+  if (mode == 3 && (!a.rmass || !a.flags)) fatal("launch_visco_batch", "fused update needs rmass and point flags");
+  dim3 grid((unsigned)nb);
+  const GLLk G = g_gll_host;
+#define SF_KV(NQ, M) hipLaunchKernelGGL((k_visco_batch<NQ, M>), grid, dim3(128 * NQ), 0, st, a, G)
+  if (mode == 3) {
+    if (sc == 4) SF_KV(1, 3);
+    else if (sc == 8) SF_KV(2, 3);
+    else fatal("launch_visco_batch", "shots per workgroup must be 4 or 8");
+  } else if (mode == 2) {
+    if (sc == 4) SF_KV(1, 2);
+    else if (sc == 8) SF_KV(2, 2);
+    else fatal("launch_visco_batch", "shots per workgroup must be 4 or 8");
+  } else {
+    fatal("launch_visco_batch", "viscoelastic batches use coloured assembly (2 or 3)");
+  }
+#undef SF_KV
+  SF_LAUNCH_CHECK("k_visco_batch");
 }
 
 }  // namespace sf

@@ -87,12 +87,25 @@ struct BatchArgs {
   const realw* rmass;      // inverse mass, 3 per global point
   const unsigned char* flags;  // per local point e*125+p: BF_FIRST | BF_LAST | BF_ZERO
   realw dt, dt2, dtsq2;
+  // 3-SLS attenuation (k_visco_batch): per-shot memory variables and strain in per-element blocks,
+  // value (e, plane, p, shot) at ((e * 18 + plane) * 125 + p) * S + shot (strain: 6 planes)
+  realw* R;
+  realw* eps;
+  const uint16_t* fc_index;    // SLS factor row per local point (or null: fc / fck planes)
+  const realw* fc_table;       // 6 planes of fc_rows
+  int fc_rows;
+  const realw* fc;
+  const realw* fck;
+  int64_t nloc;
+  realw alpha[3], beta[3], gamma[3];
 };
 enum : unsigned char { BF_FIRST = 1, BF_LAST = 2, BF_ZERO = 4 };
 // sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour),
 // 3 one colour with the Newmark update fused (see k_batch.hip); overlay 1 shares the LDS of the
 // displacement tile and the stress products
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact);
+// viscoelastic (3 SLS) batched stiffness; mode 2 or 3
+void launch_visco_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
 
 struct AcousticArgs {
   int nelem;

@@ -7,7 +7,8 @@
   step of every shot, with eager steps and host synchronisation between the replays;
 * fast kernels: each shot within 1e-5 rel-L2 of the FP32 oracle, for every autotuner candidate;
 * uneven record lengths: a shot stops at its own length (its final displacement equals the
-  single-shot run's).
+  single-shot run's);
+* 3-SLS viscoelastic batches: every shot bit-identical to its single-shot viscoelastic run.
 """
 import numpy as np
 import pytest
@@ -89,4 +90,36 @@ def test_autotuned_batch_matches_oracle(small_batch, oracle_mod):
         e = rel_l2(b.seismograms(i), ref)
         print(f"[gpu] shot {i} ({n} steps): rel-L2 vs FP32 oracle {e:.2e}", flush=True)
         assert e < 1e-5
+    b.close()
+
+
+@pytest.fixture(scope="module")
+def visco_batch():
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.attenuation import setup_attenuation
+    w = W.config3(scale=0.1, nshots=5, nrec=5, nstep_range=(60, 160), seed=13)
+    att = setup_attenuation(w.db, qmu=60.0, qkappa=9000.0)
+    refs = []
+    from specfem3d_amd.solver import GPUSolver
+    for src, rec, n in w.shots:
+        g = GPUSolver(w.db, src, rec, nstep=n, attenuation=att, exact=True, assembly=2, element_order="morton")
+        s = g.run().copy()
+        u, _, _ = g.fields()
+        g.close()
+        refs.append((s, u))
+    return w, att, refs
+
+
+@pytest.mark.parametrize("assembly,sc", [(2, 4), (3, 8)])
+def test_visco_batch_bitwise_equal_single_shot_runs(visco_batch, assembly, sc):
+    from specfem3d_amd.batch import BatchSolver
+    w, att, refs = visco_batch
+    print(f"[gpu] viscoelastic batch of {len(w.shots)} shots, assembly {assembly}, shots per group {sc}", flush=True)
+    b = BatchSolver(w.db, w.shots, exact=True, shots_per_group=sc, assembly=assembly, attenuation=att)
+    b.run(graph_steps=10)
+    for i, (ref, uref) in enumerate(refs):
+        out = b.seismograms(i)
+        assert np.abs(ref).max() > 0
+        assert np.array_equal(out, ref), (i, rel_l2(out, ref))
+        assert np.array_equal(b.displ(i), uref), i
     b.close()
