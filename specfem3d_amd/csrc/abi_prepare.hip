@@ -8,6 +8,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <unordered_map>
 #include <vector>
 
@@ -19,6 +22,19 @@ void free_checkpoints(Mesh* m);
 Mesh* from_handle(long* p) {
   if (!p || !*p) fatal("Mesh_pointer", "null handle (prepare_constants_device not called?)");
   return reinterpret_cast<Mesh*>(*p);
+}
+
+// Set-up progress on stderr when SF_VERBOSE_PREPARE=1 (large meshes take minutes to prepare).
+static void prep_log(const char* what) {
+  static const bool on = [] {
+    const char* v = std::getenv("SF_VERBOSE_PREPARE");
+    return v && v[0] == '1';
+  }();
+  if (!on) return;
+  static const auto t0 = std::chrono::steady_clock::now();
+  const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  std::fprintf(stderr, "[prepare] %s at %.1fs\n", what, s);
+  std::fflush(stderr);
 }
 
 template <typename T>
@@ -301,6 +317,7 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
   H->ibool.resize(nloc);
   for (int64_t q = 0; q < nloc; q++) H->ibool[q] = h_ibool[q] - 1;
   m->ibool = dupload(H->ibool.data(), nloc);
+  prep_log("ibool uploaded");
   m->irregular = dupload(h_irregular_element_number, m->nspec);
 
   // irregular Jacobians: 9 planes from the host + the determinant recomputed as the inverse of
@@ -482,17 +499,25 @@ namespace sf {
 // R_*(N_SLS, NGLL3, NSPEC), SLS fastest.
 static realw* upload_strain_blocks(const realw* const src[6], int64_t nloc) {
   std::vector<realw> E(6 * (size_t)nloc, 0.f);
+  const int64_t ne = nloc / 125;
   for (int c = 0; c < 6; c++) {
     if (!src[c]) continue;
-    for (int64_t q = 0; q < nloc; q++) E[(size_t)eps_idx(q / 125, c, (int)(q % 125))] = src[c][q];
+    for (int64_t e = 0; e < ne; e++)
+      std::memcpy(&E[(size_t)eps_idx(e, c, 0)], src[c] + e * 125, sizeof(realw) * 125);
   }
   return dupload(E.data(), 6 * nloc);
 }
 static realw* upload_memory_blocks(const realw* const rs[6], int64_t nloc) {
   std::vector<realw> R(18 * (size_t)nloc);
-  for (int c = 0; c < 6; c++)
-    for (int64_t q = 0; q < nloc; q++)
-      for (int s = 0; s < 3; s++) R[(size_t)r_idx(q / 125, c * 3 + s, (int)(q % 125))] = rs[c][3 * q + s];
+  const int64_t ne = nloc / 125;
+  for (int64_t e = 0; e < ne; e++)
+    for (int c = 0; c < 6; c++) {
+      const realw* src = rs[c] + 3 * e * 125;
+      for (int s = 0; s < 3; s++) {
+        realw* dst = &R[(size_t)r_idx(e, c * 3 + s, 0)];
+        for (int p = 0; p < 125; p++) dst[p] = src[3 * p + s];
+      }
+    }
   return dupload(R.data(), 18 * nloc);
 }
 }  // namespace sf
@@ -535,11 +560,13 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
     }
     m->rmass = dupload(rm.data(), 3 * n);
   }
+  prep_log("elastic mass uploaded");
   {
     std::vector<float2> km(nloc);
     for (int64_t q = 0; q < nloc; q++) km[q] = make_float2(h_kappav[q], h_muv[q]);
     m->kappamu = dupload(km.data(), nloc);
   }
+  prep_log("elastic moduli uploaded");
   // element list: phase_ispec_inner_elastic(num_phase, 2); at NPROC=1 column 2 holds all elements
   {
     const int np = *num_phase_ispec_elastic;
@@ -572,6 +599,7 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
   if (m->stacey && m->num_abs_boundary_faces > 0) {
     std::vector<int> es, pl;
     build_face_points(m, H->abs, H->is_el.data(), H->ibool.data(), m->abs_el, &es, &pl);
+    prep_log("elastic Stacey points built");
     std::vector<realw> a(m->abs_el.n), b(m->abs_el.n);
     for (int i = 0; i < m->abs_el.n; i++) {
       a[i] = rho_vp[(int64_t)es[i] * 125 + pl[i]];
@@ -590,11 +618,13 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
     const realw* src[6] = {epsilondev_xx, epsilondev_yy, epsilondev_xy, epsilondev_xz, epsilondev_yz,
                            m->attenuation ? epsilondev_trace : nullptr};
     m->att[0].eps = upload_strain_blocks(src, nloc);
+    prep_log("strain uploaded");
   }
   if (m->attenuation) {
     if (*R_size != 3 * nloc) fatal("prepare_fields_elastic_device", "R_size must be N_SLS*125*NSPEC");
     const realw* rs[6] = {R_xx, R_yy, R_xy, R_xz, R_yz, R_trace};
     m->att[0].R = upload_memory_blocks(rs, nloc);
+    prep_log("memory variables uploaded");
     std::vector<realw> F(3 * (size_t)nloc), Fk(3 * (size_t)nloc);
     for (int64_t q = 0; q < nloc; q++)
       for (int s = 0; s < 3; s++) {
@@ -603,7 +633,9 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
       }
     m->factor_common = dupload(F.data(), 3 * nloc);
     m->factor_common_kappa = dupload(Fk.data(), 3 * nloc);
+    prep_log("attenuation factors uploaded");
     build_factor_table(m, factor_common, factor_common_kappa, nloc);
+    prep_log("attenuation factor table built");
     for (int s = 0; s < 3; s++) {
       m->alphaval[s] = alphaval[s];
       m->betaval[s] = betaval[s];
@@ -771,6 +803,7 @@ extern "C" void FC_FUNC_(prepare_fields_acoustic_device, PREPARE_FIELDS_ACOUSTIC
   if (m->stacey && m->num_abs_boundary_faces > 0) {
     std::vector<int> es, pl;
     build_face_points(m, H->abs, H->is_ac.data(), H->ibool.data(), m->abs_ac, &es, &pl);
+    prep_log("acoustic Stacey points built");
     std::vector<realw> r(m->abs_ac.n), c(m->abs_ac.n);
     for (int i = 0; i < m->abs_ac.n; i++) {
       const int64_t q = (int64_t)es[i] * 125 + pl[i];

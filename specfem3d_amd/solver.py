@@ -101,6 +101,15 @@ class GPUSolver:
 
     # ------------------------------------------------------------------------------ set-up
     def _prepare(self, exact, assembly, save_forward, device):
+        import os
+        import sys
+        import time
+        t0 = time.perf_counter()
+        verbose = os.environ.get("SF_VERBOSE_PREPARE") == "1"
+
+        def stage(name):
+            if verbose:
+                print(f"[prepare] {name} done at {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
         L, db, A = self.L, self.db, abi.Args()
         self._keep = A
         ndev = C.c_int(0)
@@ -147,6 +156,7 @@ class GPUSolver:
             A.da(nu.ravel()), A.da(np.zeros(9 * max(nsrc, 1))), A.ia(np.zeros(max(nrec, 1), np.int32)),
             A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(0), A.i(1), A.i(0),
             A.i(int(self.undo_att)), A.i(0))
+        stage("prepare_constants_device")
         L.sf_set_options(self.MP, C.c_int(int(exact)), C.c_int(int(assembly)))
         for name, value in self.tuning.items():
             L.sf_set_tuning(self.MP, name.encode(), C.c_int(int(value)))
@@ -166,6 +176,7 @@ class GPUSolver:
             if self.sim == 3:
                 # reconstructed potentials and the acoustic kernel arrays
                 L.prepare_fields_acoustic_adj_dev_(self.MP, A.i(0))
+        stage("acoustic fields")
         if self.elastic:
             n, ph = _fortran_phase_list(db.elastic_elements())
             att = self.att
@@ -216,6 +227,7 @@ class GPUSolver:
             else:
                 raise ValueError(f"unknown element order {self.element_order!r}")
             L.sf_set_element_order(self.MP, C.c_int(len(order)), abi.ptr(order))
+        stage("elastic fields")
         n3 = 3 * db.nglob
         self.displ = np.zeros(n3, np.float32)
         self.veloc = np.zeros(n3, np.float32)
