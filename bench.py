@@ -73,19 +73,31 @@ def stiffness_bytes_per_launch(db, attenuation=None) -> float:
     return float(per_glob * nglob_el + per_loc * nloc_el + 40 * nirr_el)
 
 
-def batch_stiffness_bytes(db, nshots: int, fused: bool = False) -> float:
+def batch_stiffness_bytes(db, nshots: int, fused: bool = False, attenuation=None) -> float:
     """Algorithmic HBM bytes of one batched stiffness pass with every shot live: per shot and global
     point u read (12 B) and accel read + write (24 B); per local point, shared by the shots, ibool +
     kappa + mu (12 B) and 40 B of Jacobian terms in irregular elements. With the Newmark update fused
     into the pass (assembly 3) the pass is the whole step: u, v, accel each read and written
-    (72 B per shot and point) plus the inverse mass (12 B per point)."""
+    (72 B per shot and point) plus the inverse mass (12 B per point). With attenuation each shot also
+    reads and writes its 18 memory variables and 6 stored strains per local point (192 B), and the
+    SLS factors are shared (a 2-byte row index, or 24 B of planes beyond 65535 distinct rows); a
+    separate (unfused) pass also reads v (12 B per shot and point)."""
     import numpy as np
     el = db.ispec_is_elastic
     nloc = int(el.sum()) * 125
     nglob_el = int(np.unique(db.ibool[el]).size)
     nirr = int((db.irregular_element_number[el] > 0).sum()) * 125
     per_shot = 72 if fused else 36
-    return float(per_shot * nglob_el * nshots + 12 * nloc + 40 * nirr + (12 * nglob_el if fused else 0))
+    per_shot_loc = 0
+    shared_loc = 12
+    if attenuation is not None:
+        per_shot += 0 if fused else 12
+        per_shot_loc = 192
+        rows = np.unique(np.concatenate([attenuation.factor_common.reshape(-1, 3),
+                                         attenuation.factor_common_kappa.reshape(-1, 3)], 1), axis=0).shape[0]
+        shared_loc += 2 if rows <= 65535 else 24
+    return float(per_shot * nglob_el * nshots + per_shot_loc * nloc * nshots + shared_loc * nloc + 40 * nirr
+                 + (12 * nglob_el if fused else 0))
 
 
 def batch_step_bytes(db, shot_steps: int, nsteps: int) -> float:
@@ -156,6 +168,60 @@ def batched_measurement(args) -> dict:
                              "bytes_per_update": round(batch_step_bytes(db, sum(nsteps), max(nsteps))
                                                        / (db.nglob * float(sum(nsteps))), 2)},
             "roofline": {"bound": "hbm", "kernel": "k_elastic_batch (all shots live"
+                         + (", Newmark update fused)" if choice["assembly"] == 3 else ")"), "achieved": round(ach, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "bytes_per_launch": kbytes, "avg_pass_ms": round(kms, 4), "step_ms": round(sms, 4)}}
+
+
+def batched_visco_measurement(args) -> dict:
+    """Many viscoelastic shots on one mesh: the 64x64x32 gradient + GRF box with 3-SLS attenuation
+    (Q_mu = 100), ``--visco-shots`` seeded random CMT shots of ``--visco-steps`` steps each (every shot
+    live for the whole run), against the single-shot viscoelastic path on the same mesh."""
+    import numpy as np
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.attenuation import setup_attenuation
+    from specfem3d_amd.batch import BatchSolver
+    from specfem3d_amd.solver import GPUSolver
+    w = W.config3(scale=args.batch_scale, nshots=args.visco_shots)
+    att = setup_attenuation(w.db, qmu=100.0, qkappa=9000.0)
+    n = args.visco_steps
+    shots = [(s, r, n) for s, r, _ in w.shots]
+    db = w.db
+    log(f"[bench] batched viscoelastic workload: NGLOB {db.nglob}, {len(shots)} shots x {n} steps")
+    b = BatchSolver(db, shots, exact=False, attenuation=att)
+    choice = b.autotune(trial_steps=3)
+    log(f"[bench] viscoelastic autotuner: {choice}")
+    kms, sms = b.time(10)
+    b.reset()
+    b.synchronize()
+    t = time.perf_counter()
+    b.run(graph_steps=10)
+    b.synchronize()
+    wall = time.perf_counter() - t
+    finite = all(bool(np.isfinite(b.seismograms(i)).all()) for i in range(len(shots)))
+    b.close()
+    value = db.nglob * float(n * len(shots)) / wall
+    kbytes = batch_stiffness_bytes(db, len(shots), fused=choice["assembly"] == 3, attenuation=att)
+    ach = kbytes / (kms / 1e3) / 1e9
+    src, rec, _ = shots[0]
+    ps = min(args.plain_steps, n)
+    pw = min(10, ps // 2)
+    g = GPUSolver(db, src, rec, nstep=ps, attenuation=att, exact=False, assembly=0)
+    for _ in range(pw):
+        g.step()
+    g.synchronize()
+    tp = time.perf_counter()
+    for _ in range(ps - pw):
+        g.step()
+    g.synchronize()
+    plain = db.nglob * (ps - pw) / (time.perf_counter() - tp)
+    g.close()
+    log(f"[bench] batched viscoelastic {value:.3e} GLL-pt updates/s in {wall:.2f}s (pass {kms:.3f} ms, step "
+        f"{sms:.3f} ms), plain viscoelastic single shot {plain:.3e}, ratio {value / plain:.2f}, finite={finite}")
+    return {"workload": f"{w.name.split(',')[0]} viscoelastic Q_mu=100, {len(shots)} shots x {n} steps",
+            "value": value, "unit": "GLL-point updates/s", "wall_s": round(wall, 3), "plain_single_shot": plain,
+            "ratio_vs_plain": value / plain, "autotuned": choice, "seismograms_finite": finite,
+            "roofline": {"bound": "hbm", "kernel": "k_visco_batch (all shots live"
                          + (", Newmark update fused)" if choice["assembly"] == 3 else ")"), "achieved": round(ach, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                          "bytes_per_launch": kbytes, "avg_pass_ms": round(kms, 4), "step_ms": round(sms, 4)}}
@@ -265,6 +331,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-shots", type=int, default=32, help="shots of the batched 64x64x32 workload (0: skip)")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="shrink the batched mesh (tests only)")
+    ap.add_argument("--visco-shots", type=int, default=16,
+                    help="viscoelastic shots batched on the 64x64x32 mesh with attenuation (0: skip)")
+    ap.add_argument("--visco-steps", type=int, default=300, help="steps of each batched viscoelastic shot")
     ap.add_argument("--plain-steps", type=int, default=300, help="steps of the single-shot run on the batched mesh")
     ap.add_argument("--batch-max-steps", type=int, default=0,
                     help="clip every shot's record length to this many steps (profiling runs; 0: full records)")
@@ -302,6 +371,7 @@ def main():
     if args.skip_single:
         out = {"metric": "GLL-point updates/s/GPU (forward, single shot)", "value": None,
                "batched": batched_measurement(args) if args.batch_shots > 0 else None,
+               "batched_viscoelastic": batched_visco_measurement(args) if args.visco_shots > 0 else None,
                "adjoint": adjoint_measurement(args) if args.adjoint_steps > 0 else None,
                "large": large_measurement(args) if args.large_steps > 0 else None}
         print(json.dumps(out), flush=True)
@@ -364,6 +434,7 @@ def main():
 
     adj = adjoint_measurement(args) if args.adjoint_steps > 0 else None
     batched = batched_measurement(args) if args.batch_shots > 0 else None
+    batched_visco = batched_visco_measurement(args) if args.visco_shots > 0 else None
     large = large_measurement(args) if args.large_steps > 0 else None
 
     out = {
@@ -386,6 +457,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "batched_viscoelastic": batched_visco,
         "adjoint": adj,
     }
     if large is not None:

@@ -220,16 +220,29 @@ k_elastic_batch(BatchArgs B, GLLk G) {
 }
 
 // Viscoelastic (3-SLS) variant: each shot also carries its memory variables and stored strain
-// (per-element blocks, shots fastest). The tile holds u and dt*v (6 rows per shot; they share the
+// (one block per element and shot: plane, point). The tile holds u and dt*v (6 rows per shot; they share the
 // LDS of the stress products t2, t3, written after a barrier). Per shot, the arithmetic and its
 // order are the single-shot point kernel's (k_elastic_pt): strain at u + dt v, isotropic stress
 // minus the memory-variable sums, memory update, stress products. Assembly as k_elastic_batch
 // (MODE 2 coloured read-modify-write, 3 coloured with the Newmark update fused).
+// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local address
+// space, so the wait before s_barrier is lgkmcnt(0) alone and global loads and stores issued
+// earlier stay in flight across it (a plain __syncthreads waits for them with vmcnt(0)).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Per-shot state is one block per (element, shot), point-fastest within each plane: the lanes of a
+// wave that read one shot read consecutive words (a shot-fastest row would spread a wave's reads
+// of one shot over S times as many cache lines), and the planes of a block sit at fixed offsets
+// from one base address.
 __device__ __forceinline__ int64_t rb_idx(int64_t e, int plane, int p, int64_t S, int s) {
-  return ((e * 18 + plane) * 125 + p) * S + s;
+  return ((e * S + s) * 18 + plane) * 125 + p;
 }
 __device__ __forceinline__ int64_t eb_idx(int64_t e, int c, int p, int64_t S, int s) {
-  return ((e * 6 + c) * 125 + p) * S + s;
+  return ((e * S + s) * 6 + c) * 125 + p;
 }
 
 template <int NQ, int MODE>
@@ -307,15 +320,29 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
         sU[((3 + c) * SC + ts + r) * PST + p] = B.dt * c4(V[c], r);
       }
   }
-  __syncthreads();
+  // memory variables and stored strain of the first shot, in flight with the tile gather; each
+  // later shot's are loaded while the previous shot is computed (two register sets)
+  realw Rc[18], Ec[6];
+#pragma unroll
+  for (int pl = 0; pl < 18; pl++) Rc[pl] = B.R[rb_idx(e, pl, pp, S, sq)];
+#pragma unroll
+  for (int cc = 0; cc < 6; cc++) Ec[cc] = B.eps[eb_idx(e, cc, pp, S, sq)];
+  lds_barrier();
 
   const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
   const realw ONE_THIRD = (realw)1 / (realw)3;
   const realw kappal = km.x, mul = km.y;
-  realw t23[4][6];                                // products t2, t3 of each shot, written after a barrier
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const int s = sq + r;
+    realw Rn[18], En[6];
+    if (r < 3) {
+#pragma unroll
+      for (int pl = 0; pl < 18; pl++) Rn[pl] = B.R[rb_idx(e, pl, pp, S, s + 1)];
+#pragma unroll
+      for (int cc = 0; cc < 6; cc++) En[cc] = B.eps[eb_idx(e, cc, pp, S, s + 1)];
+    }
+    __builtin_amdgcn_sched_barrier(0);            // the next shot's loads are issued first
     const realw* su = sU + (ts + r) * PST;
     const Grad9 tg = tile_gradient<1>(su, su + CPL, su + 2 * CPL, sG, i, j, k);
     const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
@@ -330,8 +357,8 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
 #pragma unroll
     for (int cc = 0; cc < 6; cc++) {
 #pragma unroll
-      for (int sl = 0; sl < 3; sl++) Rv[cc][sl] = B.R[rb_idx(e, cc * 3 + sl, pp, S, s)];
-      Sn[cc] = B.eps[eb_idx(e, cc, pp, S, s)];
+      for (int sl = 0; sl < 3; sl++) Rv[cc][sl] = Rc[cc * 3 + sl];
+      Sn[cc] = Ec[cc];
     }
     const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
     const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
@@ -363,18 +390,21 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
 #pragma unroll
       for (int m = 0; m < 3; m++) sT[(m * SC + ts + r) * PST + p] = t[m];
     }
+    // products t2, t3 overlay this shot's u / dt*v rows: written once every lane is past them
+    lds_barrier();
+    if (active) {
 #pragma unroll
-    for (int m = 0; m < 6; m++) t23[r][m] = t[3 + m];
-    __builtin_amdgcn_sched_barrier(0);            // one shot's state in registers at a time
+      for (int m = 3; m < 9; m++) sT[(m * SC + ts + r) * PST + p] = t[m];
+    }
+    if (r < 3) {
+#pragma unroll
+      for (int pl = 0; pl < 18; pl++) Rc[pl] = Rn[pl];
+#pragma unroll
+      for (int cc = 0; cc < 6; cc++) Ec[cc] = En[cc];
+    }
+    __builtin_amdgcn_sched_barrier(0);            // at most two shots' state in registers
   }
-  __syncthreads();                                // every lane is done with the u / dt*v tile
-  if (active) {
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-      for (int m = 3; m < 9; m++) sT[(m * SC + ts + r) * PST + p] = t23[r][m - 3];
-  }
-  __syncthreads();
+  lds_barrier();
   if (!mine) return;
 
   realw* a = B.a + row;
