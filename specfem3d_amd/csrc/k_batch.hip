@@ -64,6 +64,15 @@ __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local address
+// space, so the wait before s_barrier is lgkmcnt(0) alone and global loads and stores issued
+// earlier stay in flight across it (a plain __syncthreads waits for them with vmcnt(0)).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // OV = 1 overlays the displacement tile on the stress-product planes (9 instead of 12 planes of
 // LDS, one more barrier; the lane keeps its own displacement in registers for the fused update),
 // which lets more workgroups share a CU.
@@ -102,15 +111,25 @@ k_elastic_batch(BatchArgs B, GLLk G) {
   const int g = B.ibool[(int64_t)e * 125 + pp];
   const int64_t row = (int64_t)g * 3 * S + sq;
   const int ts = 4 * q;                           // tile slot of the quad
+  // Load order matters (the vector-memory counter retires in order): point flags, displacement,
+  // moduli and Jacobian terms first, then the lane's accumulator and velocity rows, which are
+  // only needed after the element computation and stay in flight across the LDS-only barriers
+  // (a: unless this lane writes first; v: only where it finishes the update).
+  const unsigned fl = MODE == 3 ? (unsigned)B.flags[(int64_t)e * 125 + pp] : 0u;
   float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;  // staged into LDS below
   if (live0) {
     u0 = *reinterpret_cast<const float4*>(B.u + row);
     u1 = *reinterpret_cast<const float4*>(B.u + row + S);
     u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
   }
-  // the lane's accumulator and velocity rows are loaded now, so they are in flight during the
-  // element computation (a: unless this lane writes first; v: only where it finishes the update)
-  const unsigned fl = MODE == 3 ? (unsigned)B.flags[(int64_t)e * 125 + pp] : 0u;
+  const float2 km = B.kappamu[(int64_t)e * 125 + pp];
+  JacPt J = {};
+  realw jacl = B.jacobian_regular;
+  if (irr) {
+    J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
+    jacl = J.jac;
+  }
+  __builtin_amdgcn_sched_barrier(0);
   const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
   const bool last = MODE == 3 && (fl & BF_LAST) != 0;
   const bool mine = active && live0;
@@ -122,6 +141,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
     if (MODE != 0 && mine && (!first || !live[3])) A[c] = *reinterpret_cast<const float4*>(B.a + row + c * S);
     if (last && mine) V[c] = *reinterpret_cast<const float4*>(B.v + row + c * S);
   }
+  __builtin_amdgcn_sched_barrier(0);
   if (active) {
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -130,14 +150,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
       sU[(2 * SC + ts + r) * PST + p] = c4(u2, r);
     }
   }
-  JacPt J = {};
-  realw jacl = B.jacobian_regular;
-  if (irr) {
-    J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
-    jacl = J.jac;
-  }
-  const float2 km = B.kappamu[(int64_t)e * 125 + pp];
-  __syncthreads();
+  lds_barrier();
 
   const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
   realw t3[4][3];                                 // products t3 of each shot (written last with OV)
@@ -156,7 +169,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
     t3[r][0] = t[6]; t3[r][1] = t[7]; t3[r][2] = t[8];
   }
   if (OV) {
-    __syncthreads();                              // every lane is done with the displacement tile
+    lds_barrier();                                // every lane is done with the displacement tile
     if (active) {
 #pragma unroll
       for (int r = 0; r < 4; r++)
@@ -164,7 +177,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
         for (int m = 6; m < 9; m++) sT[(m * SC + ts + r) * PST + p] = t3[r][m - 6];
     }
   }
-  __syncthreads();
+  lds_barrier();
   if (!mine) return;
 
   // one force component at a time for the lane's four shots (keeps few values live)
@@ -225,15 +238,6 @@ k_elastic_batch(BatchArgs B, GLLk G) {
 // order are the single-shot point kernel's (k_elastic_pt): strain at u + dt v, isotropic stress
 // minus the memory-variable sums, memory update, stress products. Assembly as k_elastic_batch
 // (MODE 2 coloured read-modify-write, 3 coloured with the Newmark update fused).
-// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local address
-// space, so the wait before s_barrier is lgkmcnt(0) alone and global loads and stores issued
-// earlier stay in flight across it (a plain __syncthreads waits for them with vmcnt(0)).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Per-shot state is one block per (element, shot), point-fastest within each plane: the lanes of a
 // wave that read one shot read consecutive words (a shot-fastest row would spread a wave's reads
 // of one shot over S times as many cache lines), and the planes of a block sit at fixed offsets
