@@ -16,6 +16,15 @@
 
 namespace sf {
 
+// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local address
+// space, so the wait before s_barrier is lgkmcnt(0) alone and global loads and stores issued
+// earlier stay in flight across it (a plain __syncthreads waits for them with vmcnt(0)).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 struct Grad9 {  // d/dxi, d/deta, d/dgamma of the three components
   realw x1, y1, z1, x2, y2, z2, x3, y3, z3;
 };

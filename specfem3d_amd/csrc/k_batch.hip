@@ -64,15 +64,6 @@ __device__ __forceinline__ int xcd_remap_b(int b, int nb) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local address
-// space, so the wait before s_barrier is lgkmcnt(0) alone and global loads and stores issued
-// earlier stay in flight across it (a plain __syncthreads waits for them with vmcnt(0)).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // OV = 1 overlays the displacement tile on the stress-product planes (9 instead of 12 planes of
 // LDS, one more barrier; the lane keeps its own displacement in registers for the fused update),
 // which lets more workgroups share a CU.

@@ -502,7 +502,7 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
 #pragma unroll
     for (int m = 0; m < 9; m++) sT[m][p] = t[m];
   }
-  __syncthreads();
+  lds_barrier();  // the memory-variable and strain stores drain during the divergence
   if (!active) return;
 
   realw f[3];
