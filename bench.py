@@ -325,6 +325,9 @@ def main():
     ap.add_argument("--element-order", default="none",
                     choices=["morton", "none", "brick1x2", "brick2x1", "brick2x2", "brick1x4", "brick4x4", "brick2x4"],
                     help="launch order of the elastic elements of the single-shot run")
+    ap.add_argument("--att-variant", type=int, default=-1,
+                    help="attenuation schedule of the single-shot stiffness kernel (-1: library default)")
+    ap.add_argument("--pipe-wgs", type=int, default=0, help="persistent workgroups of schedule 5 (0: default)")
     ap.add_argument("--skip-single", action="store_true",
                     help="skip the single-shot 96x96x48 measurement (profiling runs of the other parts)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU oracle sample (0: skip)")
@@ -383,9 +386,14 @@ def main():
     db = w.db
     log(f"[bench] built {w.name}: {db.summary()} in {time.time() - t0:.1f}s")
 
+    tuning = {}
+    if args.att_variant >= 0:
+        tuning["att_variant"] = args.att_variant
+    if args.pipe_wgs > 0:
+        tuning["pipe_wgs"] = args.pipe_wgs
     g = GPUSolver(db, w.sources, w.receivers, nstep=nstep_total, attenuation=w.attenuation,
                   exact=bool(args.exact), assembly=args.assembly,
-                  element_order=None if args.element_order == "none" else args.element_order)
+                  element_order=None if args.element_order == "none" else args.element_order, tuning=tuning)
     log("[bench] prepared device arrays")
     for _ in range(args.warmup):
         g.step()
@@ -453,7 +461,8 @@ def main():
         "config": {"workload": w.name, "nglob": nglob, "nspec": db.nspec, "nspec_acoustic": db.nspec_acoustic,
                    "attenuation": "3 SLS, Q_mu=100", "receivers": w.receivers.n, "exact": args.exact,
                    "assembly": ["atomic", "ordered", "coloured"][args.assembly],
-                   "element_order": args.element_order},
+                   "element_order": args.element_order,
+                   "att_variant": "default" if args.att_variant < 0 else args.att_variant},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -77,6 +77,7 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
     a.gamma[s] = f ? m->b_gammaval[s] : m->gammaval[s];
   }
   a.local_out = m->asm_buf;
+  a.pipe_wgs = m->pipe_wgs;
   return a;
 }
 

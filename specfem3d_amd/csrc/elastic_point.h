@@ -16,14 +16,12 @@
 
 namespace sf {
 
-// Workgroup barrier that orders LDS accesses only: the fences are restricted to the local address
-// space, so the wait before s_barrier is lgkmcnt(0) alone and global loads and stores issued
-// earlier stay in flight across it (a plain __syncthreads waits for them with vmcnt(0)).
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
+// Workgroup barrier that orders LDS accesses only: this wave's LDS operations complete
+// (lgkmcnt(0)), then s_barrier; the "memory" clobber keeps the compiler from moving memory
+// accesses across it. Global loads, stores and atomics issued earlier, and LDS-direct loads
+// (global_load_lds, which complete through vmcnt and are read back only by the lane that issued
+// them), stay in flight across it; __syncthreads would wait for all of them (vmcnt(0)).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 struct Grad9 {  // d/dxi, d/deta, d/dgamma of the three components
   realw x1, y1, z1, x2, y2, z2, x3, y3, z3;

@@ -367,6 +367,15 @@ __device__ __forceinline__ void load_att_state(const ElasticArgs& A, int e, int 
 typedef __attribute__((address_space(3))) void* sf_lds_ptr;
 typedef __attribute__((address_space(1))) void* sf_global_ptr;
 
+// One dword per lane from global memory straight into LDS at lds_base + 4 * lane, issued as
+// assembly so the compiler does not track it: its waits for LDS-direct data would be vmcnt(0)
+// before every read of any LDS array it cannot tell apart from the target. The caller waits
+// (vmcnt) before the lane reads its slot back. (M0 needs one wait state before the load.)
+__device__ __forceinline__ void lds_dma_dword(const realw* g, uint32_t lds_base) {
+  const uint32_t b = __builtin_amdgcn_readfirstlane(lds_base);  // wave-uniform
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(b) : "memory", "m0");
+}
+
 template <bool STRAIN, bool ATT, int MODE, int AV>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(AV >= 2 ? 5 : 1)))
 k_elastic_pt(ElasticArgs A, GLLk G) {
@@ -526,6 +535,256 @@ k_elastic_pt(ElasticArgs A, GLLk G) {
   }
 }
 
+// Schedule 5 (viscoelastic only): a persistent workgroup walks the elements ie, ie + NG, ie + 2 NG,
+// ... (NG = grid size) as a software pipeline, so that the next element's loads are in flight
+// while this element is computed: its displacement and velocity, moduli and factor row are
+// loaded into registers at the top of the iteration, and its memory variables and stored strain
+// go straight into LDS (global_load_lds) once this lane has read back its own. Element indices
+// run three deep (the list entry two elements ahead, the ibool row one ahead) so no load in the
+// loop waits for another. The stress products have LDS planes of their own, so the state planes
+// are free for the next element as soon as they are read. Barriers order LDS only (lds_barrier),
+// leaving global loads, stores and atomics in flight. Per element the arithmetic and its order
+// are k_elastic_pt's, so the results are the same bits.
+template <int MODE>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4)))
+k_elastic_pipe(ElasticArgs A, GLLk G) {
+  __shared__ realw sU[6][128];   // u (0..2) and dt*v (3..5) of the current element
+  __shared__ realw sX[24][128];  // memory variables (0..17) and stored strain (18..23), by DMA
+  __shared__ realw sT[9][128];   // stress products
+  __shared__ realw sG[75];
+  const int p = threadIdx.x;
+  const bool active = p < 125;
+  stage_gll(sG, G, p);
+  const int q = active ? p : 124;
+  const int k = q / 25, j = (q / 5) % 5, i = q % 5;
+  const int w64 = p & 64;
+  const int64_t nl = A.nloc;
+  const int NG = gridDim.x;
+  const int nelem = A.nelem;
+  int ie = xcd_remap(blockIdx.x, NG);
+  if (ie >= nelem) return;  // the launcher keeps NG <= nelem
+  const bool tab = A.fc_index != nullptr;
+  // element indices of this and the next two iterations (a finished walk repeats its last
+  // element in the look-ahead slots, so every load stays in bounds; nothing is written for them)
+  int ie_v = ie;
+  asm volatile("" : "+v"(ie_v));
+  int e = A.elems[ie_v];
+  int e_n = ie + NG < nelem ? A.elems[ie_v + NG] : e;
+  int e_nn = ie + 2 * NG < nelem ? A.elems[ie_v + 2 * NG] : e_n;
+  int64_t ep = (int64_t)e * 125 + q;
+  int g = A.ibool[ep];
+  int g_n = A.ibool[(int64_t)e_n * 125 + q];
+  {
+    const realw* d = A.displ + 3 * (int64_t)g;
+    const realw* v = A.veloc + 3 * (int64_t)g;
+    const realw u0 = d[0], u1 = d[1], u2 = d[2], v0 = v[0], v1 = v[1], v2 = v[2];
+    sU[0][p] = u0; sU[1][p] = u1; sU[2][p] = u2;
+    sU[3][p] = A.deltat * v0; sU[4][p] = A.deltat * v1; sU[5][p] = A.deltat * v2;
+  }
+  float2 km = A.kappamu[ep];
+  int irr = A.irregular[e];
+  int fcr = tab ? (int)A.fc_index[ep] : 0;
+  asm volatile("" : "+v"(irr));
+  const uint32_t xb = (uint32_t)(uintptr_t)(sf_lds_ptr)&sX[0][w64];  // this wave's slots of plane 0
+#pragma unroll
+  for (int pl = 0; pl < 18; pl++) lds_dma_dword(A.R + r_idx(e, pl, q), xb + 512 * pl);
+#pragma unroll
+  for (int c = 0; c < 6; c++) lds_dma_dword(A.eps + eps_idx(e, c, q), xb + 512 * (18 + c));
+  lds_barrier();
+
+  const realw ONE_THIRD = (realw)1 / (realw)3;
+  for (bool first = true;; first = false) {
+    const bool has_next = ie + NG < nelem;  // uniform over the workgroup
+    // next element: u, v, moduli, factor row, Jacobian index; the ibool row after it; the list
+    // entry after that. This element: its factor-table row and Jacobian terms.
+    const int64_t ep_n = (int64_t)e_n * 125 + q;
+    const int g_nn = A.ibool[(int64_t)e_nn * 125 + q];
+    // list entries and Jacobian indices are read through a vector register: loaded from a
+    // uniform address they would be moved to a scalar register right after the load, which
+    // waits for every memory operation issued before it (this element's state, the previous
+    // element's stores and atomics)
+    int ie3 = ie + 3 * NG < nelem ? ie + 3 * NG : ie;
+    asm volatile("" : "+v"(ie3));
+    const int e_nnn = ie + 3 * NG < nelem ? A.elems[ie3] : e_nn;
+    realw un0, un1, un2, vn0, vn1, vn2;
+    {
+      const realw* d = A.displ + 3 * (int64_t)g_n;
+      const realw* v = A.veloc + 3 * (int64_t)g_n;
+      un0 = d[0]; un1 = d[1]; un2 = d[2]; vn0 = v[0]; vn1 = v[1]; vn2 = v[2];
+    }
+    const float2 km_n = A.kappamu[ep_n];
+    int en_v = e_n;
+    asm volatile("" : "+v"(en_v));
+    const int irr_n = A.irregular[en_v];
+    const int fcr_n = tab ? (int)A.fc_index[ep_n] : 0;
+    realw fcv[3], fckv[3];
+    if (tab) {
+#pragma unroll
+      for (int sl = 0; sl < 3; sl++) {
+        fcv[sl] = A.fc_table[sl * A.fc_rows + fcr];
+        fckv[sl] = A.fc_table[(3 + sl) * A.fc_rows + fcr];
+      }
+    } else {
+#pragma unroll
+      for (int sl = 0; sl < 3; sl++) {
+        fcv[sl] = A.fc[sl * nl + ep];
+        fckv[sl] = A.fck[sl * nl + ep];
+      }
+    }
+    JacPt J = {};
+    realw jacl = A.jacobian_regular;
+    if (irr) {
+      J = load_jac(A.jac, A.jac_plane, (int64_t)(irr - 1) * 125 + q);
+      jacl = J.jac;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+
+    const Grad9 tg = tile_gradient<1>(sU[0], sU[1], sU[2], sG, i, j, k);
+    const Du9 du = map_gradient(tg, irr != 0, J, A.xix_regular);
+    const Du9 st = map_gradient(tile_gradient_continue<1>(tg, sU[3], sU[4], sU[5], sG, i, j, k), irr != 0, J,
+                                A.xix_regular);
+    const realw e_tr = (st.xx + st.yy + st.zz);
+    const realw templ = ONE_THIRD * (st.xx + st.yy + st.zz);
+    if (A.eps_tr3 && active) A.eps_tr3[ep] = templ;
+    const realw Snp1[6] = {st.xx - templ, st.yy - templ, (realw)0.5 * (st.xy + st.yx), (realw)0.5 * (st.zx + st.xz),
+                           (realw)0.5 * (st.zy + st.yz), e_tr};
+    const realw kappal = km.x, mul = km.y;
+    Sigma6 sg = iso_stress(du, kappal, mul);
+
+    // this element's state has landed; read own slots. In the first iteration everything is
+    // waited for; later, at least the 24 memory-variable and strain stores of the previous
+    // element were issued after this element's LDS-direct loads, so vmcnt(24) suffices and those
+    // stores (and the previous element's assembly) stay in flight.
+    if (first) {
+      __builtin_amdgcn_s_waitcnt(0);
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x4f78);  // vmcnt(24)
+    }
+    // The 24 slots are read in one asm block that ends with lgkmcnt(0): read through the
+    // compiler, each read of a plane that an LDS-direct load wrote would get a vmcnt(0) first,
+    // which would also wait for the previous element's stores and atomics.
+    realw Rv[6][3], Sn[6];
+    {
+      const uint32_t sa = (uint32_t)(uintptr_t)(sf_lds_ptr)&sX[0][p];
+      double x[12];
+      asm volatile(
+          "ds_read2st64_b32 %0, %12 offset0:0 offset1:2\n\t"
+          "ds_read2st64_b32 %1, %12 offset0:4 offset1:6\n\t"
+          "ds_read2st64_b32 %2, %12 offset0:8 offset1:10\n\t"
+          "ds_read2st64_b32 %3, %12 offset0:12 offset1:14\n\t"
+          "ds_read2st64_b32 %4, %12 offset0:16 offset1:18\n\t"
+          "ds_read2st64_b32 %5, %12 offset0:20 offset1:22\n\t"
+          "ds_read2st64_b32 %6, %12 offset0:24 offset1:26\n\t"
+          "ds_read2st64_b32 %7, %12 offset0:28 offset1:30\n\t"
+          "ds_read2st64_b32 %8, %12 offset0:32 offset1:34\n\t"
+          "ds_read2st64_b32 %9, %12 offset0:36 offset1:38\n\t"
+          "ds_read2st64_b32 %10, %12 offset0:40 offset1:42\n\t"
+          "ds_read2st64_b32 %11, %12 offset0:44 offset1:46\n\t"
+          "s_waitcnt lgkmcnt(0)"
+          : "=v"(x[0]), "=v"(x[1]), "=v"(x[2]), "=v"(x[3]), "=v"(x[4]), "=v"(x[5]), "=v"(x[6]), "=v"(x[7]),
+            "=v"(x[8]), "=v"(x[9]), "=v"(x[10]), "=v"(x[11])
+          : "v"(sa)
+          : "memory");
+      realw pl[24];
+#pragma unroll
+      for (int h = 0; h < 12; h++) {
+        const float2 f2 = __builtin_bit_cast(float2, x[h]);
+        pl[2 * h] = f2.x;
+        pl[2 * h + 1] = f2.y;
+      }
+#pragma unroll
+      for (int cc = 0; cc < 6; cc++)
+#pragma unroll
+        for (int sl = 0; sl < 3; sl++) Rv[cc][sl] = pl[cc * 3 + sl];
+#pragma unroll
+      for (int c = 0; c < 6; c++) Sn[c] = pl[18 + c];
+    }
+    // every register operand of the memory update is taken before the next element's loads are
+    // issued, so no wait after them can include those loads
+    realw flk[3], flm[3];
+#pragma unroll
+    for (int sl = 0; sl < 3; sl++) {
+      flk[sl] = km.x * fckv[sl];
+      flm[sl] = km.y * fcv[sl];
+    }
+    asm volatile("" ::"v"(flk[0]), "v"(flk[1]), "v"(flk[2]), "v"(flm[0]), "v"(flm[1]), "v"(flm[2]));
+    if (has_next) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int pl = 0; pl < 18; pl++) lds_dma_dword(A.R + r_idx(e_n, pl, q), xb + 512 * pl);
+#pragma unroll
+      for (int c = 0; c < 6; c++) lds_dma_dword(A.eps + eps_idx(e_n, c, q), xb + 512 * (18 + c));
+      __builtin_amdgcn_sched_barrier(0);
+    }
+
+    const realw rxx = (Rv[0][0] + Rv[0][1]) + Rv[0][2];
+    const realw ryy = (Rv[1][0] + Rv[1][1]) + Rv[1][2];
+    const realw rtr = (Rv[5][0] + Rv[5][1]) + Rv[5][2];
+    sg.xx = sg.xx - rxx - rtr;
+    sg.yy = sg.yy - ryy - rtr;
+    sg.zz = sg.zz + rxx + ryy - rtr;
+    sg.xy = sg.xy - ((Rv[2][0] + Rv[2][1]) + Rv[2][2]);
+    sg.xz = sg.xz - ((Rv[3][0] + Rv[3][1]) + Rv[3][2]);
+    sg.yz = sg.yz - ((Rv[4][0] + Rv[4][1]) + Rv[4][2]);
+    if (active) {
+#pragma unroll
+      for (int sl = 0; sl < 3; sl++) {
+        A.R[r_idx(e, 5 * 3 + sl, q)] =
+            A.alpha[sl] * Rv[5][sl] + flk[sl] * (A.beta[sl] * Sn[5] + A.gamma[sl] * Snp1[5]);
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++)
+          A.R[r_idx(e, cc * 3 + sl, q)] =
+              A.alpha[sl] * Rv[cc][sl] + flm[sl] * (A.beta[sl] * Sn[cc] + A.gamma[sl] * Snp1[cc]);
+      }
+      realw* Ep = A.eps;
+#pragma unroll
+      for (int c = 0; c < 6; c++) Ep[eps_idx(e, c, q)] = Snp1[c];
+    }
+    {
+      realw t[9];
+      stress_products(sg, irr != 0, J, jacl, A.xix_regular, t);
+#pragma unroll
+      for (int m = 0; m < 9; m++) sT[m][p] = t[m];
+    }
+    lds_barrier();  // products complete; every lane is past this element's u tile
+    if (has_next) {
+      sU[0][p] = un0; sU[1][p] = un1; sU[2][p] = un2;
+      sU[3][p] = A.deltat * vn0; sU[4][p] = A.deltat * vn1; sU[5][p] = A.deltat * vn2;
+    }
+    if (active) {
+      realw f[3];
+      tile_divergence<1>(&sT[0][0], 128, sG + 25, sG + 50, i, j, k, f);
+      if (MODE == 1) {
+        A.local_out[ep] = f[0];
+        A.local_out[nl + ep] = f[1];
+        A.local_out[2 * nl + ep] = f[2];
+      } else if (MODE == 2) {
+        realw* a = A.accel + 3 * (int64_t)g;
+        a[0] = a[0] + (-f[0]);
+        a[1] = a[1] + (-f[1]);
+        a[2] = a[2] + (-f[2]);
+      } else {
+        realw* a = A.accel + 3 * (int64_t)g;
+        atomicAdd(a + 0, -f[0]);
+        atomicAdd(a + 1, -f[1]);
+        atomicAdd(a + 2, -f[2]);
+      }
+    }
+    lds_barrier();  // products read; the next element's tile is staged
+    if (!has_next) break;
+    ie += NG;
+    e = e_n;
+    e_n = e_nn;
+    e_nn = e_nnn;
+    ep = ep_n;
+    g = g_n;
+    g_n = g_nn;
+    km = km_n;
+    irr = irr_n;
+    fcr = fcr_n;
+  }
+}
+
 // element strain of a displacement field (compute_element_strain.F90:28-130); one lane per point
 __global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, realw* eps, realw* eps_tr3,
                                                         int with_trace) {
@@ -611,6 +870,15 @@ void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool 
                          int att_variant) {
   if (a.nelem <= 0) return;
   const GLLk G = g_gll_host;
+  if (shape == 1 && att && att_variant == 5) {
+    const int ng = a.pipe_wgs > 0 && a.pipe_wgs < a.nelem ? a.pipe_wgs : a.nelem;
+    dim3 grid(ng), block(128);
+    if (mode == 1) hipLaunchKernelGGL((k_elastic_pipe<1>), grid, block, 0, s, a, G);
+    else if (mode == 2) hipLaunchKernelGGL((k_elastic_pipe<2>), grid, block, 0, s, a, G);
+    else hipLaunchKernelGGL((k_elastic_pipe<0>), grid, block, 0, s, a, G);
+    SF_LAUNCH_CHECK("k_elastic_pipe");
+    return;
+  }
   if (shape == 1) {
     dim3 grid(a.nelem), block(128);
 #define SF_K(S, T, O, V) hipLaunchKernelGGL((k_elastic_pt<S, T, O, V>), grid, block, 0, s, a, G)

@@ -53,6 +53,7 @@ struct ElasticArgs {
   int64_t nloc;            // nspec*125
   realw alpha[3], beta[3], gamma[3];
   realw* local_out;        // ordered assembly: 3 planes of nloc
+  int pipe_wgs;            // schedule 5: persistent workgroups (at most nelem are launched)
 };
 // mode: 0 atomic scatter, 1 store local contributions (ordered gather), 2 plain read-modify-write
 // (the element list is one colour: its elements share no global point); shape: 0 five elements

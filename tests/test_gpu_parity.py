@@ -18,12 +18,15 @@ from conftest import GOLDEN, rel_l2, small_elastic_box, smooth_warp
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, shape=0, att_variant=0):
+def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, shape=0, att_variant=0, pipe_wgs=None):
     from specfem3d_amd.solver import GPUSolver
     print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly} shape={shape} "
-          f"att_variant={att_variant}", flush=True)
+          f"att_variant={att_variant} pipe_wgs={pipe_wgs}", flush=True)
+    tuning = {"elastic_shape": shape, "att_variant": att_variant}
+    if pipe_wgs is not None:
+        tuning["pipe_wgs"] = pipe_wgs
     g = GPUSolver(db, src, rec, nstep=nstep, exact=exact, assembly=assembly, attenuation=attenuation,
-                  tuning={"elastic_shape": shape, "att_variant": att_variant})
+                  tuning=tuning)
     out = g.run().copy()
     g.close()
     return out
@@ -63,14 +66,33 @@ def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     assert rel_l2(fast, ref) < 1e-5
 
 
-@pytest.mark.parametrize("att_variant", [1, 2, 3, 4])
-def test_attenuation_schedules_bitwise(oracle_mod, att_variant):
-    """The point kernel's attenuation-state schedules give the oracle's results bit for bit."""
+@pytest.mark.parametrize("att_variant,pipe_wgs", [(1, None), (2, None), (3, None), (4, None), (5, None), (5, 7),
+                                                   (5, 100)])
+def test_attenuation_schedules_bitwise(oracle_mod, att_variant, pipe_wgs):
+    """The point kernel's attenuation-state schedules give the oracle's results bit for bit; the
+    pipelined schedule 5 with one element per workgroup (default size at this mesh) and with
+    7 or 100 persistent workgroups walking 864 elements (uneven walks)."""
     from specfem3d_amd import workloads as W
     w = W.config2(scale=0.125, nstep=150, with_attenuation=True, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 150, w.attenuation)
-    out = run_gpu(w.db, w.sources, w.receivers, 150, True, 1, w.attenuation, shape=1, att_variant=att_variant)
+    out = run_gpu(w.db, w.sources, w.receivers, 150, True, 1, w.attenuation, shape=1, att_variant=att_variant,
+                  pipe_wgs=pipe_wgs)
     assert np.array_equal(out, ref), rel_l2(out, ref)
+
+
+def test_pipelined_schedule_irregular_and_coloured(oracle_mod):
+    """Schedule 5 on a warped (irregular) viscoelastic box: ordered assembly equals the oracle bit
+    for bit with 3 persistent workgroups, and coloured assembly equals schedule 4's coloured run."""
+    from specfem3d_amd.attenuation import setup_attenuation
+    db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0))
+    att = setup_attenuation(db, qmu=60.0, qkappa=9000.0)
+    ref = run_oracle(oracle_mod, db, src, rec, 120, att)
+    out = run_gpu(db, src, rec, 120, True, 1, att, shape=1, att_variant=5, pipe_wgs=3)
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(out, ref), rel_l2(out, ref)
+    c4 = run_gpu(db, src, rec, 120, True, 2, att, shape=1, att_variant=4)
+    c5 = run_gpu(db, src, rec, 120, True, 2, att, shape=1, att_variant=5, pipe_wgs=5)
+    assert np.array_equal(c5, c4), rel_l2(c5, c4)
 
 
 @pytest.mark.parametrize("shape", [0, 1])
