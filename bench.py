@@ -28,6 +28,29 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 
+def copy_peak_gbs() -> float:
+    """Achievable HBM bandwidth on this GPU: a 1 GiB device-to-device copy (torch ``copy_``), read +
+    written bytes over the average time of 20 copies (HIP events on torch's stream)."""
+    import torch
+    n = 256 * 1024 * 1024
+    x = torch.ones(n, device="cuda:0", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(20):
+        y.copy_(x)
+    t1.record()
+    t1.synchronize()
+    ms = t0.elapsed_time(t1) / 20
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * 4 * n / (ms / 1e3) / 1e9
+
+
 def measured_traffic(assembly: int, exact: int):
     """HBM bytes per stiffness launch from the committed rocprofv3 PMC summary of this mode.
 
@@ -426,6 +449,11 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_elastic_pt (viscoelastic 3-SLS stiffness)", "bytes_per_launch": kbytes,
                 "avg_launch_ms": round(kavg_s * 1e3, 4), "traffic_source": tsrc}
+
+    copy_gbs = copy_peak_gbs()
+    roofline["measured_copy_peak"] = round(copy_gbs, 1)
+    roofline["frac_of_copy_peak"] = round(achieved / copy_gbs, 4)
+    log(f"[bench] 1 GiB device copy: {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:

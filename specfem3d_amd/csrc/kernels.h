@@ -88,8 +88,9 @@ struct BatchArgs {
   const realw* rmass;      // inverse mass, 3 per global point
   const unsigned char* flags;  // per local point e*125+p: BF_FIRST | BF_LAST | BF_ZERO
   realw dt, dt2, dtsq2;
-  // 3-SLS attenuation (k_visco_batch): per-shot memory variables and strain in per-element blocks,
-  // value (e, plane, p, shot) at ((e * 18 + plane) * 125 + p) * S + shot (strain: 6 planes)
+  // 3-SLS attenuation (k_visco_batch): per-shot memory variables and strain, one block per
+  // (element, shot): value (e, shot, plane, p) at ((e * S + shot) * 18 + plane) * 125 + p
+  // (strain: 6 planes)
   realw* R;
   realw* eps;
   const uint16_t* fc_index;    // SLS factor row per local point (or null: fc / fck planes)
