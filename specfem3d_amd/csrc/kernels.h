@@ -100,11 +100,12 @@ struct BatchArgs {
   const realw* fck;
   int64_t nloc;
   realw alpha[3], beta[3], gamma[3];
+  int pipe_wgs;                // persistent workgroups of the pipelined kernel (0: default)
 };
 enum : unsigned char { BF_FIRST = 1, BF_LAST = 2, BF_ZERO = 4 };
 // sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour),
 // 3 one colour with the Newmark update fused (see k_batch.hip); overlay 1 shares the LDS of the
-// displacement tile and the stress products
+// displacement tile and the stress products, 2 selects the persistent pipelined kernel (modes 2, 3)
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact);
 // viscoelastic (3 SLS) batched stiffness; mode 2 or 3
 void launch_visco_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
