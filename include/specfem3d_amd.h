@@ -246,9 +246,10 @@ void unregister_host_array_(realw* h_array);
    colouring with a plain read-modify-write scatter (deterministic, no atomics). */
 void sf_set_options(long* Mesh_pointer, int exact, int assembly);
 /* tuning switch by name: "elastic_shape" 0 = column-per-lane stiffness kernel (five elements per
-   workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0..4 =
+   workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0..5 =
    scheduling of the attenuation state in the point kernel (same results; 4, the default, loads it
-   straight into LDS); "acoustic_kernel_field"
+   straight into LDS; 5 = persistent workgroups that load the next element while computing one);
+   "pipe_wgs" >= 1 = workgroups of schedule 5 (default 2048); "acoustic_kernel_field"
    0 = compute_kernels_acoustic_cuda pairs the adjoint potential with the reconstructed field as the
    reference does (compute_kernels.f90:269,325), 1 = it uses the adjoint potential_dot_dot instead
    (the misfit gradient when the adjoint field enters the fluid through the coupling);
@@ -282,7 +283,8 @@ void sf_batch_finalize(long batch);
 void sf_batch_set_options(long batch, int shots_per_group, int assembly);
 void sf_batch_get_options(long batch, int* shots_per_group, int* assembly);
 /* kernel variants with the same results: "lds_overlay" 0/1 (displacement tile and stress products
-   share LDS); unknown names are fatal */
+   share LDS), 2 (persistent pipelined kernel; elastic shots, coloured assembly); "pipe_wgs" >= 0
+   (workgroups of the pipelined kernel, 0 = its default); unknown names are fatal */
 void sf_batch_set_tuning(long batch, const char* name, int value);
 int sf_batch_get_tuning(long batch, const char* name);
 double sf_batch_autotune(long batch, int trial_steps);

@@ -357,7 +357,8 @@ __device__ __forceinline__ void load_att_state(const ElasticArgs& A, int e, int 
 // between phases; the per-point state (memory variables, stored strain, SLS factors, moduli,
 // Jacobian terms) is loaded before the first barrier and is in flight while the tile is
 // gathered. Same arithmetic, in the same order, as k_elastic.
-// AV selects how the attenuation state is scheduled (the autotuner's choice; same results):
+// AV selects how the attenuation state is scheduled (the "att_variant" tuning switch; same
+// results; schedule 5 is the persistent kernel k_elastic_pipe below):
 // 0 every per-point array is loaded before the first barrier; 1 only the memory variables are,
 // the stored strain and SLS factors are loaded where the memory update needs them (fewer live
 // registers); 2 as 1 with the register budget capped for 5 waves per SIMD; 3 as 0 with that cap;
