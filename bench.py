@@ -59,7 +59,7 @@ def measured_traffic(assembly: int, exact: int):
     """
     if exact:
         return None, None
-    name = os.path.join("profiles", f"k_elastic_pt_config2_{['atomic', 'ordered', 'coloured'][assembly]}.md")
+    name = os.path.join("profiles", f"k_elastic_pt_sea_over_visco_96x96x48_{['atomic', 'ordered', 'coloured'][assembly]}.md")
     try:
         txt = open(os.path.join(HERE, name)).read()
     except OSError:
@@ -145,7 +145,7 @@ def batched_measurement(args) -> dict:
     from specfem3d_amd import workloads as W
     from specfem3d_amd.batch import BatchSolver
     from specfem3d_amd.solver import GPUSolver
-    w = W.config3(scale=args.batch_scale, nshots=args.batch_shots)
+    w = W.gradient_shots_box(scale=args.batch_scale, nshots=args.batch_shots)
     if args.batch_max_steps > 0:
         w.shots = [(s, r, min(n, args.batch_max_steps)) for s, r, n in w.shots]
     db = w.db
@@ -205,7 +205,7 @@ def batched_visco_measurement(args) -> dict:
     from specfem3d_amd.attenuation import setup_attenuation
     from specfem3d_amd.batch import BatchSolver
     from specfem3d_amd.solver import GPUSolver
-    w = W.config3(scale=args.batch_scale, nshots=args.visco_shots)
+    w = W.gradient_shots_box(scale=args.batch_scale, nshots=args.visco_shots)
     att = setup_attenuation(w.db, qmu=100.0, qkappa=9000.0)
     n = args.visco_steps
     shots = [(s, r, n) for s, r, _ in w.shots]
@@ -259,7 +259,7 @@ def adjoint_measurement(args) -> dict:
     from specfem3d_amd import workloads as W
     from specfem3d_amd.autograd import perturbed_database
     from specfem3d_amd.solver import GPUSolver
-    w = W.config4(scale=args.adjoint_scale, nstep=args.adjoint_steps)
+    w = W.gradient_adjoint_box(scale=args.adjoint_scale, nstep=args.adjoint_steps)
     db = w.db
     n = args.adjoint_steps
     g = GPUSolver(perturbed_database(db, dlnmu=0.03 * W.gaussian_bump(db)), w.sources, w.receivers, nstep=n,
@@ -298,7 +298,7 @@ def large_measurement(args) -> dict:
                          "single-shot run stalls after set-up for a reason not yet found")
     t0 = time.time()
     n = args.large_steps + 3
-    w = W.config5(scale=args.large_scale, nshots=1, nstep=n)
+    w = W.sea_over_visco_large(scale=args.large_scale, nshots=1, nstep=n)
     db = w.db
     log(f"[bench] built {w.name}: NGLOB {db.nglob} in {time.time() - t0:.1f}s")
     os.environ["SF_VERBOSE_PREPARE"] = "1"     # the set-up of 131.9 M points takes minutes: report its stages
@@ -405,7 +405,7 @@ def main():
 
     t0 = time.time()
     nstep_total = args.warmup + args.steps
-    w = W.config2(scale=args.scale, nstep=max(nstep_total, 1))
+    w = W.sea_over_visco_box(scale=args.scale, nstep=max(nstep_total, 1))
     db = w.db
     log(f"[bench] built {w.name}: {db.summary()} in {time.time() - t0:.1f}s")
 

@@ -1,18 +1,18 @@
 """The five workloads the solver loop is measured and checked on (synthetic, seeded).
 
-1. ``config1`` -- 20^3-element homogeneous elastic box (10 km cube, vp 2800, vs 1500, rho 2300),
+1. ``homogeneous_box`` -- 20^3-element homogeneous elastic box (10 km cube, vp 2800, vs 1500, rho 2300),
    free top + Stacey on 5 faces, one CMT source at 5 km depth (Ricker, f0 = 1 Hz), DT 0.01 s,
    2000 steps, 10 surface receivers: the reference's CPU-runnable case.
-2. ``config2`` -- 96 x 96 x 48 elements (500 m cubes), top 6 layers acoustic (vp 1500, rho 1020)
+2. ``sea_over_visco_box`` -- 96 x 96 x 48 elements (500 m cubes), top 6 layers acoustic (vp 1500, rho 1020)
    coupled to an elastic half-space with linear depth gradient x (1 + 5% seeded Gaussian random
    field, 2 km correlation), viscoelastic Q_mu = 100 with 3 SLS, one seeded random moment-tensor
    source, 4000 steps, 100 receivers on the sea floor.
-3. ``config3`` -- 64 x 64 x 32 elastic box (same gradient + GRF model, no attenuation), 32 shots
+3. ``gradient_shots_box`` -- 64 x 64 x 32 elastic box (same gradient + GRF model, no attenuation), 32 shots
    with seeded random locations, moment tensors, Ricker f0 in [0.5, 2] Hz, record lengths in
    [1000, 4000] steps, 50 receivers per shot.
-4. ``config4`` -- 48^3 elastic box for adjoint sensitivity kernels, 1 source, 4000 steps,
+4. ``gradient_adjoint_box`` -- 48^3 elastic box for adjoint sensitivity kernels, 1 source, 4000 steps,
    64 receivers.
-5. ``config5`` -- 160 x 160 x 80 (2.05 M elements, 131.9 M GLL points) viscoelastic with an
+5. ``sea_over_visco_large`` -- 160 x 160 x 80 (2.05 M elements, 131.9 M GLL points) viscoelastic with an
    acoustic top layer (10 element layers), 8 shots, 6000 steps.
 
 Each builder accepts a ``scale`` to shrink the element counts for tests; the GLL-point count
@@ -41,7 +41,7 @@ class Workload:
     meta: dict = field(default_factory=dict)
 
 
-CFG1_CMT = dict(Mrr=-7.4e19, Mtt=-1.13e21, Mpp=1.21e21, Mrt=3.81e20, Mrp=-8.0e18, Mtp=1.17e21)
+HOMOGENEOUS_BOX_CMT = dict(Mrr=-7.4e19, Mtt=-1.13e21, Mpp=1.21e21, Mrt=3.81e20, Mrp=-8.0e18, Mtp=1.17e21)
 
 
 def cmt_from_rtp(x, y, depth_km, hdur, Mrr, Mtt, Mpp, Mrt, Mrp, Mtp, tshift=0.0):
@@ -51,15 +51,15 @@ def cmt_from_rtp(x, y, depth_km, hdur, Mrr, Mtt, Mpp, Mrt, Mrp, Mtp, tshift=0.0)
                        Mxx=Mpp * s, Myy=Mtt * s, Mzz=Mrr * s, Mxy=-Mtp * s, Mxz=Mrp * s, Myz=-Mrt * s)
 
 
-def config1(n: int = 20, nstep: int = 2000, dt: float = 0.01) -> Workload:
+def homogeneous_box(n: int = 20, nstep: int = 2000, dt: float = 0.01) -> Workload:
     L = 10000.0
     db = build_box(n, n, n, (0.0, L, 0.0, L, -L, 0.0), material=homogeneous(2300.0, 2800.0, 1500.0))
     db.finalize(dt)
-    cmt = cmt_from_rtp(5000.0, 5000.0, 5.0, 1.0, **CFG1_CMT)
+    cmt = cmt_from_rtp(5000.0, 5000.0, 5.0, 1.0, **HOMOGENEOUS_BOX_CMT)
     src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
     st = [S.Station(f"X{i + 1:02d}", "DB", 5000.0 + 300.0 * (i % 2), 1000.0 + 900.0 * i, 0.0, 0.0) for i in range(10)]
     rec = S.setup_receivers(db, st)
-    return Workload("config1: 20x20x20 elastic box, 2000 steps", db, src, rec, nstep, dt)
+    return Workload("20x20x20 homogeneous elastic box, 2000 steps", db, src, rec, nstep, dt)
 
 
 def _model_extent(nx, ny, nz, h):
@@ -87,8 +87,8 @@ def receiver_grid(extent, n_side, z_below_top=0.0, prefix="R"):
     return st
 
 
-def config2(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int = 2, with_attenuation: bool = True,
-            qmu: float = 100.0, nrec_side: int = 10) -> Workload:
+def sea_over_visco_box(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int = 2,
+                       with_attenuation: bool = True, qmu: float = 100.0, nrec_side: int = 10) -> Workload:
     nx = max(2, int(round(96 * scale)))
     nz = max(8, int(round(48 * scale)))
     h = 500.0
@@ -108,12 +108,12 @@ def config2(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int 
     cmt.depth_km = max(cmt.depth_km, (seafloor + 2 * h) / 1000.0)
     src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
     rec = S.setup_receivers(db, receiver_grid(ext, nrec_side, z_below_top=seafloor + 1.0))
-    return Workload(f"config2: {nx}x{nx}x{nz} acoustic-over-viscoelastic, {nstep} steps", db, src, rec, nstep, dt,
+    return Workload(f"{nx}x{nx}x{nz} acoustic-over-viscoelastic, {nstep} steps", db, src, rec, nstep, dt,
                     attenuation=att, meta=dict(nx=nx, nz=nz, n_acoustic_layers=nac))
 
 
-def config3(scale: float = 1.0, nshots: int = 32, seed: int = 3, dt: float = 0.006, nrec: int = 50,
-            nstep_range=(1000, 4000)) -> Workload:
+def gradient_shots_box(scale: float = 1.0, nshots: int = 32, seed: int = 3, dt: float = 0.006, nrec: int = 50,
+                       nstep_range=(1000, 4000)) -> Workload:
     nx = max(2, int(round(64 * scale)))
     nz = max(2, int(round(32 * scale)))
     h = 500.0
@@ -132,11 +132,12 @@ def config3(scale: float = 1.0, nshots: int = 32, seed: int = 3, dt: float = 0.0
         rec = S.setup_receivers(db, st)
         n = int(rng.integers(nstep_range[0], nstep_range[1] + 1))
         shots.append((src, rec, n))
-    return Workload(f"config3: {nx}x{nx}x{nz} elastic, {nshots} shots", db, shots[0][0], shots[0][1],
+    return Workload(f"{nx}x{nx}x{nz} elastic gradient + GRF, {nshots} shots", db, shots[0][0], shots[0][1],
                     max(s[2] for s in shots), dt, shots=shots)
 
 
-def config4(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int = 4, nrec_side: int = 8) -> Workload:
+def gradient_adjoint_box(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int = 4,
+                         nrec_side: int = 8) -> Workload:
     n = max(2, int(round(48 * scale)))
     h = 500.0
     ext = _model_extent(n, n, n, h)
@@ -147,11 +148,11 @@ def config4(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006, seed: int 
     cmt = random_cmt(rng, ext, f0=(1.0, 1.0))
     src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
     rec = S.setup_receivers(db, receiver_grid(ext, nrec_side, 0.0))
-    return Workload(f"config4: {n}^3 elastic adjoint, {nstep} steps", db, src, rec, nstep, dt)
+    return Workload(f"{n}^3 elastic gradient + GRF adjoint, {nstep} steps", db, src, rec, nstep, dt)
 
 
-def config5(scale: float = 1.0, nshots: int = 8, nstep: int = 6000, dt: float = 0.006, seed: int = 5,
-            qmu: float = 100.0, nrec_side: int = 8) -> Workload:
+def sea_over_visco_large(scale: float = 1.0, nshots: int = 8, nstep: int = 6000, dt: float = 0.006, seed: int = 5,
+                         qmu: float = 100.0, nrec_side: int = 8) -> Workload:
     """160 x 160 x 80 elements (scale 1): the top 10 layers acoustic (vp 1500, rho 1020) over the
     gradient + GRF solid with Q_mu = 100 (3 SLS), ``nshots`` seeded random CMT sources below the sea
     floor sharing a sea-floor receiver grid."""
@@ -174,7 +175,7 @@ def config5(scale: float = 1.0, nshots: int = 8, nstep: int = 6000, dt: float = 
         cmt = random_cmt(rng, ext, zmin_frac=0.3, zmax_frac=0.7, f0=(0.5, 2.0), z_top=0.0)
         cmt.depth_km = max(cmt.depth_km, (seafloor + 2 * h) / 1000.0)
         shots.append((S.setup_cmt_sources(db, [cmt], dt, ricker=True), rec, nstep))
-    return Workload(f"config5: {nx}x{nx}x{nz} acoustic-over-viscoelastic, {nshots} shots, {nstep} steps", db,
+    return Workload(f"{nx}x{nx}x{nz} acoustic-over-viscoelastic, {nshots} shots, {nstep} steps", db,
                     shots[0][0], rec, nstep, dt, attenuation=att, shots=shots,
                     meta=dict(nx=nx, nz=nz, n_acoustic_layers=nac))
 

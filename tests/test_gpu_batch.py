@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def small_batch():
     from specfem3d_amd import workloads as W
-    return W.config3(scale=0.1, nshots=5, nrec=5, nstep_range=(60, 200), seed=11)
+    return W.gradient_shots_box(scale=0.1, nshots=5, nrec=5, nstep_range=(60, 200), seed=11)
 
 
 def single_shot(w, i, exact, assembly):
@@ -105,7 +105,7 @@ def test_autotuned_batch_matches_oracle(small_batch, oracle_mod):
 def visco_batch():
     from specfem3d_amd import workloads as W
     from specfem3d_amd.attenuation import setup_attenuation
-    w = W.config3(scale=0.1, nshots=5, nrec=5, nstep_range=(60, 160), seed=13)
+    w = W.gradient_shots_box(scale=0.1, nshots=5, nrec=5, nstep_range=(60, 160), seed=13)
     att = setup_attenuation(w.db, qmu=60.0, qkappa=9000.0)
     refs = []
     from specfem3d_amd.solver import GPUSolver

@@ -4,7 +4,7 @@
   (elastic regular/irregular, acoustic, fluid-solid coupling, 3-SLS attenuation, Stacey, sources,
   receivers);
 * the production mode (FMA + float atomics) stays within 1e-5 aggregate rel-L2 of the oracle;
-* config 1 (2000 steps) against the reference's FP32 and FP64 CPU seismograms (<= 1e-5);
+* the 20^3 homogeneous box (2000 steps) against the reference's FP32 and FP64 CPU seismograms (<= 1e-5);
 * the reference example homogeneous_halfspace_HEX8_elastic_absorbing_Stacey_5sides (5000 steps,
   irregular elements) against its REF_SEIS.
 """
@@ -58,7 +58,7 @@ def test_elastic_atomic_within_tolerance(oracle_mod, exact):
 @pytest.mark.parametrize("att", [False, True])
 def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     from specfem3d_amd import workloads as W
-    w = W.config2(scale=0.125, nstep=200, with_attenuation=att, nrec_side=3)
+    w = W.sea_over_visco_box(scale=0.125, nstep=200, with_attenuation=att, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
     out = run_gpu(w.db, w.sources, w.receivers, 200, True, 1, w.attenuation, shape=shape)
     assert np.array_equal(out, ref), rel_l2(out, ref)
@@ -73,7 +73,7 @@ def test_attenuation_schedules_bitwise(oracle_mod, att_variant, pipe_wgs):
     pipelined schedule 5 with one element per workgroup (default size at this mesh) and with
     7 or 100 persistent workgroups walking 864 elements (uneven walks)."""
     from specfem3d_amd import workloads as W
-    w = W.config2(scale=0.125, nstep=150, with_attenuation=True, nrec_side=3)
+    w = W.sea_over_visco_box(scale=0.125, nstep=150, with_attenuation=True, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 150, w.attenuation)
     out = run_gpu(w.db, w.sources, w.receivers, 150, True, 1, w.attenuation, shape=1, att_variant=att_variant,
                   pipe_wgs=pipe_wgs)
@@ -100,7 +100,7 @@ def test_pipelined_schedule_irregular_and_coloured(oracle_mod):
 def test_colored_assembly_deterministic_within_tolerance(oracle_mod, exact, shape):
     """Coloured scatter (no atomics): same answer on every run, within 1e-5 of the oracle."""
     from specfem3d_amd import workloads as W
-    w = W.config2(scale=0.125, nstep=200, with_attenuation=True, nrec_side=3)
+    w = W.sea_over_visco_box(scale=0.125, nstep=200, with_attenuation=True, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
     a = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation, shape=shape)
     b = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation, shape=shape)
@@ -108,13 +108,13 @@ def test_colored_assembly_deterministic_within_tolerance(oracle_mod, exact, shap
     assert rel_l2(a, ref) < 1e-5
 
 
-def test_config1_against_reference_fp32_fp64():
+def test_homogeneous_box_against_reference_fp32_fp64():
     from specfem3d_amd import workloads as W
-    ref = np.load(GOLDEN + "/cfg1_reference_seismograms.npz")
-    w = W.config1()
+    ref = np.load(GOLDEN + "/homogeneous_box_reference_seismograms.npz")
+    w = W.homogeneous_box()
     out = run_gpu(w.db, w.sources, w.receivers, 2000, False, 0)
     e32, e64 = rel_l2(out, ref["fp32"]), rel_l2(out, ref["fp64"])
-    print(f"[gpu] config1 2000 steps: rel-L2 vs reference FP32 {e32:.3e}, vs FP64 {e64:.3e}", flush=True)
+    print(f"[gpu] 20^3 homogeneous box, 2000 steps: rel-L2 vs reference FP32 {e32:.3e}, vs FP64 {e64:.3e}", flush=True)
     assert e32 < 1e-5 and e64 < 1e-5
 
 

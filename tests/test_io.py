@@ -1,7 +1,7 @@
-"""Seismogram / adjoint-source files against the reference's own output (config 1 run, FP32 CPU solver).
+"""Seismogram / adjoint-source files against the reference's own output (20^3 homogeneous box, FP32 CPU solver).
 
 The head of the reference's ``OUTPUT_FILES/DB.X01.HXX.semd`` is committed in tests/golden. Writing
-the reference's FP32 seismogram values (tests/golden/cfg1_reference_seismograms.npz) must reproduce
+the reference's FP32 seismogram values (tests/golden/homogeneous_box_reference_seismograms.npz) must reproduce
 its file name, its time column text exactly, and values that read back to the same float32 (the
 reference runtime's choice between two equally short digit strings is not reproduced everywhere).
 """
@@ -15,7 +15,7 @@ from conftest import GOLDEN
 def test_semd_files_match_reference_output(tmp_path):
     from specfem3d_amd import io
     from specfem3d_amd.sources import Station
-    ref = np.load(os.path.join(GOLDEN, "cfg1_reference_seismograms.npz"))
+    ref = np.load(os.path.join(GOLDEN, "homogeneous_box_reference_seismograms.npz"))
     seis = ref["fp32"]
     stations = [Station(f"X{i + 1:02d}", "DB", 0.0, 0.0, 0.0, 0.0) for i in range(seis.shape[1])]
     t0 = 1.2                         # Ricker f0 = 1 Hz: t0 = 1.2 / f0 (times are (it-1) DT - t0 in FP64)

@@ -6,8 +6,8 @@ receivers) against mathematics and against the reference solver's own output.
 * rigid translations produce no force, and the stiffness action is symmetric;
 * the Newmark scheme conserves the discrete energy of a closed box (no Stacey);
 * Stacey boundaries only remove energy;
-* config 1 (20^3 box, 2000 steps) reproduces the seismograms of the reference's FP32 and FP64
-  CPU builds (fixture tests/golden/cfg1_reference_seismograms.npz; first 400 steps here).
+* the 20^3 homogeneous box (2000 steps) reproduces the seismograms of the reference's FP32 and FP64
+  CPU builds (fixture tests/golden/homogeneous_box_reference_seismograms.npz; first 400 steps here).
 """
 import numpy as np
 import pytest
@@ -133,11 +133,11 @@ def test_stacey_only_removes_energy(oracle_mod):
     assert max(kin[-50:]) < 0.2 * max(kin)
 
 
-def test_config1_matches_reference_fp32_and_fp64(oracle_mod):
-    """The first 400 steps of config 1 against the reference's own FP32 / FP64 CPU seismograms."""
+def test_homogeneous_box_matches_reference_fp32_and_fp64(oracle_mod):
+    """The first 400 steps of the 20^3 homogeneous box against the reference's own FP32 / FP64 CPU seismograms."""
     from specfem3d_amd import workloads as W
-    ref = np.load(GOLDEN + "/cfg1_reference_seismograms.npz")
-    w = W.config1(nstep=400)
+    ref = np.load(GOLDEN + "/homogeneous_box_reference_seismograms.npz")
+    w = W.homogeneous_box(nstep=400)
     o = oracle_mod.OracleSolver(w.db, w.sources, w.receivers, "f32")
     s = o.run(400)
     e32 = rel_l2(s, ref["fp32"][:400])
