@@ -985,21 +985,22 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
 }
 
 // Tuning switches read by the step calls: "elastic_shape" (0 column-per-lane stiffness kernel,
-// 1 point-per-lane, the default), "att_variant" (0..5: attenuation-state scheduling of the
-// point kernel, same results; 4 is the default, 5 the persistent pipelined kernel), "pipe_wgs"
-// (workgroups of schedule 5, >= 1), "acoustic_kernel_field" (below). Unknown names are a fatal
-// error.
+// 1 point-per-lane, the default, 2 persistent pipelined point kernels), "att_variant" (0..5:
+// attenuation-state scheduling of the point kernel, same results; 4 is the default, 5 the
+// persistent pipelined kernel), "pipe_wgs" (workgroups of the pipelined kernels, 0 = their
+// default: 2048 with attenuation, 2560 without), "acoustic_kernel_field" (below). Unknown names
+// are a fatal error.
 extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
   if (std::strcmp(name, "elastic_shape") == 0) {
-    if (value < 0 || value > 1) fatal("sf_set_tuning", "elastic_shape must be 0 or 1");
+    if (value < 0 || value > 2) fatal("sf_set_tuning", "elastic_shape must be 0, 1 or 2");
     m->elastic_shape = value;
   } else if (std::strcmp(name, "att_variant") == 0) {
     if (value < 0 || value > 5) fatal("sf_set_tuning", "att_variant must be 0..5");
     m->att_variant = value;
   } else if (std::strcmp(name, "pipe_wgs") == 0) {
-    if (value < 1) fatal("sf_set_tuning", "pipe_wgs must be >= 1");
+    if (value < 0) fatal("sf_set_tuning", "pipe_wgs must be >= 0");
     m->pipe_wgs = value;
   } else if (std::strcmp(name, "acoustic_kernel_field") == 0) {
     // which adjoint array compute_kernels_acoustic_cuda pairs with the reconstructed field:

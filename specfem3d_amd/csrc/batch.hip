@@ -31,6 +31,7 @@ void ensure_element_colors(Mesh* m);
 struct Batch {
   Mesh* m = nullptr;
   int nshots = 0, S = 0;
+  int S0 = 0;                    // padded shot count at creation; S shrinks as shots finish
   int sc = 4, mode = 2;          // shots per workgroup, assembly (0 atomic, 2 coloured, 3 fused)
   int overlay = 0;               // 1: displacement tile and stress products share LDS; 2: pipelined kernel
   int pipe_wgs = 0;              // persistent workgroups of the pipelined kernel (0: its default)
@@ -42,6 +43,8 @@ struct Batch {
   int* d_it = nullptr;
   int h_it = 1;                  // host mirror of the step counter
   realw *u = nullptr, *v = nullptr, *a = nullptr;
+  realw* u_last = nullptr;       // final displacement of slots dropped by compaction (stride S0)
+  std::vector<int> h_nstep;      // per slot (S0 entries, decreasing, padding 0)
   bool att = false;              // 3-SLS attenuation: per-shot memory variables and strain
   realw *R = nullptr, *eps = nullptr;
   unsigned char* flags = nullptr;  // fused mode: BF_* per local point (first/last colour, cleared)
@@ -212,6 +215,51 @@ __global__ void k_batch_fill(int64_t n, realw* p) {
   if (t < n) p[t] = 0.f;
 }
 
+// Compaction of shot-interleaved rows (grid-stride loops over int64 counts; launched with at most
+// 2^20 workgroups): dst[r * S2 + s] = src[r * S + s] for s < S2 (S2 <= S, multiples of 8; float4)
+__global__ void k_batch_restride(int64_t nrow, int S, int S2, const realw* __restrict__ src,
+                                 realw* __restrict__ dst) {
+  const int nq = S2 >> 2;
+  const int64_t total = nrow * nq;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / nq;
+    const int k = (int)(t - r * nq);
+    *reinterpret_cast<float4*>(dst + r * S2 + 4 * k) = *reinterpret_cast<const float4*>(src + r * S + 4 * k);
+  }
+}
+
+// per-(element, shot) blocks of `blk` values: dst block (e, s) at (e * S2 + s) * blk = src block at
+// (e * S + s) * blk, for s < S2
+__global__ void k_batch_restride_blocks(int64_t nspec, int S, int S2, int blk, const realw* __restrict__ src,
+                                        realw* __restrict__ dst) {
+  const int64_t per_e = (int64_t)S2 * blk;
+  const int64_t total = nspec * per_e;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = t / per_e;
+    const int64_t w = t - e * per_e;  // s * blk + j
+    dst[t] = src[e * (int64_t)S * blk + w];
+  }
+}
+
+// final displacement of the slots [S2, S) that compaction drops: arch[r * S0 + s] = u[r * S + s]
+__global__ void k_batch_archive(int64_t nrow, int S, int S2, int S0, const realw* __restrict__ u,
+                                realw* __restrict__ arch) {
+  const int nd = S - S2;
+  const int64_t total = nrow * nd;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / nd;
+    const int s = S2 + (int)(t - r * nd);
+    arch[r * S0 + s] = u[r * S + s];
+  }
+}
+
+static dim3 stride_grid(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > (1 << 20)) b = 1 << 20;
+  return dim3((unsigned)b);
+}
+
 // ------------------------------------------------------------------------------ host side
 template <typename T>
 static T* balloc(int64_t n) {
@@ -238,7 +286,71 @@ static void drop_graph(Batch* b) {
   b->graph_steps = 0;
 }
 
+// Give the live shots (a prefix of the slots, which are sorted by decreasing record length) rows
+// of their own width once at least 8 slots have finished: with S = 32 and 8 live shots, every row
+// read would fetch four times the bytes it uses. Slots [S2, S) are finished; their final
+// displacement is kept for sf_batch_displ. Kernel arguments change, so captured graphs are dropped.
+static void compact(Batch* b, int S2) {
+  Mesh* m = b->m;
+  hipStream_t st = m->stream;
+  drop_graph(b);
+  const int64_t nrow = 3 * (int64_t)m->nglob;
+  if (!b->u_last) b->u_last = balloc<realw>(nrow * b->S0);
+  hipLaunchKernelGGL(k_batch_archive, stride_grid(nrow * (b->S - S2)), dim3(256), 0, st, nrow, b->S, S2, b->S0,
+                     b->u, b->u_last);
+  SF_LAUNCH_CHECK("k_batch_archive");
+  for (realw** f : {&b->u, &b->v, &b->a}) {
+    realw* d = balloc<realw>(nrow * S2);
+    hipLaunchKernelGGL(k_batch_restride, stride_grid(nrow * (S2 / 4)), dim3(256), 0, st, nrow, b->S, S2, *f, d);
+    SF_LAUNCH_CHECK("k_batch_restride");
+    SF_CHECK(hipStreamSynchronize(st));
+    bfree(*f);
+    *f = d;
+  }
+  if (b->att) {
+    const int64_t ne = m->nspec;
+    for (auto fb : {std::make_pair(&b->R, 18 * 125), std::make_pair(&b->eps, 6 * 125)}) {
+      realw* d = balloc<realw>(ne * S2 * fb.second);
+      hipLaunchKernelGGL(k_batch_restride_blocks, stride_grid(ne * S2 * fb.second), dim3(256), 0, st, ne, b->S, S2,
+                         fb.second, *fb.first, d);
+      SF_LAUNCH_CHECK("k_batch_restride_blocks");
+      SF_CHECK(hipStreamSynchronize(st));
+      bfree(*fb.first);
+      *fb.first = d;
+    }
+  }
+  b->S = S2;
+}
+
+static void maybe_compact(Batch* b) {
+  int nlive = 0;
+  for (int s = 0; s < b->S; s++) nlive += b->h_nstep[s] >= b->h_it ? 1 : 0;
+  const int S2 = std::max(8, (nlive + 7) / 8 * 8);
+  if (S2 < b->S) compact(b, S2);
+}
+
+// back to rows of the full width (a rewind after compaction); the contents are zeroed next
+static void restore_width(Batch* b) {
+  if (b->S == b->S0) return;
+  Mesh* m = b->m;
+  drop_graph(b);
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  const int64_t n = 3 * (int64_t)m->nglob * b->S0;
+  for (realw** f : {&b->u, &b->v, &b->a}) {
+    bfree(*f);
+    *f = balloc<realw>(n);
+  }
+  if (b->att) {
+    bfree(b->R);
+    bfree(b->eps);
+    b->R = balloc<realw>(18 * (int64_t)m->nspec * 125 * b->S0);
+    b->eps = balloc<realw>(6 * (int64_t)m->nspec * 125 * b->S0);
+  }
+  b->S = b->S0;
+}
+
 static void zero_state(Batch* b) {
+  restore_width(b);
   Mesh* m = b->m;
   const int64_t n = 3 * (int64_t)m->nglob * b->S;
   for (realw* p : {b->u, b->v, b->a})
@@ -431,6 +543,7 @@ static void run_steps(Batch* b, int n, int graph_steps) {
     build_point_flags(b);
   }
   while (n > 0) {
+    maybe_compact(b);
     if (b->h_it == 1 || graph_steps <= 1 || n < graph_steps) {
       enqueue_step(b, m->stream, b->h_it == 1);
       b->h_it++;
@@ -493,6 +606,8 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
     ns[k] = nstep[order[k]];
   }
   b->d_nstep = bupload(ns.data(), b->S);
+  b->h_nstep = ns;
+  b->S0 = b->S;
   b->d_it = balloc<int>(1);
   const int64_t n = 3 * (int64_t)m->nglob * b->S;
   b->u = balloc<realw>(n);
@@ -700,10 +815,12 @@ extern "C" void sf_batch_displ(long h, int shot, realw* out) {
   if (shot < 0 || shot >= b->nshots) fatal("sf_batch_displ", "shot index out of range");
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   const int64_t n = 3 * (int64_t)b->m->nglob;
-  std::vector<realw> all((size_t)n * b->S);
-  SF_CHECK(hipMemcpy(all.data(), b->u, sizeof(realw) * all.size(), hipMemcpyDeviceToHost));
   const int s = b->slot_of[shot];
-  for (int64_t q = 0; q < n; q++) out[q] = all[(size_t)q * b->S + s];
+  const bool dropped = s >= b->S;  // finished and compacted away: its final state was archived
+  const int stride = dropped ? b->S0 : b->S;
+  std::vector<realw> all((size_t)n * stride);
+  SF_CHECK(hipMemcpy(all.data(), dropped ? b->u_last : b->u, sizeof(realw) * all.size(), hipMemcpyDeviceToHost));
+  for (int64_t q = 0; q < n; q++) out[q] = all[(size_t)q * stride + s];
 }
 
 // Rewind to step 1 with zero wavefields and seismograms (keeps sources, receivers and options).
@@ -792,6 +909,7 @@ extern "C" void sf_batch_destroy(long h) {
   SF_CHECK(hipStreamSynchronize(b->m->stream));
   drop_graph(b);
   bfree(b->d_nstep); bfree(b->d_it); bfree(b->u); bfree(b->v); bfree(b->a); bfree(b->R); bfree(b->eps);
+  bfree(b->u_last);
   bfree(b->flags);
   bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
   bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);

@@ -786,6 +786,129 @@ k_elastic_pipe(ElasticArgs A, GLLk G) {
   }
 }
 
+// Elastic (no attenuation) counterpart of k_elastic_pipe ("elastic_shape" 2): persistent
+// workgroups walk ie, ie + NG, ... and load the next element's displacement, moduli and Jacobian
+// index (and the ibool row of the one after it, and the list entry after that) while computing
+// this one. Without memory variables the point kernel moves few bytes per element and waits on the
+// dependent ibool -> displacement loads at the start of every element; here they are a whole
+// element ahead. Per element the arithmetic and its order are k_elastic_pt's (same bits).
+template <bool STRAIN, int MODE>
+__global__ void __launch_bounds__(128) k_elastic_pipe_el(ElasticArgs A, GLLk G) {
+  __shared__ realw sU[3][128];   // u of the current element
+  __shared__ realw sT[9][128];   // stress products
+  __shared__ realw sG[75];
+  const int p = threadIdx.x;
+  const bool active = p < 125;
+  stage_gll(sG, G, p);
+  const int q = active ? p : 124;
+  const int k = q / 25, j = (q / 5) % 5, i = q % 5;
+  const int64_t nl = A.nloc;
+  const int NG = gridDim.x;
+  const int nelem = A.nelem;
+  int ie = xcd_remap(blockIdx.x, NG);
+  if (ie >= nelem) return;  // the launcher keeps NG <= nelem
+  int ie_v = ie;
+  asm volatile("" : "+v"(ie_v));
+  int e = A.elems[ie_v];
+  int e_n = ie + NG < nelem ? A.elems[ie_v + NG] : e;
+  int e_nn = ie + 2 * NG < nelem ? A.elems[ie_v + 2 * NG] : e_n;
+  int64_t ep = (int64_t)e * 125 + q;
+  int g = A.ibool[ep];
+  int g_n = A.ibool[(int64_t)e_n * 125 + q];
+  {
+    const realw* d = A.displ + 3 * (int64_t)g;
+    const realw u0 = d[0], u1 = d[1], u2 = d[2];
+    sU[0][p] = u0; sU[1][p] = u1; sU[2][p] = u2;
+  }
+  float2 km = A.kappamu[ep];
+  int irr = A.irregular[e];
+  asm volatile("" : "+v"(irr));
+  lds_barrier();
+
+  const realw ONE_THIRD = (realw)1 / (realw)3;
+  for (;;) {
+    const bool has_next = ie + NG < nelem;  // uniform over the workgroup
+    const int64_t ep_n = (int64_t)e_n * 125 + q;
+    const int g_nn = A.ibool[(int64_t)e_nn * 125 + q];
+    int ie3 = ie + 3 * NG < nelem ? ie + 3 * NG : ie;
+    asm volatile("" : "+v"(ie3));
+    const int e_nnn = ie + 3 * NG < nelem ? A.elems[ie3] : e_nn;
+    realw un0, un1, un2;
+    {
+      const realw* d = A.displ + 3 * (int64_t)g_n;
+      un0 = d[0]; un1 = d[1]; un2 = d[2];
+    }
+    const float2 km_n = A.kappamu[ep_n];
+    int en_v = e_n;
+    asm volatile("" : "+v"(en_v));
+    const int irr_n = A.irregular[en_v];
+    JacPt J = {};
+    realw jacl = A.jacobian_regular;
+    if (irr) {
+      J = load_jac(A.jac, A.jac_plane, (int64_t)(irr - 1) * 125 + q);
+      jacl = J.jac;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+
+    const Grad9 tg = tile_gradient<1>(sU[0], sU[1], sU[2], sG, i, j, k);
+    const Du9 du = map_gradient(tg, irr != 0, J, A.xix_regular);
+    if (STRAIN) {
+      const Du9& st = du;
+      const realw templ = ONE_THIRD * (st.xx + st.yy + st.zz);
+      if (A.eps_tr3 && active) A.eps_tr3[ep] = templ;
+      if (active) {
+        realw* Ep = A.eps;
+        Ep[eps_idx(e, 0, q)] = st.xx - templ;
+        Ep[eps_idx(e, 1, q)] = st.yy - templ;
+        Ep[eps_idx(e, 2, q)] = (realw)0.5 * (st.xy + st.yx);
+        Ep[eps_idx(e, 3, q)] = (realw)0.5 * (st.zx + st.xz);
+        Ep[eps_idx(e, 4, q)] = (realw)0.5 * (st.zy + st.yz);
+      }
+    }
+    const Sigma6 sg = iso_stress(du, km.x, km.y);
+    {
+      realw t[9];
+      stress_products(sg, irr != 0, J, jacl, A.xix_regular, t);
+#pragma unroll
+      for (int m = 0; m < 9; m++) sT[m][p] = t[m];
+    }
+    lds_barrier();  // products complete; every lane is past this element's u tile
+    if (has_next) {
+      sU[0][p] = un0; sU[1][p] = un1; sU[2][p] = un2;
+    }
+    if (active) {
+      realw f[3];
+      tile_divergence<1>(&sT[0][0], 128, sG + 25, sG + 50, i, j, k, f);
+      if (MODE == 1) {
+        A.local_out[ep] = f[0];
+        A.local_out[nl + ep] = f[1];
+        A.local_out[2 * nl + ep] = f[2];
+      } else if (MODE == 2) {
+        realw* a = A.accel + 3 * (int64_t)g;
+        a[0] = a[0] + (-f[0]);
+        a[1] = a[1] + (-f[1]);
+        a[2] = a[2] + (-f[2]);
+      } else {
+        realw* a = A.accel + 3 * (int64_t)g;
+        atomicAdd(a + 0, -f[0]);
+        atomicAdd(a + 1, -f[1]);
+        atomicAdd(a + 2, -f[2]);
+      }
+    }
+    lds_barrier();  // products read; the next element's tile is staged
+    if (!has_next) break;
+    ie += NG;
+    e = e_n;
+    e_n = e_nn;
+    e_nn = e_nnn;
+    ep = ep_n;
+    g = g_n;
+    g_n = g_nn;
+    km = km_n;
+    irr = irr_n;
+  }
+}
+
 // element strain of a displacement field (compute_element_strain.F90:28-130); one lane per point
 __global__ void __launch_bounds__(128) k_element_strain(ElasticArgs A, GLLk G, realw* eps, realw* eps_tr3,
                                                         int with_trace) {
@@ -871,13 +994,31 @@ void SFX(launch_elastic)(hipStream_t s, const ElasticArgs& a, bool strain, bool 
                          int att_variant) {
   if (a.nelem <= 0) return;
   const GLLk G = g_gll_host;
-  if (shape == 1 && att && att_variant == 5) {
-    const int ng = a.pipe_wgs > 0 && a.pipe_wgs < a.nelem ? a.pipe_wgs : a.nelem;
+  if ((shape == 1 && att && att_variant == 5) || (shape == 2 && att)) {
+    const int want = a.pipe_wgs > 0 ? a.pipe_wgs : 2048;
+    const int ng = want < a.nelem ? want : a.nelem;
     dim3 grid(ng), block(128);
     if (mode == 1) hipLaunchKernelGGL((k_elastic_pipe<1>), grid, block, 0, s, a, G);
     else if (mode == 2) hipLaunchKernelGGL((k_elastic_pipe<2>), grid, block, 0, s, a, G);
     else hipLaunchKernelGGL((k_elastic_pipe<0>), grid, block, 0, s, a, G);
     SF_LAUNCH_CHECK("k_elastic_pipe");
+    return;
+  }
+  if (shape == 2) {
+    // elastic: ten workgroups per CU (90-95 VGPRs, 5 waves per SIMD)
+    const int want = a.pipe_wgs > 0 ? a.pipe_wgs : 2560;
+    const int ng = want < a.nelem ? want : a.nelem;
+    dim3 grid(ng), block(128);
+#define SF_KE(S, O) hipLaunchKernelGGL((k_elastic_pipe_el<S, O>), grid, block, 0, s, a, G)
+    if (mode == 1) {
+      if (strain) SF_KE(true, 1); else SF_KE(false, 1);
+    } else if (mode == 2) {
+      if (strain) SF_KE(true, 2); else SF_KE(false, 2);
+    } else {
+      if (strain) SF_KE(true, 0); else SF_KE(false, 0);
+    }
+#undef SF_KE
+    SF_LAUNCH_CHECK("k_elastic_pipe_el");
     return;
   }
   if (shape == 1) {

@@ -8,7 +8,10 @@
 * fast kernels: each shot within 1e-5 rel-L2 of the FP32 oracle, for every autotuner candidate;
 * uneven record lengths: a shot stops at its own length (its final displacement equals the
   single-shot run's);
-* 3-SLS viscoelastic batches: every shot bit-identical to its single-shot viscoelastic run.
+* 3-SLS viscoelastic batches: every shot bit-identical to its single-shot viscoelastic run;
+* compaction: once at least 8 shots have finished, the live shots get narrower rows; with 12
+  shots (rows of 16) every shot still equals its single-shot run, elastic and viscoelastic,
+  eager and graph-replayed, and finished shots keep their final displacement.
 """
 import numpy as np
 import pytest
@@ -130,4 +133,48 @@ def test_visco_batch_bitwise_equal_single_shot_runs(visco_batch, assembly, sc):
         assert np.abs(ref).max() > 0
         assert np.array_equal(out, ref), (i, rel_l2(out, ref))
         assert np.array_equal(b.displ(i), uref), i
+    b.close()
+
+
+@pytest.fixture(scope="module")
+def twelve_shots():
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.attenuation import setup_attenuation
+    from specfem3d_amd.solver import GPUSolver
+    w = W.gradient_shots_box(scale=0.1, nshots=12, nrec=3, nstep_range=(30, 150), seed=21)
+    att = setup_attenuation(w.db, qmu=60.0, qkappa=9000.0)
+    refs = {}
+    for key, a in (("elastic", None), ("visco", att)):
+        out = []
+        for src, rec, n in w.shots:
+            g = GPUSolver(w.db, src, rec, nstep=n, attenuation=a, exact=True, assembly=2, element_order="morton")
+            s = g.run().copy()
+            u, _, _ = g.fields()
+            g.close()
+            out.append((s, u))
+        refs[key] = out
+    return w, att, refs
+
+
+@pytest.mark.parametrize("kind,assembly,graph_steps", [("elastic", 3, 1), ("elastic", 3, 10), ("elastic", 2, 10),
+                                                       ("visco", 3, 10)])
+def test_compaction_bitwise_equal_single_shot_runs(twelve_shots, kind, assembly, graph_steps):
+    from specfem3d_amd.batch import BatchSolver
+    w, att, refs = twelve_shots
+    ns = sorted((s[2] for s in w.shots), reverse=True)
+    assert ns[8] < ns[0]  # the live shots drop to 8 or fewer before the longest record ends
+    print(f"[gpu] 12-shot {kind} batch, assembly {assembly}, graph steps {graph_steps}, nstep {ns}", flush=True)
+    b = BatchSolver(w.db, w.shots, exact=True, shots_per_group=8, assembly=assembly,
+                    attenuation=att if kind == "visco" else None)
+    b.run(graph_steps=graph_steps)
+    for i, (ref, uref) in enumerate(refs[kind]):
+        out = b.seismograms(i)
+        assert np.abs(ref).max() > 0
+        assert np.array_equal(out, ref), (i, rel_l2(out, ref))
+        assert np.array_equal(b.displ(i), uref), i
+    # a rewind restores the full rows: the second run reproduces the first
+    b.reset()
+    b.run(graph_steps=graph_steps)
+    for i, (ref, _) in enumerate(refs[kind]):
+        assert np.array_equal(b.seismograms(i), ref), i
     b.close()

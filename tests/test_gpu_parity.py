@@ -36,14 +36,40 @@ def run_oracle(oracle_mod, db, src, rec, nstep, attenuation=None):
     return oracle_mod.OracleSolver(db, src, rec, "f32", attenuation=attenuation).run(nstep)
 
 
-@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("shape,pipe_wgs", [(0, None), (1, None), (2, None), (2, 5)])
 @pytest.mark.parametrize("warp", [False, True])
-def test_elastic_exact_ordered_bitwise(oracle_mod, warp, shape):
+def test_elastic_exact_ordered_bitwise(oracle_mod, warp, shape, pipe_wgs):
+    """Every stiffness kernel shape equals the oracle bit for bit (exact build, CPU element order);
+    shape 2 with one element per workgroup (default size at 64 elements) and with 5 persistent
+    workgroups walking the elements."""
     db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0) if warp else None)
     ref = run_oracle(oracle_mod, db, src, rec, 200)
-    out = run_gpu(db, src, rec, 200, True, 1, shape=shape)
+    out = run_gpu(db, src, rec, 200, True, 1, shape=shape, pipe_wgs=pipe_wgs)
     assert np.abs(ref).max() > 0
     assert np.array_equal(out, ref), rel_l2(out, ref)
+
+
+def test_pipelined_elastic_coloured_and_strain():
+    """Shape 2 (pipelined, no attenuation) with coloured assembly equals shape 1's coloured run bit
+    for bit; with 3 persistent workgroups, the strain-storing variant of a kernel run (the
+    adjoint's forward) gives shape 1's kernels."""
+    from specfem3d_amd import adjoint as A
+    db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0))
+    c1 = run_gpu(db, src, rec, 150, True, 2, shape=1)
+    c2 = run_gpu(db, src, rec, 150, True, 2, shape=2, pipe_wgs=3)
+    assert np.array_equal(c2, c1), rel_l2(c2, c1)
+    out = []
+    for shape, pw in ((1, None), (2, 3)):
+        tuning = {"elastic_shape": shape}
+        if pw is not None:
+            tuning["pipe_wgs"] = pw
+        print(f"[gpu] kernel run, shape {shape}, pipe_wgs {pw}", flush=True)
+        ks = A.KernelSolver(db, src, rec, 120, nt_dump=16, exact=True, assembly=1, tuning=tuning)
+        syn = ks.forward()
+        out.append((syn,) + tuple(ks.adjoint(A.waveform_adjoint_source(syn, 0.5 * syn))))
+        ks.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b), rel_l2(a, b)
 
 
 @pytest.mark.parametrize("exact", [True, False])
