@@ -188,6 +188,8 @@ def batched_measurement(args) -> dict:
     return {"workload": w.name, "value": value, "unit": "GLL-point updates/s", "wall_s": round(wall, 3),
             "shot_steps": int(sum(nsteps)), "plain_single_shot": plain, "plain_assembly": "coloured",
             "ratio_vs_plain": value / plain,
+            "all_live_value": db.nglob * len(w.shots) / (sms / 1e3),
+            "all_live_ratio_vs_plain": db.nglob * len(w.shots) / (sms / 1e3) / plain,
             "autotuned": choice, "seismograms_finite": finite,
             "run_roofline": {"bound": "hbm", "achieved": round(run_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(run_gbs / HBM_PEAK_GBS, 4),
