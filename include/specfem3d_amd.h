@@ -283,7 +283,8 @@ void sf_batch_finalize(long batch);
 void sf_batch_set_options(long batch, int shots_per_group, int assembly);
 void sf_batch_get_options(long batch, int* shots_per_group, int* assembly);
 /* kernel variants with the same results: "lds_overlay" 0/1 (displacement tile and stress products
-   share LDS), 2 (persistent pipelined kernel; elastic shots, coloured assembly); "pipe_wgs" >= 0
+   share LDS), 2 (persistent pipelined kernel), 3 (shot-packed tiles: one 16-byte LDS read serves
+   four shots; 2 and 3 for elastic shots with coloured assembly); "pipe_wgs" >= 0
    (workgroups of the pipelined kernel, 0 = its default); unknown names are fatal */
 void sf_batch_set_tuning(long batch, const char* name, int value);
 int sf_batch_get_tuning(long batch, const char* name);

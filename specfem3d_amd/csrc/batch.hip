@@ -477,7 +477,7 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
   } else {
     A.elems = m->elastic_elems;
     A.nelem = m->nspec_elastic;
-    launch_elastic_batch(st, A, b->sc, 0, b->overlay == 2 ? 1 : b->overlay, m->exact != 0);  // atomic: no pipeline
+    launch_elastic_batch(st, A, b->sc, 0, b->overlay >= 2 ? 1 : b->overlay, m->exact != 0);  // atomic: plain kernel
   }
 }
 
@@ -745,14 +745,15 @@ extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) 
 }
 
 // Kernel variants that give the same results: "lds_overlay" 0/1 (the displacement tile shares LDS
-// with the stress products), 2 (persistent pipelined kernel; coloured elastic assembly only);
+// with the stress products), 2 (persistent pipelined kernel), 3 (shot-packed tiles; 2 and 3 for
+// coloured elastic assembly only);
 // "pipe_wgs" (workgroups of the pipelined kernel, 0 = its default). Unknown names are fatal.
 extern "C" void sf_batch_set_tuning(long h, const char* name, int value) {
   Batch* b = batch_of(h);
   if (std::strcmp(name, "lds_overlay") == 0) {
-    if (value < 0 || value > 2) fatal("sf_batch_set_tuning", "lds_overlay must be 0, 1 or 2");
-    if (value == 2 && (b->att || b->mode == 0))
-      fatal("sf_batch_set_tuning", "the pipelined kernel needs elastic shots and coloured assembly");
+    if (value < 0 || value > 3) fatal("sf_batch_set_tuning", "lds_overlay must be 0..3");
+    if (value >= 2 && (b->att || b->mode == 0))
+      fatal("sf_batch_set_tuning", "the pipelined and shot-packed kernels need elastic shots and coloured assembly");
     SF_CHECK(hipStreamSynchronize(b->m->stream));
     drop_graph(b);
     b->overlay = value;
@@ -875,10 +876,10 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
   int best_sc = b->sc, best_mode = b->mode, best_ov = b->overlay;
   for (int mode : {3, 2, 0})
     for (int sc : {4, 8})
-    for (int ov : {2, 1, 0}) {
+    for (int ov : {3, 2, 1, 0}) {
       if (b->S % sc) continue;
       if (b->att && (mode == 0 || ov != 1)) continue;  // coloured only; no overlay switch
-      if (ov == 2 && mode == 0) continue;             // the pipelined kernel is coloured only
+      if (ov >= 2 && mode == 0) continue;             // the pipelined and packed kernels are coloured only
       b->sc = sc;
       b->mode = mode;
       b->overlay = ov;

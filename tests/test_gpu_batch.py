@@ -46,7 +46,9 @@ def single_runs(small_batch):
     (2, 4, 0, 0), (3, 4, 0, 0), (3, 8, 0, 0), (3, 4, 1, 0), (3, 8, 1, 0),
     # persistent pipelined kernel: one pair per workgroup (default size at this mesh) and walks
     # of several (element, shot group) pairs per workgroup, uneven over the workgroups
-    (3, 8, 2, 0), (3, 8, 2, 3), (3, 4, 2, 2), (2, 4, 2, 5)])
+    (3, 8, 2, 0), (3, 8, 2, 3), (3, 4, 2, 2), (2, 4, 2, 5),
+    # shot-packed tiles
+    (3, 8, 3, 0), (3, 4, 3, 0), (2, 8, 3, 0)])
 def test_batch_shots_bitwise_equal_single_shot_runs(small_batch, single_runs, assembly, sc, overlay, pipe_wgs):
     from specfem3d_amd.batch import BatchSolver
     w = small_batch
@@ -65,7 +67,7 @@ def test_batch_shots_bitwise_equal_single_shot_runs(small_batch, single_runs, as
     b.close()
 
 
-@pytest.mark.parametrize("assembly,overlay", [(2, 1), (3, 1), (3, 2)])
+@pytest.mark.parametrize("assembly,overlay", [(2, 1), (3, 1), (3, 2), (3, 3)])
 def test_graph_replay_equals_eager_steps(small_batch, assembly, overlay):
     from specfem3d_amd.batch import BatchSolver
     w = small_batch
@@ -94,7 +96,7 @@ def test_autotuned_batch_matches_oracle(small_batch, oracle_mod):
     b = BatchSolver(w.db, w.shots, exact=False)
     choice = b.autotune(trial_steps=3)
     print(f"[gpu] autotuner choice {choice}", flush=True)
-    assert choice["shots_per_group"] in (4, 8) and choice["assembly"] in (0, 2, 3) and choice["lds_overlay"] in (0, 1, 2)
+    assert choice["shots_per_group"] in (4, 8) and choice["assembly"] in (0, 2, 3) and choice["lds_overlay"] in (0, 1, 2, 3)
     b.run(graph_steps=10)
     for i, (src, rec, n) in enumerate(w.shots):
         ref = oracle_mod.OracleSolver(w.db, src, rec, "f32").run(n)
