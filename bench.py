@@ -378,9 +378,33 @@ def main():
                     help="steps of one shot of the 160x160x80 viscoelastic workload (0: skip; the mesh build "
                          "takes minutes)")
     ap.add_argument("--large-scale", type=float, default=1.0, help="shrink the large mesh (tests only)")
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="only run a short fixed sequence of batched and single-shot steps (PMC counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+
+    if args.kernel_probe:
+        # short fixed sequence for PMC counter runs: 4 batched steps of the 32-shot workload with
+        # each batched kernel variant, then 6 single-shot steps with coloured assembly
+        from specfem3d_amd import workloads as W
+        from specfem3d_amd.batch import BatchSolver
+        from specfem3d_amd.solver import GPUSolver
+        w = W.gradient_shots_box(scale=args.batch_scale, nshots=32)
+        w.shots = [(s_, r_, min(n_, 40)) for s_, r_, n_ in w.shots]
+        b = BatchSolver(w.db, w.shots, exact=False, shots_per_group=8, assembly=3)
+        for ov in (1, 3):
+            b.set_tuning("lds_overlay", ov)
+            b.reset()
+            b.run(4, graph_steps=1)
+            log(f"[bench] kernel probe: 4 batched steps, lds_overlay {ov}")
+        b.close()
+        src, rec, _ = w.shots[0]
+        g = GPUSolver(w.db, src, rec, nstep=6, exact=False, assembly=2)
+        g.run()
+        g.close()
+        log("[bench] kernel probe: 6 single-shot steps, coloured assembly")
+        return
 
     if args.copy_probe:
         import torch
