@@ -354,6 +354,9 @@ def main():
     ap.add_argument("--element-order", default="none",
                     choices=["morton", "none", "brick1x2", "brick2x1", "brick2x2", "brick1x4", "brick4x4", "brick2x4"],
                     help="launch order of the elastic elements of the single-shot run")
+    ap.add_argument("--elastic-shape", type=int, default=-1,
+                    help="single-shot stiffness kernel: 0 column-per-lane, 1 point-per-lane, 2 pipelined "
+                         "(-1: library default, 1)")
     ap.add_argument("--att-variant", type=int, default=-1,
                     help="attenuation schedule of the single-shot stiffness kernel (-1: library default)")
     ap.add_argument("--pipe-wgs", type=int, default=0, help="persistent workgroups of schedule 5 (0: default)")
@@ -442,6 +445,8 @@ def main():
     log(f"[bench] built {w.name}: {db.summary()} in {time.time() - t0:.1f}s")
 
     tuning = {}
+    if args.elastic_shape >= 0:
+        tuning["elastic_shape"] = args.elastic_shape
     if args.att_variant >= 0:
         tuning["att_variant"] = args.att_variant
     if args.pipe_wgs > 0:
