@@ -80,7 +80,7 @@ def test_elastic_atomic_within_tolerance(oracle_mod, exact):
     assert rel_l2(out, ref) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("att", [False, True])
 def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     from specfem3d_amd import workloads as W
@@ -121,7 +121,7 @@ def test_pipelined_schedule_irregular_and_coloured(oracle_mod):
     assert np.array_equal(c5, c4), rel_l2(c5, c4)
 
 
-@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("exact", [True, False])
 def test_colored_assembly_deterministic_within_tolerance(oracle_mod, exact, shape):
     """Coloured scatter (no atomics): same answer on every run, within 1e-5 of the oracle."""
