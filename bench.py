@@ -496,14 +496,25 @@ def main():
     if not args.no_cpu_baseline and args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import oracle
-        ora = oracle.OracleSolver(db, w.sources, w.receivers, "f32", attenuation=w.attenuation)
+        # the host's cores (16 on the GPU box, OMP_NUM_THREADS there), capped at 16
+        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)))
+        ora = oracle.OracleSolver(db, w.sources, w.receivers, "f32", attenuation=w.attenuation, threads=threads)
+        nthr = args.cpu_steps * 4 if threads > 1 else args.cpu_steps
         tc = time.perf_counter()
-        for _ in range(args.cpu_steps):
+        for _ in range(nthr):
             ora.step()
         ct = time.perf_counter() - tc
-        cpu = {"value": nglob * args.cpu_steps / ct, "unit": "GLL-point updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} time steps of the same {w.name} mesh (FP32 C oracle, 1 thread)"}
-        log(f"[bench] CPU oracle {args.cpu_steps} steps in {ct:.1f}s")
+        cpu = {"value": nglob * nthr / ct, "unit": "GLL-point updates/s", "cores": threads, "kind": "port",
+               "sample": f"{nthr} time steps of the same {w.name} mesh (FP32 C oracle, {threads} OpenMP threads)"}
+        log(f"[bench] CPU oracle {nthr} steps on {threads} threads in {ct:.1f}s")
+        if threads > 1:
+            ora1 = oracle.OracleSolver(db, w.sources, w.receivers, "f32", attenuation=w.attenuation)
+            tc = time.perf_counter()
+            for _ in range(args.cpu_steps // 2 or 1):
+                ora1.step()
+            c1 = time.perf_counter() - tc
+            cpu["single_thread_value"] = nglob * (args.cpu_steps // 2 or 1) / c1
+            log(f"[bench] CPU oracle {args.cpu_steps // 2 or 1} steps on 1 thread in {c1:.1f}s")
 
     adj = adjoint_measurement(args) if args.adjoint_steps > 0 else None
     batched = batched_measurement(args) if args.batch_shots > 0 else None

@@ -50,6 +50,22 @@ def lib():
     return _LIB
 
 
+_LIB_OMP = None
+
+
+def lib_omp():
+    """The multi-threaded FP32 build (element loops in parallel, atomic scatter: the summation
+    order at shared points varies), for the CPU baseline only; parity checks use ``lib()``."""
+    global _LIB_OMP
+    if _LIB_OMP is None:
+        build_oracle()
+        path = os.path.join(_HERE, "liboracle_omp.so")
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", _HERE], check=True)
+        _LIB_OMP = C.CDLL(path)
+    return _LIB_OMP
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
@@ -67,12 +83,19 @@ class OracleSolver:
     """Forward (and adjoint-with-stored-forward) time stepping on the CPU."""
 
     def __init__(self, db, sources=None, receivers=None, precision: str = "f32", attenuation=None,
-                 compute_strain: bool = False, store_eps_trace3: bool = False):
+                 compute_strain: bool = False, store_eps_trace3: bool = False, threads: int = 1):
+        """``threads`` > 1 runs the multi-threaded FP32 build (CPU baseline timing only)."""
         self.db = db
         self.prec = precision
         self.dt_np = np.float32 if precision == "f32" else np.float64
         self.sfx = "_f32" if precision == "f32" else "_f64"
-        self.L = lib()
+        if threads > 1:
+            if precision != "f32":
+                raise ValueError("the multi-threaded oracle is built for FP32 only")
+            self.L = lib_omp()
+            self.L.orc_set_threads_f32(C.c_int(int(threads)))
+        else:
+            self.L = lib()
         R = self.dt_np
         from specfem3d_amd.gll import get_gll
         g = get_gll()

@@ -37,11 +37,26 @@ typedef ORC_REAL R;
 #define NGLL 5
 #define NGLL3 125
 
+/* -DORC_OMP builds the multi-threaded CPU baseline (liboracle_omp.so): element loops run in
+ * parallel with atomic scatter-adds, so the summation order at shared points varies between
+ * runs. The parity tests use the serial build only. */
+#ifdef ORC_OMP
+#include <omp.h>
+#define ORC_PRAGMA(x) _Pragma(#x)
+#define ORC_PAR_FOR ORC_PRAGMA(omp parallel for schedule(static))
+#define ORC_ATOMIC ORC_PRAGMA(omp atomic)
+void FN(orc_set_threads)(int n) { omp_set_num_threads(n); }
+#else
+#define ORC_PAR_FOR
+#define ORC_ATOMIC
+#endif
+
 /* ------------------------------------------------------------------------- */
 /* Newmark predictor: update_displ_elastic, update_displacement_scheme.f90:359-410
  * (and update_displ_acoustic :206-227 with n scalar values). */
 void FN(orc_update_displ)(int64_t n, R *displ, R *veloc, R *accel, R deltat, R deltatover2, R deltatsqover2)
 {
+    ORC_PAR_FOR
     for (int64_t i = 0; i < n; i++) {
         displ[i] = displ[i] + deltat * veloc[i] + deltatsqover2 * accel[i];
         veloc[i] = veloc[i] + deltatover2 * accel[i];
@@ -54,11 +69,13 @@ void FN(orc_update_displ)(int64_t n, R *displ, R *veloc, R *accel, R deltat, R d
 void FN(orc_kernel3_elastic)(int64_t nglob, R *veloc, R *accel, const R *rmassx, const R *rmassy,
                              const R *rmassz, R deltatover2)
 {
+    ORC_PAR_FOR
     for (int64_t i = 0; i < nglob; i++) {
         accel[3 * i + 0] = accel[3 * i + 0] * rmassx[i];
         accel[3 * i + 1] = accel[3 * i + 1] * rmassy[i];
         accel[3 * i + 2] = accel[3 * i + 2] * rmassz[i];
     }
+    ORC_PAR_FOR
     for (int64_t i = 0; i < 3 * nglob; i++) veloc[i] = veloc[i] + deltatover2 * accel[i];
 }
 
@@ -142,14 +159,15 @@ void FN(orc_forces_elastic)(int nelem, const int *elems, const int *ibool, const
 {
     const R ONE_THIRD = (R)1 / (R)3;
     const R FOUR_THIRDS = (R)4 / (R)3;
-    R dx[125], dy[125], dz[125];
-    R tx1[125], tx2[125], tx3[125], ty1[125], ty2[125], ty3[125], tz1[125], tz2[125], tz3[125];
-    R ax[125], ay[125], az[125];
-    R ax1[125], ax2[125], ax3[125], ay1[125], ay2[125], ay3[125], az1[125], az2[125], az3[125];
-    R nx1[125], nx2[125], nx3[125], ny1[125], ny2[125], ny3[125], nz1[125], nz2[125], nz3[125];
-    R e_tr[125], e_xx[125], e_yy[125], e_xy[125], e_xz[125], e_yz[125];
 
+    ORC_PAR_FOR
     for (int ie = 0; ie < nelem; ie++) {
+        R dx[125], dy[125], dz[125];
+        R tx1[125], tx2[125], tx3[125], ty1[125], ty2[125], ty3[125], tz1[125], tz2[125], tz3[125];
+        R ax[125], ay[125], az[125];
+        R ax1[125], ax2[125], ax3[125], ay1[125], ay2[125], ay3[125], az1[125], az2[125], az3[125];
+        R nx1[125], nx2[125], nx3[125], ny1[125], ny2[125], ny3[125], nz1[125], nz2[125], nz3[125];
+        R e_tr[125], e_xx[125], e_yy[125], e_xy[125], e_xz[125], e_yz[125];
         const int e = elems[ie];
         const int *ib = ibool + (int64_t)e * 125;
         for (int p = 0; p < 125; p++) {
@@ -312,9 +330,15 @@ void FN(orc_forces_elastic)(int nelem, const int *elems, const int *ibool, const
                     R fac1 = wgllwgll[j * 5 + k];
                     R fac2 = wgllwgll[i * 5 + k];
                     R fac3 = wgllwgll[i * 5 + j];
-                    accel[3 * g + 0] = accel[3 * g + 0] - (fac1 * nx1[p] + fac2 * nx2[p] + fac3 * nx3[p]);
-                    accel[3 * g + 1] = accel[3 * g + 1] - (fac1 * ny1[p] + fac2 * ny2[p] + fac3 * ny3[p]);
-                    accel[3 * g + 2] = accel[3 * g + 2] - (fac1 * nz1[p] + fac2 * nz2[p] + fac3 * nz3[p]);
+                    const R fx = fac1 * nx1[p] + fac2 * nx2[p] + fac3 * nx3[p];
+                    const R fy = fac1 * ny1[p] + fac2 * ny2[p] + fac3 * ny3[p];
+                    const R fz = fac1 * nz1[p] + fac2 * nz2[p] + fac3 * nz3[p];
+                    ORC_ATOMIC
+                    accel[3 * g + 0] -= fx;
+                    ORC_ATOMIC
+                    accel[3 * g + 1] -= fy;
+                    ORC_ATOMIC
+                    accel[3 * g + 2] -= fz;
                 }
 
         if (att) {
@@ -355,8 +379,9 @@ void FN(orc_forces_acoustic)(int nelem, const int *elems, const int *ibool, cons
                              const R *hprime, const R *hprimewgll, const R *wgllwgll,
                              const R *rhostore, const R *pot, R *potdd)
 {
-    R chi[125], t1[125], t2[125], t3[125], n1[125], n2[125], n3[125];
+    ORC_PAR_FOR
     for (int ie = 0; ie < nelem; ie++) {
+        R chi[125], t1[125], t2[125], t3[125], n1[125], n2[125], n3[125];
         const int e = elems[ie];
         const int *ib = ibool + (int64_t)e * 125;
         for (int p = 0; p < 125; p++) chi[p] = pot[ib[p]];
@@ -391,8 +416,9 @@ void FN(orc_forces_acoustic)(int nelem, const int *elems, const int *ibool, cons
             for (int j = 0; j < 5; j++)
                 for (int i = 0; i < 5; i++) {
                     int p = i + 5 * j + 25 * k;
-                    potdd[ib[p]] = potdd[ib[p]] - (wgllwgll[j * 5 + k] * n1[p] + wgllwgll[i * 5 + k] * n2[p]
-                                                   + wgllwgll[i * 5 + j] * n3[p]);
+                    const R f = wgllwgll[j * 5 + k] * n1[p] + wgllwgll[i * 5 + k] * n2[p] + wgllwgll[i * 5 + j] * n3[p];
+                    ORC_ATOMIC
+                    potdd[ib[p]] -= f;
                 }
     }
 }
