@@ -142,6 +142,12 @@ def test_homogeneous_box_against_reference_fp32_fp64():
     e32, e64 = rel_l2(out, ref["fp32"]), rel_l2(out, ref["fp64"])
     print(f"[gpu] 20^3 homogeneous box, 2000 steps: rel-L2 vs reference FP32 {e32:.3e}, vs FP64 {e64:.3e}", flush=True)
     assert e32 < 1e-5 and e64 < 1e-5
+    # the reference check script's per-trace metrics (correlation, normalised L2, time shift)
+    from specfem3d_amd.compare import compare
+    m = compare(ref["fp32"], out, 0.01)
+    print(f"[gpu] per-trace: min correlation {m['corr_min']:.8f}, max error {m['err_max']:.3e}, "
+          f"max shift {m['shift_max']:.2e} s", flush=True)
+    assert m["corr_min"] > 0.99999 and m["err_max"] < 1e-4 and abs(m["shift_max"]) < 1e-3
 
 
 def test_stacey5_reference_example():
