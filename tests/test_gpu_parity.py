@@ -92,6 +92,21 @@ def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
     assert rel_l2(fast, ref) < 1e-5
 
 
+def test_large_coupled_workload_reduced(oracle_mod):
+    """The 160x160x80 coupled workload's builder at scale 0.1 (16x16x8 elements, one acoustic layer
+    over the 3-SLS solid, the second of its shots): exact + ordered equals the oracle bit for bit,
+    FMA + atomics within 1e-5."""
+    from specfem3d_amd import workloads as W
+    w = W.sea_over_visco_large(scale=0.1, nshots=2, nstep=150, nrec_side=3)
+    src, rec, n = w.shots[1]
+    ref = run_oracle(oracle_mod, w.db, src, rec, n, w.attenuation)
+    out = run_gpu(w.db, src, rec, n, True, 1, w.attenuation, shape=1)
+    assert np.abs(ref).max() > 0
+    assert np.array_equal(out, ref), rel_l2(out, ref)
+    fast = run_gpu(w.db, src, rec, n, False, 0, w.attenuation, shape=1)
+    assert rel_l2(fast, ref) < 1e-5
+
+
 @pytest.mark.parametrize("att_variant,pipe_wgs", [(1, None), (2, None), (3, None), (4, None), (5, None), (5, 7),
                                                    (5, 100)])
 def test_attenuation_schedules_bitwise(oracle_mod, att_variant, pipe_wgs):

@@ -10,6 +10,14 @@ from specfem3d_amd.mesh.box import build_box, homogeneous
 from specfem3d_amd.workloads import nglob_box
 
 
+def test_workload_point_counts():
+    """The unique GLL-point counts the throughput metric is quoted on: 20^3 box, 96x96x48 and
+    160x160x80 (131,892,801 points)."""
+    assert nglob_box(20, 20, 20) == 81 ** 3
+    assert nglob_box(96, 96, 48) == 28_607_425
+    assert nglob_box(160, 160, 80) == 131_892_801
+
+
 def test_box_database_geometry():
     db = build_box(3, 2, 4, (0, 1500, 0, 1000, -2000, 0), material=homogeneous(2000, 3000, 1700))
     db.finalize(0.005)
