@@ -297,11 +297,17 @@ def large_measurement(args) -> dict:
     import numpy as np
     from specfem3d_amd import workloads as W
     from specfem3d_amd.solver import GPUSolver
-    if args.large_scale >= 1.0:
+    nx = max(2, int(round(160 * args.large_scale)))
+    nz = max(8, int(round(80 * args.large_scale)))
+    r_bytes = 18 * 125 * 4 * nx * nx * (nz - max(1, nz // 8))
+    if r_bytes >= 4 * 2 ** 30:
         # At 160x160x80 (131.9 M points) the stepping has twice produced no output for minutes after a
-        # complete 11 s set-up; until the cause is found this measurement refuses to run at full size.
-        raise SystemExit("bench.py: --large-steps at --large-scale 1 is disabled: the 131.9 M-point "
-                         "single-shot run stalls after set-up for a reason not yet found")
+        # complete 11 s set-up; the one size-related difference to the runs that work is per-point state
+        # above 4 GiB (memory variables 18.4 GB). Until the cause is found, any size with a
+        # memory-variable array of 4 GiB or more is refused.
+        raise SystemExit(f"bench.py: --large-steps with a {r_bytes / 2 ** 30:.1f} GiB memory-variable array is "
+                         "disabled: the 131.9 M-point single-shot run stalled after set-up for a reason not yet "
+                         "found (runs below 4 GiB of per-point state only)")
     t0 = time.time()
     n = args.large_steps + 3
     w = W.sea_over_visco_large(scale=args.large_scale, nshots=1, nstep=n)
@@ -535,7 +541,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded gradient + Gaussian random field model, random CMT source)",
         "config": {"workload": w.name, "nglob": nglob, "nspec": db.nspec, "nspec_acoustic": db.nspec_acoustic,
-                   "attenuation": "3 SLS, Q_mu=100", "receivers": w.receivers.n, "exact": args.exact,
+                   "attenuation": (f"{len(w.attenuation.tau_sigma)} SLS, Q_mu={w.meta.get('qmu'):g}"
+                                   if w.attenuation is not None else "none"),
+                   "receivers": w.receivers.n, "exact": args.exact,
                    "assembly": ["atomic", "ordered", "coloured"][args.assembly],
                    "element_order": args.element_order,
                    "att_variant": "default" if args.att_variant < 0 else args.att_variant},

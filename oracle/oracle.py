@@ -107,6 +107,7 @@ class OracleSolver:
         self.jac9 = np.ascontiguousarray(db.jac9.reshape(-1), dtype=R)
         self.jac = np.ascontiguousarray(db.jacobian.reshape(-1), dtype=R)
         self.kappa = np.ascontiguousarray(db.kappastore.reshape(-1), dtype=R)
+        self.kappa_ac = self.kappa    # kappastore of the fluid source terms (not rescaled by attenuation)
         self.mu = np.ascontiguousarray(db.mustore.reshape(-1), dtype=R)
         self.rho = np.ascontiguousarray(db.rhostore.reshape(-1), dtype=R)
         self.rho_vp = np.ascontiguousarray(db.rho_vp.reshape(-1), dtype=R)
@@ -245,6 +246,17 @@ class OracleSolver:
                 n, isp, ijk, nrm, jw = self.cpl
                 self._call("orc_coupling_ac_el", C.c_int(n), _p(isp), _p(ijk), _p(nrm), _p(jw), _p(self.ibool),
                            _p(self.displ), _p(self.potdd), C.c_int(self.adjoint_sign ^ self.plant_flip_ac_el))
+            # sources and adjoint sources in fluid elements (compute_forces_acoustic_calling_routine.f90:
+            # after Stacey and coupling, before the stiffness of phase 2)
+            if self.sources is not None and self.sources.n:
+                stf = np.ascontiguousarray(self.stf_values(it), self.dt_np)
+                self._call("orc_add_sources_ac", C.c_int(self.sources.n), _p(self.src_ispec), _p(self.src_arr),
+                           _p(stf), _p(self.ibool), _p(self.is_ac), _p(self.kappa_ac), _p(self.potdd))
+            if adjoint_source is not None:
+                adj = np.ascontiguousarray(adjoint_source, self.dt_np)
+                self._call("orc_add_adjoint_sources_ac", C.c_int(self.receivers.n), _p(self.rec_ispec),
+                           _p(self.hx), _p(self.hy), _p(self.hz), _p(adj), _p(self.ibool), _p(self.is_ac),
+                           _p(self.potdd))
             self.forces_acoustic()
             self._call("orc_kernel3_acoustic", C.c_int64(self.db.nglob), _p(self.potd), _p(self.potdd),
                        _p(self.rmass_ac), self._r(self.deltatover2))
@@ -269,7 +281,8 @@ class OracleSolver:
             if adjoint_source is not None:
                 adj = np.ascontiguousarray(adjoint_source, self.dt_np)
                 self._call("orc_add_adjoint_sources_el", C.c_int(self.receivers.n), _p(self.rec_ispec),
-                           _p(self.hx), _p(self.hy), _p(self.hz), _p(adj), _p(self.ibool), _p(self.accel))
+                           _p(self.hx), _p(self.hy), _p(self.hz), _p(adj), _p(self.ibool), _p(self.accel),
+                           _p(self.is_el))
             self.forces_elastic()
             self._call("orc_kernel3_elastic", C.c_int64(self.db.nglob), _p(self.veloc), _p(self.accel),
                        _p(self.rmassx), _p(self.rmassy), _p(self.rmassz), self._r(self.deltatover2))
@@ -298,10 +311,21 @@ class OracleSolver:
                    _p(self.rmassx), _p(self.rmassy), _p(self.rmassz), self._r(self.deltatover2))
 
     def seismogram(self, field: str = "displ") -> np.ndarray:
+        """One sample of every receiver: ``field`` in "displ", "veloc", "accel" (vectors; in fluid
+        elements (1/rho) grad of the potential of the same order), "pressure" (-potential_dot_dot in fluid
+        elements) or "pressure_trick" (-potential, the source STF being the second derivative)."""
         out = np.zeros(3 * self.receivers.n, self.dt_np)
-        f = {"displ": self.displ, "veloc": self.veloc, "accel": self.accel}[field]
-        self._call("orc_seismo_elastic", C.c_int(self.receivers.n), _p(self.rec_ispec), _p(self.hx), _p(self.hy),
-                   _p(self.hz), _p(self.nu), _p(self.ibool), _p(f), _p(out))
+        if field in ("displ", "veloc", "accel") and self.elastic:
+            f = {"displ": self.displ, "veloc": self.veloc, "accel": self.accel}[field]
+            self._call("orc_seismo_elastic", C.c_int(self.receivers.n), _p(self.rec_ispec), _p(self.hx),
+                       _p(self.hy), _p(self.hz), _p(self.nu), _p(self.ibool), _p(f), _p(out), _p(self.is_el))
+        if self.acoustic:
+            pot = {"displ": self.pot, "veloc": self.potd, "accel": self.potdd, "pressure": self.potdd,
+                   "pressure_trick": self.pot}[field]
+            self._call("orc_seismo_acoustic", C.c_int(self.receivers.n), _p(self.rec_ispec), _p(self.hx),
+                       _p(self.hy), _p(self.hz), _p(self.nu), _p(self.ibool), _p(self.is_ac), _p(self.irr),
+                       self._r(self.db.xix_regular), _p(self.jac9), _p(self.hprime), _p(self.rho), _p(pot),
+                       C.c_int(int(field.startswith("pressure"))), _p(out))
         return out.reshape(-1, 3)
 
     def run(self, nstep: int, record: str = "displ", callback=None) -> np.ndarray:

@@ -555,13 +555,51 @@ void FN(orc_add_sources_el)(int nsrc, const int *src_ispec, const R *sourcearray
     }
 }
 
-/* adjoint sources at receivers: compute_add_sources_viscoelastic.F90:226-245;
- * adj[r*3 + c] is the adjoint source value for this step, h* are the interpolants (double). */
-void FN(orc_add_adjoint_sources_el)(int nrec, const int *rec_ispec, const double *hxir, const double *hetar,
-                                    const double *hgammar, const R *adj, const int *ibool, R *accel)
+/* point sources in acoustic elements: compute_add_sources_acoustic.f90:95-124,
+ * potential_dot_dot -= sourcearrays(isource,1,i,j,k) * stf_used / kappastore(i,j,k,ispec). */
+void FN(orc_add_sources_ac)(int nsrc, const int *src_ispec, const R *sourcearrays, const R *stf,
+                            const int *ibool, const unsigned char *is_acoustic, const R *kappastore, R *potdd)
+{
+    for (int s = 0; s < nsrc; s++) {
+        int e = src_ispec[s];
+        if (!is_acoustic[e]) continue;
+        R st = stf[s];
+        for (int p = 0; p < 125; p++) {
+            int64_t g = ibool[(int64_t)e * 125 + p];
+            potdd[g] = potdd[g] - sourcearrays[((int64_t)s * 125 + p) * 3] * st / kappastore[(int64_t)e * 125 + p];
+        }
+    }
+}
+
+/* adjoint sources at receivers in acoustic elements: compute_add_sources_acoustic.f90:205-235,
+ * potential_dot_dot += source_adjoint(1,irec,it_index) * hlagrange (CUSTOM_REAL interpolants). */
+void FN(orc_add_adjoint_sources_ac)(int nrec, const int *rec_ispec, const double *hxir, const double *hetar,
+                                    const double *hgammar, const R *adj, const int *ibool,
+                                    const unsigned char *is_acoustic, R *potdd)
 {
     for (int r = 0; r < nrec; r++) {
         int e = rec_ispec[r];
+        if (!is_acoustic[e]) continue;
+        for (int k = 0; k < 5; k++)
+            for (int j = 0; j < 5; j++)
+                for (int i = 0; i < 5; i++) {
+                    int64_t g = ibool[(int64_t)e * 125 + i + 5 * j + 25 * k];
+                    R h = (R)hxir[r * 5 + i] * (R)hetar[r * 5 + j] * (R)hgammar[r * 5 + k];
+                    potdd[g] = potdd[g] + adj[r * 3] * h;
+                }
+    }
+}
+
+/* adjoint sources at receivers: compute_add_sources_viscoelastic.F90:226-245 (receivers in elastic
+ * elements; is_elastic may be NULL when all are);
+ * adj[r*3 + c] is the adjoint source value for this step, h* are the interpolants (double). */
+void FN(orc_add_adjoint_sources_el)(int nrec, const int *rec_ispec, const double *hxir, const double *hetar,
+                                    const double *hgammar, const R *adj, const int *ibool, R *accel,
+                                    const unsigned char *is_elastic)
+{
+    for (int r = 0; r < nrec; r++) {
+        int e = rec_ispec[r];
+        if (is_elastic && !is_elastic[e]) continue;
         for (int k = 0; k < 5; k++)
             for (int j = 0; j < 5; j++)
                 for (int i = 0; i < 5; i++) {
@@ -579,10 +617,11 @@ void FN(orc_add_adjoint_sources_el)(int nrec, const int *rec_ispec, const double
  * :28-100, accumulated in double; out[r*3 + c] rotated by nu[r*9 + 3*a + b]. */
 void FN(orc_seismo_elastic)(int nrec, const int *rec_ispec, const double *hxir, const double *hetar,
                             const double *hgammar, const double *nu, const int *ibool, const R *field,
-                            R *out)
+                            R *out, const unsigned char *is_elastic)
 {
     for (int r = 0; r < nrec; r++) {
         int e = rec_ispec[r];
+        if (is_elastic && !is_elastic[e]) continue;
         double dx = 0, dy = 0, dz = 0;
         for (int k = 0; k < 5; k++)
             for (int j = 0; j < 5; j++)
@@ -718,6 +757,53 @@ static void acoustic_gradient(const R *hp, const R *chi, int irr, R xix_regular,
             gy[p] = t2[p] * xix_regular * rho_invl;
             gz[p] = t3[p] * xix_regular * rho_invl;
         }
+    }
+}
+
+/* receivers in acoustic elements: compute_seismograms.f90:127-144 with compute_interpolated_dva_acoust
+ * (compute_interpolated_dva.f90:219-328), FP64 sums in (i, j, k) order:
+ *   pressure != 0: out[r*3] = -sum h field(ibool) (field = potential with the pressure trick, else
+ *                  potential_dot_dot), out[r*3 + 1..2] = 0;
+ *   pressure == 0: (1/rho) grad(field) at the element's GLL points (compute_gradient_in_acoustic),
+ *                  interpolated and rotated by nu[r*9 + 3*a + b]. Receivers in other domains are skipped. */
+void FN(orc_seismo_acoustic)(int nrec, const int *rec_ispec, const double *hxir, const double *hetar,
+                             const double *hgammar, const double *nu, const int *ibool,
+                             const unsigned char *is_acoustic, const int *irregular, R xix_regular, const R *jac9,
+                             const R *hprime, const R *rhostore, const R *field, int pressure, R *out)
+{
+    R chi[125], gx[125], gy[125], gz[125];
+    for (int r = 0; r < nrec; r++) {
+        int e = rec_ispec[r];
+        if (!is_acoustic[e]) continue;
+        const int *ib = ibool + (int64_t)e * 125;
+        for (int p = 0; p < 125; p++) chi[p] = field[ib[p]];
+        double dx = 0, dy = 0, dz = 0;
+        if (pressure) {
+            for (int k = 0; k < 5; k++)
+                for (int j = 0; j < 5; j++)
+                    for (int i = 0; i < 5; i++) {
+                        double h = hxir[r * 5 + i] * hetar[r * 5 + j] * hgammar[r * 5 + k];
+                        dx = dx - h * (double)chi[i + 5 * j + 25 * k];
+                    }
+            out[r * 3 + 0] = (R)dx;
+            out[r * 3 + 1] = 0;
+            out[r * 3 + 2] = 0;
+            continue;
+        }
+        acoustic_gradient(hprime, chi, irregular[e], xix_regular, jac9, rhostore + (int64_t)e * 125, gx, gy, gz);
+        for (int k = 0; k < 5; k++)
+            for (int j = 0; j < 5; j++)
+                for (int i = 0; i < 5; i++) {
+                    int p = i + 5 * j + 25 * k;
+                    double h = hxir[r * 5 + i] * hetar[r * 5 + j] * hgammar[r * 5 + k];
+                    dx = dx + h * (double)gx[p];
+                    dy = dy + h * (double)gy[p];
+                    dz = dz + h * (double)gz[p];
+                }
+        const double *n = nu + (int64_t)r * 9;
+        out[r * 3 + 0] = (R)(n[0] * dx + n[1] * dy + n[2] * dz);
+        out[r * 3 + 1] = (R)(n[3] * dx + n[4] * dy + n[5] * dz);
+        out[r * 3 + 2] = (R)(n[6] * dx + n[7] * dy + n[8] * dz);
     }
 }
 

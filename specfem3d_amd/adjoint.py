@@ -130,8 +130,8 @@ def read_kernel_bin(path: str) -> np.ndarray:
 class KernelSolver(GPUSolver):
     """Checkpointed sensitivity-kernel run on the GPU (SIMULATION_TYPE=3, undo-attenuation loop).
 
-    Elastic, viscoelastic and coupled acoustic-elastic meshes (sources and receivers in solid
-    elements); the forward/adjoint field is ``FORWARD_OR_ADJOINT`` 1 and the reconstructed forward
+    Elastic, viscoelastic and coupled acoustic-elastic meshes (sources and receivers in solid or
+    fluid elements); the forward/adjoint field is ``FORWARD_OR_ADJOINT`` 1 and the reconstructed forward
     field 3, as in the reference. Each step runs the acoustic domain first, then the elastic one
     (compute_forces_acoustic_calling, then compute_forces_viscoelastic_calling).
 
@@ -162,10 +162,12 @@ class KernelSolver(GPUSolver):
     def _subset_len(self, slot: int) -> int:
         return self.nt_dump if slot < self.nsub - 1 else self.nstep - (self.nsub - 1) * self.nt_dump
 
-    def _acoustic_step(self, foa: int):
+    def _acoustic_step(self, foa: int, fwd_it: int = 0, it: int = 0, adj: np.ndarray | None = None):
         """Fluid part of one step of field ``foa`` (compute_forces_acoustic_calling /
-        _backward_calling, undo-attenuation Stacey): free surface, stiffness, Stacey and the
-        solid-to-fluid coupling, corrector, free surface."""
+        _backward_calling, undo-attenuation Stacey): free surface, stiffness, Stacey, the
+        solid-to-fluid coupling and the sources in fluid elements (the CMT sources at forward step
+        ``fwd_it`` on the reconstructed field, the adjoint sources of step ``it`` on the adjoint field),
+        corrector, free surface."""
         L, MP = self.L, self.MP
         f, zero = self._ref(foa), self._ref(0)
         dt2 = C.byref(self._c["dt2"])
@@ -177,6 +179,14 @@ class KernelSolver(GPUSolver):
             if iphase == 1:
                 L.compute_stacey_acoustic_undoatt_cuda_(MP, C.byref(ip), f)
                 L.compute_coupling_ac_el_cuda_(MP, C.byref(ip), C.byref(ncf), f)
+                if foa == 3 and self.stf_table is not None:
+                    stf = np.ascontiguousarray(self.stf_table[fwd_it - 1], np.float64)
+                    L.compute_add_sources_ac_s3_cuda_(MP, C.byref(C.c_int(len(stf))),
+                                                      stf.ctypes.data_as(C.c_void_p), zero)
+                if foa == 1 and adj is not None and it < self.nstep:
+                    nrec, ntr = C.c_int(self.receivers.n), C.c_int(self.nstep)
+                    L.add_sources_ac_sim_2_or_3_cuda_(MP, abi.ptr(adj), C.byref(nrec), C.byref(nrec), C.byref(ntr),
+                                                      C.byref(C.c_int(it)))
         L.kernel_3_acoustic_cuda_(MP, dt2, dt2, f)
         L.acoustic_enforce_free_surf_cuda_(MP, zero, f)
 
@@ -190,7 +200,7 @@ class KernelSolver(GPUSolver):
             L.update_displacement_ac_cuda_(MP, dt, dt2, dtsq2, three)
         L.update_displacement_cuda_(MP, dt, dt2, dtsq2, three)
         if self.acoustic:
-            self._acoustic_step(3)
+            self._acoustic_step(3, fwd_it=fwd_it)
         strain = C.c_int(1)
         att = C.c_int(int(self.att is not None))
         ncf = C.c_int(self.db.coupling_faces.n)
@@ -215,7 +225,7 @@ class KernelSolver(GPUSolver):
             L.update_displacement_ac_cuda_(MP, dt, dt2, dtsq2, one)
         L.update_displacement_cuda_(MP, dt, dt2, dtsq2, one)
         if self.acoustic:
-            self._acoustic_step(1)
+            self._acoustic_step(1, it=it, adj=adj)
         strain = C.c_int(1)
         att = C.c_int(int(self.att is not None))
         nrec = C.c_int(self.receivers.n)

@@ -11,7 +11,11 @@ Produces the inputs of the viscoelastic stiffness kernel -- ``factor_common``,
 * tau_epsilon per Q fitted by a Nelder-Mead simplex search of the summed relative misfit of
   ``tan(delta) = B/A`` of the Maxwell bodies against ``1/Q`` at 100 log-spaced frequencies
   (``attenuation_invert_by_simplex``/``attenuation_eval``/``attenuation_maxwell`` :976-1215),
-  started at ``tau_eps = tau_s (1 + 2/Q)`` and cached per Q rounded to 0.1 (:879-970);
+  started at ``tau_eps = tau_s (1 + 2/Q)`` and cached per Q rounded to 0.1 (:879-970). The search
+  is the reference's own ``fminsearch`` (:1222-1607) step for step -- coefficients, the 5 % initial
+  simplex, the absolute tolerances 1e-4 on tau and misfit, 200 n iterations, the stable bubble sort
+  of the vertices -- because with an absolute tolerance of 1e-4 on tau values of 3e-3 the point it
+  stops at depends on the path, not only on the minimum;
 * ``beta = tau_eps/tau_s - 1``, ``one_minus_sum_beta = sum tau_eps/tau_s`` (:599-628);
 * modulus scale factor to the reference frequency (``get_attenuation_scale_factor`` :634-694);
 * ``factor_common = 2 beta / tau_s / one_minus_sum_beta`` (mu) and ``beta / tau_s / ...`` (kappa)
@@ -19,10 +23,9 @@ Produces the inputs of the viscoelastic stiffness kernel -- ``factor_common``,
 * RK coefficients = 4th-order Taylor series of exp(-dt/tau_s) (``get_attenuation_memory_values``
   :464-493), evaluated in float32.
 
-The simplex search is scipy's Nelder-Mead (the same algorithm as the reference's
-``fminsearch``); its converged tau_eps agree with the reference to the search tolerance, so
-factor values are reproduced to about 1e-4 relative -- they are inputs of the hot path, not
-part of it.
+The reference's viscoelastic benchmark (EXAMPLES/benchmarks/attenuation/viscoelastic) lists the
+tau_sigma, tau_epsilon and scale factors its analytical solution was computed with (Q_kappa = 10,
+Q_mu = 20, f0 = 18 Hz on its mesh); these routines reproduce them (tests/test_attenuation_reference.py).
 """
 from __future__ import annotations
 
@@ -30,7 +33,6 @@ import math
 from dataclasses import dataclass
 
 import numpy as np
-from scipy.optimize import fmin
 
 N_SLS = 3
 ATTENUATION_COMP_MAXIMUM = 9000.0
@@ -55,36 +57,135 @@ def source_freq(min_period: float, max_period: float) -> float:
 
 
 def _maxwell(f, tau_s, tau_eps):
+    """A, B of the N_SLS Maxwell bodies at log10-frequencies f (``attenuation_maxwell``), each
+    accumulated over the bodies in order."""
     PI = 3.14159265358979
     w = 2.0 * PI * 10.0 ** f
     A = np.zeros_like(w)
     B = np.zeros_like(w)
     for j in range(len(tau_s)):
         den = 1.0 + w ** 2 * tau_s[j] ** 2
-        A += (1.0 + w ** 2 * tau_eps[j] * tau_s[j]) / den
+        A += (1.0 + (w ** 2 * tau_eps[j] * tau_s[j])) / den
         B += w * (tau_eps[j] - tau_s[j]) / den
     return A, B
+
+
+# REAL (single precision) literals of the reference's fminsearch, as the double values they become
+_F32 = lambda v: float(np.float32(v))
+
+
+def fminsearch(funk, x0: np.ndarray):
+    """Nelder-Mead search as ``fminsearch`` (get_attenuation_model.f90:1222-1473) with the tolerances
+    ``attenuation_invert_by_simplex`` passes (absolute 1e-4 on the simplex size and on the spread of
+    the function values, 200 n iterations and evaluations). Returns (x, f(x), iterations, evaluations)."""
+    n = len(x0)
+    rho, chi, psi, sigma = 1.0, 2.0, 0.5, 0.5
+    maxiter = maxfun = 200 * n
+    tolx = tolf = _F32(1.0e-4)
+    usual_delta = _F32(0.05)
+    zero_term_delta = _F32(0.00025)
+    xin = np.asarray(x0, np.float64).copy()
+    v = [xin.copy()]
+    fv = [funk(xin)]
+    for j in range(n):
+        y = xin.copy()
+        y[j] = (1.0 + usual_delta) * y[j] if y[j] != 0.0 else zero_term_delta
+        v.append(y)
+        fv.append(funk(y))
+
+    def sort():
+        # qsort_local: bubble sort on the values, stable (swaps only on a strict decrease)
+        idx = list(range(n + 1))
+        f = list(fv)
+        for j in range(n + 1):
+            for k in range(n - j):
+                if f[k + 1] < f[k]:
+                    f[k], f[k + 1] = f[k + 1], f[k]
+                    idx[k], idx[k + 1] = idx[k + 1], idx[k]
+        return [v[i] for i in idx], f
+
+    v, fv = sort()
+    itercount, func_evals = 1, n + 1
+    while func_evals < maxfun and itercount < maxiter:
+        size = max(abs(v[j][i] - v[0][i]) for i in range(n) for j in range(1, n + 1))
+        spread = max(abs(fv[0] - fv[i]) for i in range(1, n + 1))
+        if size <= tolx and spread <= tolf:
+            break
+        xbar = np.zeros(n)
+        for i in range(n):
+            acc = 0.0
+            for j in range(n):
+                acc = acc + v[j][i]
+            xbar[i] = acc / (n * 1.0)
+        xr = (1 + rho) * xbar - rho * v[n]
+        fxr = funk(xr)
+        func_evals += 1
+        if fxr < fv[0]:
+            xe = (1 + rho * chi) * xbar - rho * chi * v[n]
+            fxe = funk(xe)
+            func_evals += 1
+            if fxe < fxr:
+                v[n], fv[n] = xe, fxe
+            else:
+                v[n], fv[n] = xr, fxr
+        elif fxr < fv[n - 1]:
+            v[n], fv[n] = xr, fxr
+        else:
+            shrink = False
+            if fxr < fv[n]:
+                xc = (1 + psi * rho) * xbar - psi * rho * v[n]
+                fxc = funk(xc)
+                func_evals += 1
+                if fxc <= fxr:
+                    v[n], fv[n] = xc, fxc
+                else:
+                    shrink = True
+            else:
+                xcc = (1 - psi) * xbar + psi * v[n]
+                fxcc = funk(xcc)
+                func_evals += 1
+                if fxcc < fv[n]:
+                    v[n], fv[n] = xcc, fxcc
+                else:
+                    shrink = True
+            if shrink:
+                for j in range(1, n + 1):
+                    v[j] = v[0] + sigma * (v[j] - v[0])
+                    fv[j] = funk(v[j])
+                func_evals += n
+        v, fv = sort()
+        itercount += 1
+    if func_evals > maxfun or itercount > maxiter:
+        raise RuntimeError("attenuation_simplex: search for strain relaxation times did not converge")
+    return v[0], fv[0], itercount, func_evals
 
 
 _TAU_EPS_CACHE: dict = {}
 
 
 def tau_eps_for_q(Q: float, tau_s: np.ndarray, min_period: float, max_period: float) -> np.ndarray:
-    key = (int(Q * 10), tuple(np.round(tau_s, 12)))
+    """Strain relaxation times fitting a constant Q over the band (``get_attenuation_tau_eps`` with
+    ``attenuation_invert_by_simplex``), cached per int(10 Q) like ``model_attenuation_storage``."""
+    key = (int(Q * 10), tuple(np.round(tau_s, 12)), min_period, max_period)
     if key in _TAU_EPS_CACHE:
         return _TAU_EPS_CACHE[key]
     nf = 100
     e1 = math.log10(1.0 / max_period)
     e2 = math.log10(1.0 / min_period)
-    f = e1 + np.arange(nf) * (e2 - e1) / (nf - 1.0)
+    f = np.array([e1 + ((i * 1.0) * (e2 - e1) / ((nf - 1) * 1.0)) for i in range(nf)])
     iQ = 1.0 / Q
+    iQ2 = iQ ** 2
 
     def misfit(x):
         A, B = _maxwell(f, tau_s, x)
-        return float(np.sum(np.sqrt(((B / A - iQ) ** 2) / iQ ** 2)))
+        xi = np.sqrt(((B / A - iQ) ** 2) / iQ2)
+        acc = 0.0
+        for val in xi.tolist():     # summed in order, as attenuation_eval
+            acc = acc + val
+        return acc
 
-    x0 = tau_s + tau_s * 2.0 / Q
-    x = fmin(misfit, x0, xtol=1e-4 * float(np.min(tau_s)), ftol=1e-4, maxiter=4000, maxfun=8000, disp=False)
+    x0 = np.array([t + (t * 2.0 / Q) for t in tau_s], np.float64)
+    x, _, _, _ = fminsearch(misfit, x0)
     _TAU_EPS_CACHE[key] = np.asarray(x, np.float64)
     return _TAU_EPS_CACHE[key]
 

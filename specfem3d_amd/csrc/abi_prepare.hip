@@ -80,6 +80,7 @@ struct HostSide {
   bool ring_ev_init = false;
   realw* stf_dev = nullptr;
   int nsrc_dev = 0;
+  std::vector<int> src_ispec;  // element of each local source (0-based)
 };
 
 static HostSide* host_of(Mesh* m);
@@ -373,26 +374,42 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
       for (int c = 0; c < 3; c++) arr.push_back(h_sourcearrays[s + (int64_t)ns * (c + 3 * p)]);
   }
   m->nsources = (int)src_ispec.size();
+  H->src_ispec = src_ispec;
   (void)nsources_local_f;
-  // elastic sources are added only in elastic elements (compute_add_sources_viscoelastic.F90:143)
+  // sources are added in the domain of their element: elastic sources to accel
+  // (compute_add_sources_viscoelastic.F90:143), acoustic ones to potential_dot_dot_acoustic
+  // (compute_add_sources_acoustic.f90:97)
   {
     m->src_array = dupload(arr.data(), (int64_t)arr.size());
     m->h_src_elastic = new unsigned char[std::max(1, m->nsources)];
-    std::vector<int> own_el, idx_el;
+    m->h_src_acoustic = new unsigned char[std::max(1, m->nsources)];
+    std::vector<int> own_el, idx_el, own_ac, idx_ac;
     for (int s = 0; s < m->nsources; s++) {
       m->h_src_elastic[s] = H->is_el[src_ispec[s]];
+      m->h_src_acoustic[s] = H->is_ac[src_ispec[s]];
       if (m->h_src_elastic[s]) {
         own_el.push_back(src_ispec[s]);
         idx_el.push_back(s);
       }
+      if (m->h_src_acoustic[s]) {
+        own_ac.push_back(src_ispec[s]);
+        idx_ac.push_back(s);
+      }
     }
+    // owner index = position in the domain's source list -> index into the full source list
+    auto remap_owner = [](FacePoints& fp, const std::vector<int>& idx) {
+      std::vector<int> own(fp.n);
+      SF_CHECK(hipMemcpy(own.data(), fp.ispec, sizeof(int) * own.size(), hipMemcpyDeviceToHost));
+      for (auto& o : own) o = idx[o];
+      SF_CHECK(hipMemcpy(fp.ispec, own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice));
+    };
     if (!own_el.empty()) {
       build_point_sets(own_el, H->ibool, m->src_pts);
-      // owner index = position among elastic sources -> index into the full source list
-      std::vector<int> own(m->src_pts.n);
-      SF_CHECK(hipMemcpy(own.data(), m->src_pts.ispec, sizeof(int) * own.size(), hipMemcpyDeviceToHost));
-      for (auto& o : own) o = idx_el[o];
-      SF_CHECK(hipMemcpy(m->src_pts.ispec, own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice));
+      remap_owner(m->src_pts, idx_el);
+    }
+    if (!own_ac.empty()) {
+      build_point_sets(own_ac, H->ibool, m->src_ac_pts);
+      remap_owner(m->src_ac_pts, idx_ac);
     }
   }
 
@@ -408,12 +425,15 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
     std::vector<double> h((size_t)m->nrec_local * 125), nu((size_t)m->nrec_local * 9);
     m->h_rec_ispec = new int[m->nrec_local];
     m->h_rec_elastic = new unsigned char[m->nrec_local];
+    std::vector<int> el_list, ac_list;
     int irl = 0;
     for (int r = 0; r < *nrec && irl < m->nrec_local; r++) {
       if (h_islice_selected_rec[r] != m->myrank) continue;
       const int e = h_ispec_selected_rec[r] - 1;
       m->h_rec_ispec[irl] = e;
       m->h_rec_elastic[irl] = H->is_el[e];
+      if (H->is_el[e]) el_list.push_back(irl);
+      else if (H->is_ac[e]) ac_list.push_back(irl);
       for (int p = 0; p < 125; p++) {
         const int i = p % 5, j = (p / 5) % 5, k = p / 25;
         ig[(size_t)irl * 125 + p] = H->ibool[(int64_t)e * 125 + p];
@@ -424,6 +444,11 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
       irl++;
     }
     m->rec_iglob = dupload(ig.data(), (int64_t)ig.size());
+    m->rec_ispec = dupload(m->h_rec_ispec, (int64_t)m->nrec_local);
+    m->nrec_el = (int)el_list.size();
+    m->nrec_ac = (int)ac_list.size();
+    m->rec_el_list = dupload(el_list.data(), (int64_t)el_list.size());
+    m->rec_ac_list = dupload(ac_list.data(), (int64_t)ac_list.size());
     m->rec_h = dupload(h.data(), (int64_t)h.size());
     m->rec_nu = dupload(nu.data(), (int64_t)nu.size());
     const int64_t sz = (int64_t)m->nlength_seismogram * m->nrec_local * 3;
@@ -551,6 +576,15 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_device, PREPARE_FIELDS_ELASTIC_D
 
   const int64_t n = m->nglob;
   const int64_t nloc = (int64_t)m->nspec * 125;
+  if (m->assembly < 0) {
+    // automatic assembly: coloured without attenuation, atomic with it (see Mesh::assembly)
+    m->assembly = *ATTENUATION ? 0 : 2;
+    if (m->assembly == 2 && m->nspec_acoustic > 0 && m->acoustic_elems && !m->ac_color_elems) {
+      std::vector<int> lst(m->nspec_acoustic);
+      SF_CHECK(hipMemcpy(lst.data(), m->acoustic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+      build_colors(m, lst, m->ac_color_elems, m->ac_color_off, m->ac_ncolor);
+    }
+  }
   {
     std::vector<realw> rm(3 * (size_t)n);
     for (int64_t g = 0; g < n; g++) {
@@ -744,6 +778,13 @@ extern "C" void FC_FUNC_(prepare_fields_acoustic_device, PREPARE_FIELDS_ACOUSTIC
   m->rmass_ac = dupload(rmass_acoustic, n);
   m->rho = dupload(rhostore, nloc);
   m->kappa_ac = dupload(kappastore, nloc);
+  if (m->src_ac_pts.n > 0) {
+    // kappastore at the points of each source element (the source term is divided by it)
+    std::vector<realw> k((size_t)m->nsources * 125);
+    for (int src = 0; src < m->nsources; src++)
+      for (int p = 0; p < 125; p++) k[(size_t)src * 125 + p] = kappastore[(int64_t)H->src_ispec[src] * 125 + p];
+    m->src_kappa = dupload(k.data(), (int64_t)k.size());
+  }
   {
     const int np = *num_phase_ispec_acoustic;
     std::vector<int> lst;
@@ -891,14 +932,23 @@ extern "C" void FC_FUNC_(prepare_sim2_or_3_const_device, PREPARE_SIM2_OR_3_CONST
   m->nadj_rec_local = *nadj_rec_local;
   m->ntstep_between_read_adjsrc = *NTSTEP_BETWEEN_READ_ADJSRC;
   if (m->nadj_rec_local <= 0) return;
-  std::vector<int> own;
+  // adjoint sources act in the domain of the receiver element: elastic ones on accel
+  // (compute_add_sources_viscoelastic.F90:226), acoustic ones on potential_dot_dot_acoustic
+  // (compute_add_sources_acoustic.f90:205)
+  std::vector<int> own_el, idx_el, own_ac, idx_ac;
   std::vector<realw> h((size_t)m->nadj_rec_local * 125);
   int ir = 0;
   m->h_adj_elastic = new unsigned char[m->nadj_rec_local];
   for (int r = 0; r < *nrec && ir < m->nadj_rec_local; r++) {
     if (h_islice_selected_rec[r] != m->myrank) continue;
     const int e = h_ispec_selected_rec[r] - 1;
-    own.push_back(e);
+    if (H->is_el[e]) {
+      own_el.push_back(e);
+      idx_el.push_back(ir);
+    } else if (H->is_ac[e]) {
+      own_ac.push_back(e);
+      idx_ac.push_back(ir);
+    }
     m->h_adj_elastic[ir] = H->is_el[e];
     for (int p = 0; p < 125; p++) {
       const int i = p % 5, j = (p / 5) % 5, k = p / 25;
@@ -906,7 +956,20 @@ extern "C" void FC_FUNC_(prepare_sim2_or_3_const_device, PREPARE_SIM2_OR_3_CONST
     }
     ir++;
   }
-  build_point_sets(own, H->ibool, m->adj_pts);
+  auto remap_owner = [](FacePoints& fp, const std::vector<int>& idx) {
+    std::vector<int> o(fp.n);
+    SF_CHECK(hipMemcpy(o.data(), fp.ispec, sizeof(int) * o.size(), hipMemcpyDeviceToHost));
+    for (auto& v : o) v = idx[v];
+    SF_CHECK(hipMemcpy(fp.ispec, o.data(), sizeof(int) * o.size(), hipMemcpyHostToDevice));
+  };
+  if (!own_el.empty()) {
+    build_point_sets(own_el, H->ibool, m->adj_pts);
+    remap_owner(m->adj_pts, idx_el);
+  }
+  if (!own_ac.empty()) {
+    build_point_sets(own_ac, H->ibool, m->adj_ac_pts);
+    remap_owner(m->adj_ac_pts, idx_ac);
+  }
   m->adj_h = dupload(h.data(), (int64_t)h.size());
   m->adj_src = dmalloc<realw>(3 * (int64_t)m->nadj_rec_local);
 }
@@ -931,11 +994,12 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   free_face_points(m->abs_el); free_face_points(m->abs_ac); free_face_points(m->free_ac);
   free_face_points(m->coupling);
   dfree(m->b_absorb_field); dfree(m->b_absorb_potential);
-  free_face_points(m->src_pts);
-  dfree(m->src_array);
-  free_face_points(m->adj_pts);
+  free_face_points(m->src_pts); free_face_points(m->src_ac_pts);
+  dfree(m->src_array); dfree(m->src_kappa);
+  free_face_points(m->adj_pts); free_face_points(m->adj_ac_pts);
   dfree(m->adj_h); dfree(m->adj_src);
   dfree(m->rec_iglob); dfree(m->rec_h); dfree(m->rec_nu);
+  dfree(m->rec_ispec); dfree(m->rec_el_list); dfree(m->rec_ac_list);
   dfree(m->seismo_d); dfree(m->seismo_v); dfree(m->seismo_a); dfree(m->seismo_p);
   dfree(m->rho_kl); dfree(m->mu_kl); dfree(m->kappa_kl); dfree(m->rho_ac_kl); dfree(m->kappa_ac_kl);
   dfree(m->asm_ptr); dfree(m->asm_idx); dfree(m->asm_buf);
@@ -946,7 +1010,7 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
     for (int i = 0; i < 2 * m->timing_max; i++) (void)hipEventDestroy(m->timing_ev[i]);
     delete[] m->timing_ev;
   }
-  delete[] m->h_src_elastic; delete[] m->h_rec_elastic; delete[] m->h_rec_ispec; delete[] m->h_adj_elastic;
+  delete[] m->h_src_elastic; delete[] m->h_src_acoustic; delete[] m->h_rec_elastic; delete[] m->h_rec_ispec; delete[] m->h_adj_elastic;
   drop_host(m);
   SF_CHECK(hipStreamDestroy(m->stream));
   delete m;

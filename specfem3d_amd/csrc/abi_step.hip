@@ -443,30 +443,56 @@ extern "C" void FC_FUNC_(add_sources_el_sim_type_2_or_3, ADD_SOURCES_EL_SIM_TYPE
   launch_adjoint_sources_el(m->stream, m->adj_pts, m->adj_h, m->adj_src, m->el[0].accel);
 }
 
+// compute_add_sources_acoustic_cuda.cu: sources in acoustic elements, chi'' -= sourcearray(1) stf / kappa
+// (compute_add_sources_acoustic.f90:95-124); stf_used = real(stf) per source
+static void add_sources_ac(Mesh* m, int f, const double* h_stf, int nsrc, const char* where) {
+  if (m->src_ac_pts.n == 0 || nsrc <= 0) return;
+  if (!m->ac[f].pot) fatal(where, f ? "backward acoustic field not prepared" : "acoustic field not prepared");
+  if (!m->src_kappa) fatal(where, "prepare_fields_acoustic_device not called");
+  if (nsrc < m->nsources) fatal(where, "NSOURCES smaller than the number of prepared sources");
+  std::vector<realw> s(m->nsources);
+  for (int i = 0; i < m->nsources; i++) s[i] = (realw)h_stf[i];
+  flush_ac(m, f);
+  realw* d = stf_device_buffer(m, m->nsources);
+  stage_to_device(m, s.data(), m->nsources, d);
+  launch_add_sources_ac(m->stream, m->src_ac_pts, m->src_array, m->src_kappa, d, m->ac[f].potdd);
+}
+
 extern "C" void FC_FUNC_(compute_add_sources_ac_cuda, COMPUTE_ADD_SOURCES_AC_CUDA)(long* Mesh_pointer,
                                                                                  int* NSOURCESf,
                                                                                  double* h_stf_pre_compute,
                                                                                  int* run_number_of_the_source) {
   Mesh* m = from_handle(Mesh_pointer);
-  (void)h_stf_pre_compute; (void)run_number_of_the_source;
-  for (int s = 0; s < std::min(*NSOURCESf, m->nsources); s++)
-    if (!m->h_src_elastic[s]) fatal("compute_add_sources_ac_cuda", "sources inside acoustic elements are not supported");
+  (void)run_number_of_the_source;  // NB_RUNS_ACOUSTIC_GPU = 1 (prepare_constants_device refuses others)
+  add_sources_ac(m, 0, h_stf_pre_compute, *NSOURCESf, "compute_add_sources_ac_cuda");
 }
 
 extern "C" void FC_FUNC_(compute_add_sources_ac_s3_cuda, COMPUTE_ADD_SOURCES_AC_s3_CUDA)(long* Mesh_pointer,
                                                                                        int* NSOURCESf,
                                                                                        double* h_stf_pre_compute,
                                                                                        int* run_number_of_the_source) {
-  FC_FUNC_(compute_add_sources_ac_cuda, X)(Mesh_pointer, NSOURCESf, h_stf_pre_compute, run_number_of_the_source);
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)run_number_of_the_source;
+  add_sources_ac(m, 1, h_stf_pre_compute, *NSOURCESf, "compute_add_sources_ac_s3_cuda");
 }
 
+// compute_add_sources_acoustic_cuda.cu (add_sources_ac_sim_2_or_3_cuda): adjoint sources at receivers in
+// acoustic elements, chi'' += adj(1, irec, isample) hxi heta hgamma with the same sample index as the
+// elastic ones
 extern "C" void FC_FUNC_(add_sources_ac_sim_2_or_3_cuda, ADD_SOURCES_AC_SIM_2_OR_3_CUDA)(
     long* Mesh_pointer, realw* h_source_adjoint, int* nrec, int* nadj_rec_local, int* NTSTEP_BETWEEN_READ_ADJSRC,
     int* it) {
   Mesh* m = from_handle(Mesh_pointer);
-  (void)h_source_adjoint; (void)nrec; (void)NTSTEP_BETWEEN_READ_ADJSRC; (void)it;
-  for (int r = 0; r < std::min(*nadj_rec_local, m->nadj_rec_local); r++)
-    if (!m->h_adj_elastic[r]) fatal("add_sources_ac_sim_2_or_3_cuda", "adjoint sources in acoustic elements are not supported");
+  (void)nrec;
+  const int nadj = *nadj_rec_local;
+  if (nadj <= 0 || m->adj_ac_pts.n == 0) return;
+  if (!m->ac[0].pot) fatal("add_sources_ac_sim_2_or_3_cuda", "acoustic field not prepared");
+  const int nt = *NTSTEP_BETWEEN_READ_ADJSRC;
+  const int isample = nt - ((*it - 1) % nt);  // 1-based
+  const realw* src = h_source_adjoint + (int64_t)3 * nadj * (isample - 1);
+  flush_ac(m, 0);
+  stage_to_device(m, src, 3 * nadj, m->adj_src);
+  launch_adjoint_sources_ac(m->stream, m->adj_ac_pts, m->adj_h, m->adj_src, m->ac[0].potdd);
 }
 
 // ---------------------------------------------------------------- seismograms (write_seismograms_cuda.cu:34)
@@ -475,27 +501,56 @@ extern "C" void FC_FUNC_(compute_seismograms_cuda, COMPUTE_SEISMOGRAMS_CUDA)(
     int* seismo_currentf, int* nlength_seismogramf, int* itf, int* it_endf, int* ACOUSTIC_SIMULATION,
     int* ELASTIC_SIMULATION, int* USE_TRICK_FOR_BETTER_PRESSURE) {
   Mesh* m = from_handle(Mesh_pointer_f);
-  (void)ACOUSTIC_SIMULATION; (void)ELASTIC_SIMULATION; (void)USE_TRICK_FOR_BETTER_PRESSURE;
+  (void)ACOUSTIC_SIMULATION; (void)ELASTIC_SIMULATION;
   if (m->nrec_local <= 0) return;
-  for (int r = 0; r < m->nrec_local; r++)
-    if (!m->h_rec_elastic[r]) fatal("compute_seismograms_cuda", "receivers in acoustic elements are not supported");
-  if (m->save_p) fatal("compute_seismograms_cuda", "pressure seismograms are not supported");
+  const bool trick = USE_TRICK_FOR_BETTER_PRESSURE && *USE_TRICK_FOR_BETTER_PRESSURE;
+  if (m->save_p && m->nrec_el > 0)
+    fatal("compute_seismograms_cuda", "pressure seismograms at receivers in elastic elements are not supported");
   const int cur = *seismo_currentf;
   // SIMULATION_TYPE 3 records the reconstructed (backward) field
   const int f = (m->simulation_type == 3 && m->el[1].displ) ? 1 : 0;
-  if (m->save_v || m->save_a) flush_el(m, f);
+  const int fa = (m->simulation_type == 3 && m->ac[1].pot) ? 1 : 0;
   const int64_t off = (int64_t)(cur - 1) * m->nrec_local * 3;
-  if (m->save_d) launch_seismo_el(m->stream, m->nrec_local, m->rec_iglob, m->rec_h, m->rec_nu, m->el[f].displ, m->seismo_d + off);
-  if (m->save_v) launch_seismo_el(m->stream, m->nrec_local, m->rec_iglob, m->rec_h, m->rec_nu, m->el[f].veloc, m->seismo_v + off);
-  if (m->save_a) launch_seismo_el(m->stream, m->nrec_local, m->rec_iglob, m->rec_h, m->rec_nu, m->el[f].accel, m->seismo_a + off);
+  if (m->nrec_el > 0) {
+    if (!m->el[f].displ) fatal("compute_seismograms_cuda", "elastic field not prepared");
+    if (m->save_v || m->save_a) flush_el(m, f);
+    const Wavefield& w = m->el[f];
+    if (m->save_d) launch_seismo_el(m->stream, m->nrec_el, m->rec_el_list, m->rec_iglob, m->rec_h, m->rec_nu, w.displ, m->seismo_d + off);
+    if (m->save_v) launch_seismo_el(m->stream, m->nrec_el, m->rec_el_list, m->rec_iglob, m->rec_h, m->rec_nu, w.veloc, m->seismo_v + off);
+    if (m->save_a) launch_seismo_el(m->stream, m->nrec_el, m->rec_el_list, m->rec_iglob, m->rec_h, m->rec_nu, w.accel, m->seismo_a + off);
+  }
+  if (m->nrec_ac > 0) {
+    // receivers in fluid elements (write_seismograms_cuda.cu: seismotype 1-3 from the gradient of the
+    // potentials, 4 = pressure from the potential with the pressure trick, else its second derivative)
+    const AcField& a = m->ac[fa];
+    if (!a.pot) fatal("compute_seismograms_cuda", "acoustic field not prepared");
+    if (m->save_v || m->save_a || (m->save_p && !trick)) flush_ac(m, fa);
+    AcReceiverArgs ra;
+    ra.n = m->nrec_ac;
+    ra.list = m->rec_ac_list;
+    ra.rec_ispec = m->rec_ispec;
+    ra.iglob = m->rec_iglob;
+    ra.h = m->rec_h;
+    ra.nu = m->rec_nu;
+    ra.irregular = m->irregular;
+    ra.xix_regular = m->xix_regular;
+    ra.jac = m->jac;
+    ra.jac_plane = (int64_t)m->nspec_irregular * 125;
+    ra.rho = m->rho;
+    if (m->save_d) launch_seismo_ac(m->stream, ra, a.pot, 0, m->seismo_d + off);
+    if (m->save_v) launch_seismo_ac(m->stream, ra, a.potd, 0, m->seismo_v + off);
+    if (m->save_a) launch_seismo_ac(m->stream, ra, a.potdd, 0, m->seismo_a + off);
+    if (m->save_p) launch_seismo_ac(m->stream, ra, trick ? a.pot : a.potdd, 1, m->seismo_p + off);
+  }
   if (cur == *nlength_seismogramf || *itf == *it_endf) {
+    // host arrays seismograms_?(NDIM, nrec_local, nlength); pressure in component 1, the others zero
     const size_t bytes = sizeof(realw) * (size_t)cur * m->nrec_local * 3;
     if (m->save_d) SF_CHECK(hipMemcpyAsync(seismograms_d, m->seismo_d, bytes, hipMemcpyDeviceToHost, m->stream));
     if (m->save_v) SF_CHECK(hipMemcpyAsync(seismograms_v, m->seismo_v, bytes, hipMemcpyDeviceToHost, m->stream));
     if (m->save_a) SF_CHECK(hipMemcpyAsync(seismograms_a, m->seismo_a, bytes, hipMemcpyDeviceToHost, m->stream));
+    if (m->save_p) SF_CHECK(hipMemcpyAsync(seismograms_p, m->seismo_p, bytes, hipMemcpyDeviceToHost, m->stream));
     SF_CHECK(hipStreamSynchronize(m->stream));
   }
-  (void)seismograms_p;
 }
 
 // ---------------------------------------------------------------- kernels (compute_kernels_cuda.cu:39)

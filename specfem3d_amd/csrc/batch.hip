@@ -281,7 +281,11 @@ static T* bupload(const T* h, int64_t n) {
 }
 
 static void drop_graph(Batch* b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  // a replay of the executable graph may still run on the stream: wait for it before destroying
+  if (b->gexec) {
+    SF_CHECK(hipStreamSynchronize(b->m->stream));
+    (void)hipGraphExecDestroy(b->gexec);
+  }
   b->gexec = nullptr;
   b->graph_steps = 0;
 }

@@ -94,6 +94,67 @@ __global__ void __launch_bounds__(128) k_compute_kernels_ac(int nelem, const int
   kappa_ac_kl[ep] = kappa_ac_kl[ep] + deltat * sa[p] * b_potdd[g];
 }
 
+// receivers in acoustic elements (compute_seismograms.f90:127-144, compute_interpolated_dva_acoust
+// compute_interpolated_dva.f90:219-328), one workgroup per listed receiver, FP64 sums like the CPU path:
+//   pressure: p = -sum_ijk h_ijk field(ibool)   (field = potential with the pressure trick, else
+//             potential_dot_dot); stored in component 0, components 1 and 2 zero;
+//   vector:   the receiver element's (1/rho) grad(field) at its GLL points
+//             (compute_gradient_in_acoustic_fast_Deville), interpolated and rotated by nu_rec.
+__global__ void __launch_bounds__(128) k_seismo_ac(AcReceiverArgs A, GLLk G, const realw* __restrict__ field,
+                                                   int pressure, realw* __restrict__ out) {
+  __shared__ realw s[125], hp[25];
+  __shared__ double red[2][3];
+  const int p = threadIdx.x;
+  const int r = A.list ? A.list[blockIdx.x] : (int)blockIdx.x;
+  if (p < 25) hp[p] = G.hp[p];
+  const int q = p < 125 ? p : 124;
+  const int g = A.iglob[(int64_t)r * 125 + q];
+  if (p < 125) s[p] = field[g];
+  __syncthreads();
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  if (p < 125) {
+    const double hl = A.h[(int64_t)r * 125 + p];
+    if (pressure) {
+      v0 = -(hl * (double)s[p]);
+    } else {
+      const int e = A.rec_ispec[r];
+      const int k = p / 25, j = (p / 5) % 5, i = p % 5;
+      const int irr = A.irregular[e];
+      const int64_t qq = (int64_t)(irr > 0 ? irr - 1 : 0) * 125 + p;
+      const realw rho_invl = (realw)1 / A.rho[(int64_t)e * 125 + p];
+      realw gv[3];
+      ac_gradient(s, hp, i, j, k, irr, A.jac, A.jac_plane, qq, A.xix_regular, rho_invl, gv);
+      v0 = hl * (double)gv[0];
+      v1 = hl * (double)gv[1];
+      v2 = hl * (double)gv[2];
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_xor(v0, off, 64);
+    v1 += __shfl_xor(v1, off, 64);
+    v2 += __shfl_xor(v2, off, 64);
+  }
+  if ((p & 63) == 0) {
+    red[p >> 6][0] = v0;
+    red[p >> 6][1] = v1;
+    red[p >> 6][2] = v2;
+  }
+  __syncthreads();
+  if (p != 0) return;
+  const double x = red[0][0] + red[1][0], y = red[0][1] + red[1][1], z = red[0][2] + red[1][2];
+  if (pressure) {
+    out[3 * r] = (realw)x;
+    out[3 * r + 1] = 0.f;
+    out[3 * r + 2] = 0.f;
+  } else {
+    const double* n = A.nu + (int64_t)r * 9;
+    out[3 * r] = (realw)(n[0] * x + n[1] * y + n[2] * z);
+    out[3 * r + 1] = (realw)(n[3] * x + n[4] * y + n[5] * z);
+    out[3 * r + 2] = (realw)(n[6] * x + n[7] * y + n[8] * z);
+  }
+}
+
 // ordered assembly: for each global point, subtract its element contributions in increasing
 // element order (the order of the reference's serial element loop). ptr/idx is a CSR list of
 // local slots e*125+p per global point, sorted by element.
@@ -155,6 +216,12 @@ void launch_compute_kernels_ac(hipStream_t s, int nelem, const int* elems, const
                      xix_regular, jac, jac_plane, rhostore, pot, b_pot, b_potdd, g_gll_host, deltat, rho_ac_kl,
                      kappa_ac_kl);
   SF_LAUNCH_CHECK("k_compute_kernels_ac");
+}
+
+void launch_seismo_ac(hipStream_t s, const AcReceiverArgs& a, const realw* field, int pressure, realw* out) {
+  if (a.n <= 0) return;
+  hipLaunchKernelGGL(k_seismo_ac, dim3((unsigned)a.n), dim3(128), 0, s, a, g_gll_host, field, pressure, out);
+  SF_LAUNCH_CHECK("k_seismo_ac");
 }
 
 void launch_ordered_gather(hipStream_t s, int nglob, int ncomp, const int* ptr, const int* idx, const realw* local,

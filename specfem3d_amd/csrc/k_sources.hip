@@ -49,13 +49,26 @@ __global__ void k_adjoint_sources_el(int n, const int* __restrict__ ig, const in
   accel[3 * g + 2] = accel[3 * g + 2] + adj[3 * r + 2] * hl;
 }
 
+// compute_add_sources_acoustic.f90:205-235 (adjoint sources in fluid elements):
+// chi'' += adj(1, irec) * hxi heta hgamma, the product of the CUSTOM_REAL interpolants, added as is
+// (no 1/kappa, positive sign)
+__global__ void k_adjoint_sources_ac(int n, const int* __restrict__ ig, const int* __restrict__ owner,
+                                     const int* __restrict__ ploc, const realw* __restrict__ h,
+                                     const realw* __restrict__ adj, realw* potdd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int g = ig[t];
+  const int r = owner[t];
+  potdd[g] = potdd[g] + adj[3 * r] * h[(int64_t)r * 125 + ploc[t]];
+}
+
 // receiver interpolation (compute_interpolated_dva.f90:84-100): double-precision sum of
-// h_i h_j h_k u over the 125 element points, rotated by nu_rec; one wave64 per receiver.
-__global__ void __launch_bounds__(64) k_seismo_el(int nrec, const int* __restrict__ ig, const double* __restrict__ h,
-                                                  const double* __restrict__ nu, const realw* __restrict__ field,
-                                                  realw* __restrict__ out) {
-  const int r = blockIdx.x;
-  if (r >= nrec) return;
+// h_i h_j h_k u over the 125 element points, rotated by nu_rec; one wave64 per listed receiver.
+__global__ void __launch_bounds__(64) k_seismo_el(int n, const int* __restrict__ list, const int* __restrict__ ig,
+                                                  const double* __restrict__ h, const double* __restrict__ nu,
+                                                  const realw* __restrict__ field, realw* __restrict__ out) {
+  if ((int)blockIdx.x >= n) return;
+  const int r = list ? list[blockIdx.x] : (int)blockIdx.x;
   const int lane = threadIdx.x;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   for (int p = lane; p < 125; p += 64) {
@@ -110,10 +123,20 @@ void launch_adjoint_sources_el(hipStream_t s, const FacePoints& f, const realw* 
   SF_LAUNCH_CHECK("k_adjoint_sources_el");
 }
 
-void launch_seismo_el(hipStream_t s, int nrec, const int* iglob, const double* h, const double* nu,
+void launch_adjoint_sources_ac(hipStream_t s, const FacePoints& f, const realw* h, const realw* adj, realw* potdd) {
+  for (int r = 0; r < f.nrounds; r++) {
+    const int o = f.round_off[r], cnt = f.round_off[r + 1] - o;
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(k_adjoint_sources_ac, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o, f.ispec + o,
+                       f.ploc + o, h, adj, potdd);
+  }
+  SF_LAUNCH_CHECK("k_adjoint_sources_ac");
+}
+
+void launch_seismo_el(hipStream_t s, int n, const int* list, const int* iglob, const double* h, const double* nu,
                       const realw* field, realw* out) {
-  if (nrec <= 0) return;
-  hipLaunchKernelGGL(k_seismo_el, dim3(nrec), dim3(64), 0, s, nrec, iglob, h, nu, field, out);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_seismo_el, dim3(n), dim3(64), 0, s, n, list, iglob, h, nu, field, out);
   SF_LAUNCH_CHECK("k_seismo_el");
 }
 

@@ -145,8 +145,27 @@ void launch_add_sources_el(hipStream_t s, const FacePoints& f, const realw* arr,
 void launch_add_sources_ac(hipStream_t s, const FacePoints& f, const realw* arr, const realw* kappa,
                            const realw* stf, realw* potdd);
 void launch_adjoint_sources_el(hipStream_t s, const FacePoints& f, const realw* h, const realw* adj, realw* accel);
-void launch_seismo_el(hipStream_t s, int nrec, const int* iglob, const double* h, const double* nu,
+void launch_adjoint_sources_ac(hipStream_t s, const FacePoints& f, const realw* h, const realw* adj, realw* potdd);
+// receivers: `list` holds the local receiver indices to record (n of them; null: receivers 0..n-1);
+// out[3 r + c] is written for each listed receiver r
+void launch_seismo_el(hipStream_t s, int n, const int* list, const int* iglob, const double* h, const double* nu,
                       const realw* field, realw* out);
+// receivers in acoustic elements: pressure = 1 records -sum h field into out[3 r] (out[3 r + 1..2] = 0),
+// pressure = 0 the vector (1/rho) grad(field) interpolated and rotated by nu
+struct AcReceiverArgs {
+  int n;
+  const int* list;
+  const int* rec_ispec;
+  const int* iglob;        // nrec*125 global points of the receiver elements
+  const double* h;         // nrec*125 interpolation weights
+  const double* nu;        // nrec*9
+  const int* irregular;
+  realw xix_regular;
+  const realw* jac;
+  int64_t jac_plane;
+  const realw* rho;        // rhostore, nloc
+};
+void launch_seismo_ac(hipStream_t s, const AcReceiverArgs& a, const realw* field, int pressure, realw* out);
 
 // kernels / norms (k_kernels.hip)
 void launch_compute_kernels_el(hipStream_t s, int nelem, const int* elems, const int* ibool, const realw* accel,

@@ -7,7 +7,10 @@
 //   * kappa/mu: one float2 per local point so each point needs one 8-byte load;
 //   * irregular Jacobians: 10 SoA planes [t][nspec_irr*125] (xix..gammaz, jacobian) so a
 //     wavefront's 64 lanes read 64 consecutive floats of one plane;
-//   * attenuation memory: SoA planes [comp*3 + sls][nspec*125], comp = xx,yy,xy,xz,yz,trace;
+//   * attenuation memory variables and stored strain: per-element blocks, the 18 memory-variable
+//     planes (comp*3 + sls, comp = xx,yy,xy,xz,yz,trace) of one element contiguous, then the next
+//     element: value (e, plane, p) at (e*18 + plane)*125 + p (r_idx in kernels.h); strain likewise
+//     with 6 planes xx,yy,xy,xz,yz,trace at (e*6 + c)*125 + p (eps_idx);
 //   * inverse mass: float[3*nglob] interleaved (rmassx,rmassy,rmassz) so the corrector is a
 //     pure float4 stream over the flat 3*nglob arrays;
 //   * boundary faces are flattened to face points with the global index, normal, weight and
@@ -33,8 +36,8 @@ struct AcField {
 };
 
 struct AttField {            // memory variables + stored strain of one elastic wavefield
-  realw* R = nullptr;        // 18 planes of nloc (comp-major, sls-minor)
-  realw* eps = nullptr;      // 6 planes of nloc: xx,yy,xy,xz,yz,trace
+  realw* R = nullptr;        // 18 * nloc: per-element blocks of 18 planes (r_idx in kernels.h)
+  realw* eps = nullptr;      // 6 * nloc: per-element blocks of 6 planes xx,yy,xy,xz,yz,trace (eps_idx)
 };
 
 struct FacePoints {          // flattened boundary/coupling face points
@@ -126,10 +129,11 @@ struct Mesh {
   // sources
   int nsources = 0;
   FacePoints src_pts;              // (source, point) pairs of sources in elastic elements
+  FacePoints src_ac_pts;           // (source, point) pairs of sources in acoustic elements
   realw* src_array = nullptr;      // nsrc*125*3
   unsigned char* h_src_elastic = nullptr;
   unsigned char* h_src_acoustic = nullptr;
-  realw* src_kappa = nullptr;      // nsrc*125 kappa at source points (acoustic sources)
+  realw* src_kappa = nullptr;      // nsrc*125 kappastore at the source element's points (acoustic sources)
 
   // receivers
   int nrec_local = 0;
@@ -144,12 +148,17 @@ struct Mesh {
   int save_d = 0, save_v = 0, save_a = 0, save_p = 0;
   unsigned char* h_rec_elastic = nullptr;
   int* h_rec_ispec = nullptr;
+  int* rec_ispec = nullptr;        // device: element of each local receiver (0-based)
+  int nrec_el = 0, nrec_ac = 0;    // local receivers in elastic / acoustic elements ...
+  int* rec_el_list = nullptr;      // ... and their local receiver indices (device)
+  int* rec_ac_list = nullptr;
 
   // adjoint sources (SIMULATION_TYPE 2/3)
   int nadj_rec_local = 0;
   int ntstep_between_read_adjsrc = 0;
   realw* adj_h = nullptr;          // nadj*125 float products (CUSTOM_REAL hlagrange)
-  FacePoints adj_pts;              // (adjoint receiver, point) pairs
+  FacePoints adj_pts;              // (adjoint receiver, point) pairs in elastic elements
+  FacePoints adj_ac_pts;           // (adjoint receiver, point) pairs in acoustic elements
   realw* adj_src = nullptr;        // one step of adjoint sources (3*nadj)
   unsigned char* h_adj_elastic = nullptr;
 
@@ -162,8 +171,13 @@ struct Mesh {
 
   // execution
   hipStream_t stream = nullptr;
-  int exact = 1;                   // 1: kernels built without fused multiply-add (oracle order)
-  int assembly = 0;                // 0 atomic scatter-add, 1 ordered gather (oracle order), 2 coloured
+  // Defaults for a host that never calls sf_set_options (the Fortran drop-in): the FMA build and,
+  // once prepare_fields_elastic_device knows whether attenuation is on, coloured assembly without it
+  // (float atomics are the limit of the light elastic kernel) and atomic assembly with it (the
+  // colour launches re-read more state than the atomics write back).
+  int exact = 0;                   // 1: kernels built without fused multiply-add (oracle order)
+  int assembly = -1;               // 0 atomic scatter-add, 1 ordered gather (oracle order), 2 coloured,
+                                   // -1 automatic (resolved at prepare_fields_elastic_device)
   int elastic_shape = 1;           // 0 column-per-lane kernel (5 elements/WG), 1 point-per-lane
   int att_variant = 4;             // attenuation-state scheduling of the point kernel (0..5)
   int pipe_wgs = 0;                // persistent workgroups of the pipelined kernels (0: their default)

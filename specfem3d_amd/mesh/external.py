@@ -164,5 +164,28 @@ def build_from_arrays(d, dt: float, stacey: bool = True) -> Database:
         if len(isp):
             ijk, nrm, jw = face_geometry(anchors, isp, ifc, xstore, ystore, zstore, ibool)
             db.free_faces = Faces(isp, ijk, nrm, jw)
+    if is_ac.any() and (~is_ac).any():
+        isp, ifc = coupling_faces(conn, is_ac)
+        if len(isp):
+            ijk, nrm, jw = face_geometry(anchors, isp, ifc, xstore, ystore, zstore, ibool)
+            db.coupling_faces = Faces(isp, ijk, nrm, jw)
     db.finalize(dt, stacey=stacey)
     return db
+
+
+def coupling_faces(conn: np.ndarray, is_acoustic: np.ndarray):
+    """Fluid-solid coupling faces (``get_coupling_surfaces_ac_el``, get_coupling_surfaces.f90:197-405):
+    every face of an acoustic element that it shares with an elastic element, listed once on the
+    acoustic element in element order (faces 1..6 per element); returns (ispec, face id). The normals
+    (from ``face_geometry``) point out of the acoustic element, i.e. away from the fluid."""
+    el_faces = set()
+    for e in np.nonzero(~is_acoustic)[0]:
+        for idx in FACE_ANCHORS.values():
+            el_faces.add(tuple(sorted(int(conn[e, k]) for k in idx)))
+    isp, ifc = [], []
+    for e in np.nonzero(is_acoustic)[0]:
+        for fid, idx in FACE_ANCHORS.items():
+            if tuple(sorted(int(conn[e, k]) for k in idx)) in el_faces:
+                isp.append(int(e))
+                ifc.append(fid)
+    return np.array(isp, np.int32), np.array(ifc, np.int32)

@@ -78,7 +78,7 @@ class GPUSolver:
                  attenuation=None, exact: bool = True, assembly: int = 0, simulation_type: int = 1,
                  save_forward: bool = False, save_seismograms=("d",), nlength: int | None = None,
                  device: int = 0, tuning: dict | None = None, undo_attenuation: bool = False,
-                 element_order: str | None = None):
+                 element_order: str | None = None, pressure_trick: bool = False):
         self.L = abi.load()
         self.db = db
         self.sources = sources
@@ -97,6 +97,9 @@ class GPUSolver:
         self.tuning = dict(tuning or {})
         self.undo_att = bool(undo_attenuation)
         self.element_order = element_order
+        # USE_TRICK_FOR_BETTER_PRESSURE: pressure recorded as -potential (the fluid sources then use the
+        # second derivative of their STF, see sources.setup_cmt_sources)
+        self.pressure_trick = bool(pressure_trick)
         self._prepare(exact, assembly, save_forward, device)
 
     # ------------------------------------------------------------------------------ set-up
@@ -154,7 +157,7 @@ class GPUSolver:
             A.ia(db.ispec_is_acoustic.astype(np.int32)), A.ia(db.ispec_is_elastic.astype(np.int32)),
             A.i(0), A.i(int(save_forward)), A.fa(hx.ravel()), A.fa(hy.ravel()), A.fa(hz.ravel()),
             A.da(nu.ravel()), A.da(np.zeros(9 * max(nsrc, 1))), A.ia(np.zeros(max(nrec, 1), np.int32)),
-            A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(0), A.i(1), A.i(0),
+            A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(sv("p")), A.i(1), A.i(0),
             A.i(int(self.undo_att)), A.i(0))
         stage("prepare_constants_device")
         L.sf_set_options(self.MP, C.c_int(int(exact)), C.c_int(int(assembly)))
@@ -243,7 +246,7 @@ class GPUSolver:
         self.seis_d = np.zeros((self.nlength, nrec, 3), np.float32)
         self.seis_v = np.zeros((self.nlength, nrec, 3), np.float32)
         self.seis_a = np.zeros((self.nlength, nrec, 3), np.float32)
-        self.seis_p = np.zeros(1, np.float32)
+        self.seis_p = np.zeros((self.nlength, nrec, 3), np.float32)
         # constants (prepare_timerun.F90:389-441)
         dt = np.float32(db.dt)
         self.deltat = dt
@@ -280,6 +283,12 @@ class GPUSolver:
                     L.compute_stacey_acoustic_cuda_(MP, C.byref(ip), None, one)
                     if self.elastic:
                         L.compute_coupling_ac_el_cuda_(MP, C.byref(ip), C.byref(ncf), one)
+                    if self.stf_table is not None:
+                        # compute_add_sources_acoustic_GPU: sources in fluid elements (the library adds
+                        # only those; the elastic ones come with the elastic forces)
+                        stf = np.ascontiguousarray(self.stf_table[it - 1], np.float64)
+                        L.compute_add_sources_ac_cuda_(MP, C.byref(C.c_int(len(stf))), stf.ctypes.data_as(C.c_void_p),
+                                                       zero)
             L.kernel_3_acoustic_cuda_(MP, dt2, b_dt2, one)
             L.acoustic_enforce_free_surf_cuda_(MP, zero, one)
         if self.elastic:
@@ -302,13 +311,18 @@ class GPUSolver:
                 MP, abi.ptr(self.seis_d), abi.ptr(self.seis_v), abi.ptr(self.seis_a), abi.ptr(self.seis_p),
                 C.byref(C.c_int(self.seismo_current)), C.byref(C.c_int(self.nlength)), C.byref(C.c_int(it)),
                 C.byref(C.c_int(self.nstep)), C.byref(C.c_int(int(self.acoustic))),
-                C.byref(C.c_int(int(self.elastic))), zero)
+                C.byref(C.c_int(int(self.elastic))), C.byref(C.c_int(int(self.pressure_trick))))
 
     def run(self, nstep: int | None = None):
+        """Step ``nstep`` times; returns the first recorded kind of seismograms (nstep, nrec, 3)."""
         n = self.nstep if nstep is None else nstep
         for _ in range(n):
             self.step()
-        return self.seis_d[: self.seismo_current]
+        return self.seismograms(self.save[0] if self.save else "d")
+
+    def seismograms(self, kind: str = "d") -> np.ndarray:
+        """Recorded seismograms of one kind ("d", "v", "a" or "p"; pressure in component 0)."""
+        return {"d": self.seis_d, "v": self.seis_v, "a": self.seis_a, "p": self.seis_p}[kind][: self.seismo_current]
 
     def synchronize(self):
         self.L.sf_synchronize(self.MP)

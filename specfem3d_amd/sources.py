@@ -54,6 +54,29 @@ def stf_gaussian(t, hdur):
     return np.exp(-a * t ** 2) / (math.sqrt(math.pi) * hdur)
 
 
+def stf_d2gaussian(t, hdur):
+    """Second time derivative of ``stf_gaussian``: ``2 a (2 a t^2 - 1) exp(-a t^2) / (sqrt(pi) h)``,
+    ``a = 1/h^2`` (``comp_source_time_function_d2gau``)."""
+    t = np.asarray(t, np.float64)
+    a = 1.0 / hdur ** 2
+    return 2.0 * a * (-1.0 + 2.0 * a * t ** 2) * np.exp(-a * t ** 2) / (math.sqrt(math.pi) * hdur)
+
+
+def stf_d2ricker(t, f0):
+    """Second time derivative of ``stf_ricker``: ``-2 a (3 - 12 a t^2 + 4 a^2 t^4) exp(-a t^2)``
+    (``comp_source_time_function_d2rck``)."""
+    t = np.asarray(t, np.float64)
+    a = math.pi ** 2 * np.asarray(f0, np.float64) ** 2
+    return -2.0 * a * (3.0 - 12.0 * a * t * t + 4.0 * a ** 2 * t * t * t * t) * np.exp(-a * t * t)
+
+
+_STF = {"ricker": lambda t, h: stf_ricker(t, h),
+        "d2ricker": lambda t, h: stf_d2ricker(t, h),
+        "heaviside": lambda t, h: stf_heaviside(t, h / SOURCE_DECAY_MIMIC_TRIANGLE),
+        "gaussian": lambda t, h: stf_gaussian(t, h / SOURCE_DECAY_MIMIC_TRIANGLE),
+        "d2gaussian": lambda t, h: stf_d2gaussian(t, h / SOURCE_DECAY_MIMIC_TRIANGLE)}
+
+
 # ----------------------------------------------------------------------------- inputs
 @dataclass
 class CMTSource:
@@ -114,6 +137,46 @@ def read_cmtsolution(path, DT: float) -> list[CMTSource]:
         m = min(s.tshift for s in out)
         for s in out:
             s.tshift -= m
+    return out
+
+
+@dataclass
+class ForceSource:
+    """One FORCESOLUTION block (``get_force.f90``): position, STF type and force vector."""
+    x: float                    # longorUTM
+    y: float                    # latorUTM
+    depth: float                # km below the surface, or the z coordinate (m) with USE_SOURCES_RECEIVERS_Z
+    tshift: float
+    hdur: float                 # half duration (Gaussian, Heaviside) or dominant frequency (Ricker)
+    force_stf: int              # 0 Gaussian, 1 Ricker, 2 Heaviside
+    factor: float               # factor force source (N)
+    e: float                    # component dir vect source E, N, Z_UP
+    n: float
+    z_up: float
+
+
+def _fortran_float(v: str) -> float:
+    return float(v.strip().lower().replace("d", "e"))
+
+
+def read_forcesolution(path) -> list[ForceSource]:
+    """Parse a FORCESOLUTION file (path or file object; one 11-line block per force)."""
+    lines = [l for l in _lines(path) if l.strip()]
+    out = []
+    for b in range(0, len(lines), 11):
+        vals = {}
+        for l in lines[b + 1:b + 11]:
+            k, v = l.split(":", 1)
+            vals[k.strip().lower()] = v.strip()
+        out.append(ForceSource(x=_fortran_float(vals["longorutm"]), y=_fortran_float(vals["latorutm"]),
+                               depth=_fortran_float(vals["depth"]), tshift=_fortran_float(vals["time shift"]),
+                               hdur=_fortran_float(vals["hdurorf0"]), force_stf=int(vals["source time function"]),
+                               factor=_fortran_float(vals["factor force source"]),
+                               e=_fortran_float(vals["component dir vect source e"]),
+                               n=_fortran_float(vals["component dir vect source n"]),
+                               z_up=_fortran_float(vals["component dir vect source z_up"])))
+    if len(out) == 1:
+        out[0].tshift = 0.0
     return out
 
 
@@ -305,6 +368,28 @@ def cmt_source_array(db: Database, loc: Located, src: CMTSource) -> np.ndarray:
     return out.reshape(125, 3).astype(np.float32)
 
 
+def acoustic_source_array(loc: Located, factor: float) -> np.ndarray:
+    """(125, 3) float32 source array of a source in a fluid element: ``factor h_i h_j h_k`` in all three
+    components (``compute_arrays_source_forcesolution`` with comp_x = comp_y = comp_z = 1 and the identity
+    ``nu_source`` of Cartesian meshes, ``setup_sources_receivers.f90:1150-1212``); only the first
+    component is used."""
+    x = get_gll().xigll
+    hx, _ = lagrange_any(loc.xi, x)
+    hy, _ = lagrange_any(loc.eta, x)
+    hz, _ = lagrange_any(loc.gamma, x)
+    h = ((hx[None, None, :] * hy[None, :, None]) * hz[:, None, None]).ravel()
+    v = (factor * h) * 1.0
+    return np.repeat(v[:, None], 3, axis=1).astype(np.float32)
+
+
+def cmt_acoustic_factor(src: "CMTSource") -> float:
+    """Scalar moment of a CMT source in a fluid element scaled to the equivalent explosion
+    (``setup_sources_receivers.f90:1190-1198``): M0 sqrt(2)/sqrt(3), M0 = sqrt(sum M_ij^2 / 2)."""
+    m0 = 1.0 / math.sqrt(2.0) * math.sqrt(src.Mxx ** 2 + src.Myy ** 2 + src.Mzz ** 2
+                                          + 2.0 * (src.Myz ** 2 + src.Mxz ** 2 + src.Mxy ** 2))
+    return m0 * math.sqrt(2.0) / math.sqrt(3.0)
+
+
 def force_source_array(loc: Located, fx: float, fy: float, fz: float, factor: float) -> np.ndarray:
     """Point-force source array ``factor * f * h_i h_j h_k`` (float64 -> float32)."""
     x = get_gll().xigll
@@ -318,7 +403,11 @@ def force_source_array(loc: Located, fx: float, fy: float, fz: float, factor: fl
 
 @dataclass
 class SourceSet:
-    """Everything the loop needs about the sources: arrays + STF per step."""
+    """Everything the loop needs about the sources: arrays + STF per step.
+
+    ``kind`` is the STF of sources in elastic elements (``get_stf_viscoelastic``) and ``kind_acoustic``
+    that of sources in fluid elements (``get_stf_acoustic``: a Gaussian, or with the pressure trick its
+    second derivative, for CMT sources without Ricker); ``acoustic[s]`` tells which applies."""
     ispec: np.ndarray               # (nsrc,) int32 0-based
     sourcearrays: np.ndarray        # (nsrc, 125, 3) float32
     kind: str                       # "ricker" | "heaviside" | "gaussian"
@@ -327,57 +416,115 @@ class SourceSet:
     t0: float
     DT: float
     located: list = field(default_factory=list)
+    acoustic: np.ndarray | None = None   # (nsrc,) bool: source element in a fluid
+    kind_acoustic: str = "gaussian"      # "gaussian" | "d2gaussian" | "ricker" | "d2ricker"
+    kinds: list | None = None            # per-source STF kinds (point forces), overriding the two above
 
     @property
     def n(self) -> int:
         return int(self.ispec.shape[0])
 
+    def _kinds(self):
+        if self.kinds is not None:
+            return list(self.kinds)
+        ac = self.acoustic if self.acoustic is not None else np.zeros(self.n, bool)
+        return [self.kind_acoustic if a else self.kind for a in ac]
+
     def stf(self, it: int) -> np.ndarray:
         """float64 STF values at step ``it`` (1-based), ``t = (it-1) DT - t0 - tshift``."""
-        t = (it - 1) * self.DT - self.t0 - self.tshift
-        if self.kind == "ricker":
-            return stf_ricker(t, self.hdur)
-        if self.kind == "heaviside":
-            return stf_heaviside(t, self.hdur / SOURCE_DECAY_MIMIC_TRIANGLE)
-        if self.kind == "gaussian":
-            return stf_gaussian(t, self.hdur / SOURCE_DECAY_MIMIC_TRIANGLE)
-        raise ValueError(self.kind)
+        t = (it - 1.0) * self.DT - self.t0 - self.tshift
+        return np.array([_STF[k](t[s], self.hdur[s]) for s, k in enumerate(self._kinds())], np.float64)
 
     def stf_table(self, nstep: int) -> np.ndarray:
         """(nstep, nsrc) float64 table of STF values for it = 1..nstep."""
         its = np.arange(1, nstep + 1, dtype=np.float64)
         t = (its[:, None] - 1.0) * self.DT - self.t0 - self.tshift[None, :]
-        if self.kind == "ricker":
-            a = math.pi ** 2 * self.hdur[None, :] ** 2
-            return (1.0 - 2.0 * a * t * t) * np.exp(-a * t * t)
-        if self.kind == "heaviside":
-            return 0.5 * (1.0 + erf(t / (self.hdur[None, :] / SOURCE_DECAY_MIMIC_TRIANGLE)))
-        if self.kind == "gaussian":
-            h = self.hdur[None, :] / SOURCE_DECAY_MIMIC_TRIANGLE
-            return np.exp(-(t ** 2) / h ** 2) / (math.sqrt(math.pi) * h)
-        raise ValueError(self.kind)
+        out = np.empty((nstep, self.n), np.float64)
+        for s, k in enumerate(self._kinds()):
+            if k not in _STF:
+                raise ValueError(k)
+            out[:, s] = _STF[k](t[:, s], self.hdur[s])
+        return out
 
 
 def setup_cmt_sources(db: Database, cmts: list[CMTSource], DT: float, ricker: bool = False,
-                      tie_break: str = "upper") -> SourceSet:
-    """Locate CMT sources and build source arrays, STF kind and t0."""
+                      tie_break: str = "upper", pressure_trick: bool = False) -> SourceSet:
+    """Locate CMT sources and build source arrays, STF kinds and t0.
+
+    Sources in elastic elements get ``compute_arrays_source_cmt``; sources in fluid elements the
+    scalar-moment pressure source (``setup_sources_receivers.f90:1188-1212``) with the Gaussian STF (or
+    Ricker), or its second derivative when ``pressure_trick`` (USE_TRICK_FOR_BETTER_PRESSURE) is set.
+    t0 (``setup_stf_constants``, ``setup_sources_receivers.f90:391-520``): -1.2 min(tshift - 1/f0) for
+    Ricker, else -2 min(tshift - hdur), raised to -3 (tshift - hdur) by a Gaussian source in a fluid."""
     loc = Locator(db)
-    isp, arrs, located = [], [], []
+    isp, arrs, located, acoustic = [], [], [], []
     for s in cmts:
         ztop = surface_z(db, s.x, s.y)
         L = loc.locate(s.x, s.y, ztop - s.depth_km * 1000.0, tie_break=tie_break)
         isp.append(L.ispec)
-        arrs.append(cmt_source_array(db, L, s))
+        ac = bool(db.ispec_is_acoustic[L.ispec])
+        acoustic.append(ac)
+        arrs.append(acoustic_source_array(L, cmt_acoustic_factor(s)) if ac else cmt_source_array(db, L, s))
         located.append(L)
     hdur = np.array([s.hdur for s in cmts], np.float64)
     tshift = np.array([s.tshift for s in cmts], np.float64)
+    acoustic = np.array(acoustic, bool)
     if ricker:
         t0 = -1.2 * float(np.min(tshift - 1.0 / hdur))
         kind = "ricker"
+        kind_ac = "d2ricker" if pressure_trick else "ricker"
     else:
         t0 = -2.0 * float(np.min(tshift - hdur))
         kind = "heaviside"
-    return SourceSet(np.array(isp, np.int32), np.stack(arrs), kind, hdur, tshift, t0, DT, located)
+        kind_ac = "d2gaussian" if pressure_trick else "gaussian"
+        for s in np.nonzero(acoustic)[0]:
+            t0 = max(t0, -3.0 * (tshift[s] - hdur[s]))
+    return SourceSet(np.array(isp, np.int32), np.stack(arrs), kind, hdur, tshift, t0, DT, located,
+                     acoustic=acoustic, kind_acoustic=kind_ac)
+
+
+def setup_force_sources(db: Database, forces: list[ForceSource], DT: float, ricker: bool = False,
+                        z_coordinates: bool = False, pressure_trick: bool = False) -> SourceSet:
+    """Point forces (USE_FORCE_POINT_SOURCE): located like CMT sources (``z_coordinates``: the depth
+    field is the z coordinate, USE_SOURCES_RECEIVERS_Z); source arrays
+    (``setup_sources_receivers.f90:1100-1160``): in elastic elements factor h_ijk times the normalised
+    (E, N, Z_UP) vector, in fluid elements factor h_ijk; STF per ``force_stf`` (Ricker for all when
+    ``ricker``, as USE_RICKER_TIME_FUNCTION does in get_force.f90:223), with the pressure trick's second
+    derivatives in fluids; t0 (``setup_stf_constants``): -min over sources of 1.5 (tshift - hdur)
+    (Gaussian, Heaviside) or 1.2 (tshift - 1/f0) (Ricker), raised to -3 (tshift - hdur) by a Gaussian
+    force in a fluid."""
+    loc = Locator(db)
+    isp, arrs, located, acoustic, kinds = [], [], [], [], []
+    t0 = 0.0
+    for f in forces:
+        z = f.depth if z_coordinates else surface_z(db, f.x, f.y) - f.depth * 1000.0
+        L = loc.locate(f.x, f.y, z)
+        ac = bool(db.ispec_is_acoustic[L.ispec])
+        stf = 1 if ricker else f.force_stf
+        if ac:
+            arr = acoustic_source_array(L, f.factor)
+        else:
+            norm = math.sqrt(f.e ** 2 + f.n ** 2 + f.z_up ** 2)
+            if norm < 1e-30:
+                raise ValueError("force point source: component vector has (almost) zero norm")
+            arr = force_source_array(L, f.e / norm, f.n / norm, f.z_up / norm, f.factor)
+        base = {0: "gaussian", 1: "ricker", 2: "heaviside"}.get(stf)
+        if base is None:
+            raise ValueError(f"unsupported force source time function {stf}")
+        kinds.append(("d2" + base) if (ac and pressure_trick and base != "heaviside") else base)
+        t0 = min(t0, 1.2 * (f.tshift - 1.0 / f.hdur) if stf == 1 else 1.5 * (f.tshift - f.hdur))
+        isp.append(L.ispec)
+        arrs.append(arr)
+        located.append(L)
+        acoustic.append(ac)
+    t0 = -t0
+    for f, ac, k in zip(forces, acoustic, kinds):
+        if ac and k in ("gaussian", "d2gaussian"):
+            t0 = max(t0, -3.0 * (f.tshift - f.hdur))
+    hdur = np.array([f.hdur for f in forces], np.float64)
+    tshift = np.array([f.tshift for f in forces], np.float64)
+    return SourceSet(np.array(isp, np.int32), np.stack(arrs), kinds[0], hdur, tshift, t0, DT, located,
+                     acoustic=np.array(acoustic, bool), kinds=kinds)
 
 
 @dataclass
@@ -395,14 +542,15 @@ class ReceiverSet:
         return int(self.ispec.shape[0])
 
 
-def setup_receivers(db: Database, stations: list[Station]) -> ReceiverSet:
-    """Locate receivers (buried by ``burial`` below the top surface) and their interpolants."""
+def setup_receivers(db: Database, stations: list[Station], z_coordinates: bool = False) -> ReceiverSet:
+    """Locate receivers (buried by ``burial`` below the top surface, or at z = ``burial`` with
+    ``z_coordinates``, USE_SOURCES_RECEIVERS_Z) and their interpolants."""
     loc = Locator(db)
     x = get_gll().xigll
     isp, hx, hy, hz, names, located = [], [], [], [], [], []
     for s in stations:
         ztop = surface_z(db, s.x, s.y)
-        L = loc.locate(s.x, s.y, ztop - s.burial)
+        L = loc.locate(s.x, s.y, s.burial if z_coordinates else ztop - s.burial)
         isp.append(L.ispec)
         # hxir_store etc. are CUSTOM_REAL in the reference (specfem3D_par.F90:181)
         hx.append(lagrange_any(L.xi, x)[0].astype(np.float32).astype(np.float64))

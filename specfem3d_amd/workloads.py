@@ -109,7 +109,8 @@ def sea_over_visco_box(scale: float = 1.0, nstep: int = 4000, dt: float = 0.006,
     src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
     rec = S.setup_receivers(db, receiver_grid(ext, nrec_side, z_below_top=seafloor + 1.0))
     return Workload(f"{nx}x{nx}x{nz} acoustic-over-viscoelastic, {nstep} steps", db, src, rec, nstep, dt,
-                    attenuation=att, meta=dict(nx=nx, nz=nz, n_acoustic_layers=nac))
+                    attenuation=att, meta=dict(nx=nx, nz=nz, n_acoustic_layers=nac,
+                                               qmu=qmu if with_attenuation else None))
 
 
 def gradient_shots_box(scale: float = 1.0, nshots: int = 32, seed: int = 3, dt: float = 0.006, nrec: int = 50,

@@ -1,0 +1,144 @@
+"""Sources, adjoint sources and receivers in fluid elements on the GPU (C-ABI path).
+
+* a coupled box with an explosive source in the fluid and receivers on both sides of the sea floor:
+  displacement, velocity and acceleration records (in the fluid, (1/rho) grad of the potentials)
+  and pressure records (with and without USE_TRICK_FOR_BETTER_PRESSURE) equal the FP32 oracle bit
+  for bit in exact/ordered mode, and stay within 1e-5 with FMA and atomics;
+* a checkpointed kernel run with the source and the adjoint receivers in the fluid against the FP64
+  oracle (kernels <= 1e-4);
+* the reference's own examples against their REF_SEIS: homogeneous_acoustic (pressure) and
+  waterlayered_halfspace (vertical d/v/a on both sides of the water bottom).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def fluid_source_box(n=4, dt=0.004, seed=5, pressure_trick=False, fluid_receivers_only=False):
+    from specfem3d_amd.mesh.box import build_box, gradient_grf_model, homogeneous
+    from specfem3d_amd import sources as S
+    L = 500.0 * n
+    ext = (0.0, L, 0.0, L, -L, 0.0)
+    db = build_box(n, n, n, ext, material=gradient_grf_model(ext, seed=seed, corr_len=700.0),
+                   acoustic_top_layers=2, acoustic_material=homogeneous(1020.0, 1500.0, 0.0),
+                   stacey_faces=(1, 2, 3, 4, 5))
+    db.finalize(dt, stacey=True)
+    M = 3e13
+    cmt = S.CMTSource(x=0.43 * L, y=0.57 * L, depth_km=0.27 * L / 1000.0, tshift=0.0, hdur=0.06,
+                      Mxx=M, Myy=0.8 * M, Mzz=1.1 * M, Mxy=0.1 * M, Mxz=0.0, Myz=-0.05 * M)
+    src = S.setup_cmt_sources(db, [cmt], dt, pressure_trick=pressure_trick)
+    rng = np.random.default_rng(seed)
+    depths = [0.1, 0.2, 0.35, 0.45] + ([] if fluid_receivers_only else [0.55, 0.7, 0.85])
+    st = [S.Station(f"R{i}", "T", float(rng.uniform(0.15, 0.85) * L), float(rng.uniform(0.15, 0.85) * L), 0.0,
+                    d * L) for i, d in enumerate(depths)]
+    rec = S.setup_receivers(db, st)
+    assert db.ispec_is_acoustic[src.ispec].all()
+    if fluid_receivers_only:
+        assert db.ispec_is_acoustic[rec.ispec].all()
+    else:
+        assert db.ispec_is_acoustic[rec.ispec].any() and db.ispec_is_elastic[rec.ispec].any()
+    return db, src, rec
+
+
+def oracle_records(oracle_mod, db, src, rec, nstep, kinds):
+    o = oracle_mod.OracleSolver(db, src, rec, "f32")
+    o.stf_table = src.stf_table(nstep)
+    out = {k: np.zeros((nstep, rec.n, 3), np.float32) for k in kinds}
+    for it in range(nstep):
+        o.step()
+        for k in kinds:
+            out[k][it] = o.seismogram(k)
+    return out
+
+
+def gpu_records(db, src, rec, nstep, save, exact, assembly, pressure_trick=False):
+    from specfem3d_amd.solver import GPUSolver
+    print(f"[gpu] fluid records save={save} exact={exact} assembly={assembly} trick={pressure_trick}", flush=True)
+    g = GPUSolver(db, src, rec, nstep=nstep, exact=exact, assembly=assembly, save_seismograms=save,
+                  pressure_trick=pressure_trick)
+    g.run()
+    out = {k: g.seismograms(k).copy() for k in save}
+    g.close()
+    return out
+
+
+@pytest.mark.parametrize("exact,assembly", [(True, 1), (False, 0)])
+def test_fluid_source_vector_records(oracle_mod, exact, assembly):
+    db, src, rec = fluid_source_box()
+    ref = oracle_records(oracle_mod, db, src, rec, 250, ("displ", "veloc", "accel"))
+    out = gpu_records(db, src, rec, 250, ("d", "v", "a"), exact, assembly)
+    for k, rk in (("d", "displ"), ("v", "veloc"), ("a", "accel")):
+        assert np.abs(ref[rk]).max() > 0
+        if exact:
+            assert np.array_equal(out[k], ref[rk]), (k, rel_l2(out[k], ref[rk]))
+        else:
+            e = rel_l2(out[k], ref[rk])
+            print(f"[gpu] {k}: rel-L2 vs FP32 oracle {e:.3e}", flush=True)
+            assert e < 1e-5
+
+
+@pytest.mark.parametrize("trick", [False, True])
+def test_fluid_pressure_records(oracle_mod, trick):
+    db, src, rec = fluid_source_box(pressure_trick=trick, fluid_receivers_only=True)
+    kind = "pressure_trick" if trick else "pressure"
+    ref = oracle_records(oracle_mod, db, src, rec, 250, (kind,))[kind]
+    out = gpu_records(db, src, rec, 250, ("p",), True, 1, pressure_trick=trick)["p"]
+    assert np.abs(ref[:, :, 0]).max() > 0 and np.all(out[:, :, 1:] == 0)
+    assert np.array_equal(out, ref), rel_l2(out, ref)
+
+
+def test_fluid_kernel_run_matches_oracle(oracle_mod):
+    """Source and adjoint receivers in the fluid (compute_add_sources_ac_s3_cuda,
+    add_sources_ac_sim_2_or_3_cuda) in a checkpointed kernel run, against the FP64 oracle."""
+    from specfem3d_amd import adjoint as A
+    from specfem3d_amd.adjoint import KernelSolver
+    nstep = 300
+    db, src, rec = fluid_source_box(fluid_receivers_only=True)
+    rng = np.random.default_rng(11)
+    syn = oracle_mod.OracleSolver(db, src, rec, "f32").run(nstep)
+    obs = syn * (1.0 + 0.05 * rng.standard_normal(syn.shape[1:]))[None].astype(np.float32)
+    adj = A.waveform_adjoint_source(syn, obs)
+    r64 = oracle_mod.kernel_run(db, src, rec, nstep, adj.astype(np.float64), 64, "f64", with_acoustic=True)
+    ks = KernelSolver(db, src, rec, nstep, nt_dump=64, exact=True, assembly=1)
+    seis = ks.forward()
+    k = ks.adjoint(adj) + ks.acoustic_kernels
+    ks.close()
+    assert rel_l2(seis, r64[0]) < 1e-5
+    for name, a, b in zip(("rho", "mu", "kappa", "rho_ac", "kappa_ac"), k, r64[1:]):
+        b = b.reshape(a.shape)
+        e = rel_l2(a, b) if np.abs(b).max() > 0 else float(np.abs(a).max())
+        print(f"[gpu] fluid-source kernel run {name}_kl rel-L2 vs FP64 oracle {e:.3e}", flush=True)
+        assert e <= 1e-4
+    assert np.abs(k[3]).max() > 0 and np.abs(k[4]).max() > 0
+
+
+def test_homogeneous_acoustic_reference_example():
+    """Pressure records of the reference example homogeneous_acoustic (1000 steps) against REF_SEIS."""
+    from specfem3d_amd.examples import homogeneous_acoustic_example
+    db, src, rec, ref, dt, nstep, _ = homogeneous_acoustic_example()
+    out = gpu_records(db, src, rec, nstep, ("p",), False, 0, pressure_trick=True)["p"][:, :, :1]
+    e = rel_l2(out, ref)
+    cc = min(np.corrcoef(out[:, r, 0], ref[:, r, 0])[0, 1] for r in range(ref.shape[1]))
+    print(f"[gpu] homogeneous_acoustic: pressure rel-L2 vs REF_SEIS {e:.3e}, min correlation {cc:.6f}", flush=True)
+    assert e < 1e-2 and cc > 0.999
+
+
+def test_waterlayered_reference_example():
+    """Vertical displacement/velocity/acceleration of the reference example waterlayered_halfspace
+    (4500 steps, receivers in the water and in the solid) against REF_SEIS."""
+    from specfem3d_amd.examples import waterlayered_example
+    db, src, rec, ref, dt, nstep, st = waterlayered_example()
+    out = gpu_records(db, src, rec, nstep, ("d", "v", "a"), False, 0)
+    worst = []
+    for c, k in enumerate("dva"):
+        have = ~np.isnan(ref[0, :, c])
+        got, want = out[k][:, have, 2], ref[:, have, c]
+        e = rel_l2(got, want)
+        cc = min(np.corrcoef(got[:, r], want[:, r])[0, 1] for r in range(want.shape[1]))
+        print(f"[gpu] waterlayered {k}: {have.sum()} traces, rel-L2 vs REF_SEIS {e:.3e}, min correlation {cc:.6f}",
+              flush=True)
+        worst.append((e, cc))
+    assert max(e for e, _ in worst) < 5e-2 and min(cc for _, cc in worst) > 0.99
