@@ -230,6 +230,8 @@ class Attenuation:
     f_c: float
     min_period: float
     max_period: float
+    scale_factor: np.ndarray | None = None        # (nspec, 125) float32 modulus scale of mu (1 outside)
+    scale_factor_kappa: np.ndarray | None = None  # (nspec, 125) float32 modulus scale of kappa
 
 
 def setup_attenuation(db, qmu, qkappa=None, min_resolved_period: float | None = None, f0_ref: float = 18.0,
@@ -296,7 +298,7 @@ def setup_attenuation(db, qmu, qkappa=None, min_resolved_period: float | None = 
     kap[el] = (kap[el] * sck[el]).astype(np.float32)
     deltat = np.float32(db.dt if dt is None else dt)
     al, be, ga = memory_values(ts32, deltat)
-    return Attenuation(fcm, fck, kap, mu, al, be, ga, ts32, fc, tmin, tmax)
+    return Attenuation(fcm, fck, kap, mu, al, be, ga, ts32, fc, tmin, tmax, sc, sck)
 
 
 def estimate_min_resolved_period(db) -> float:

@@ -105,6 +105,10 @@ class Database:
     rmassy: np.ndarray | None = None
     rmassz: np.ndarray | None = None
     rmass_acoustic: np.ndarray | None = None
+    # their parts as the database file stores them (create_mass_matrices.f90): the elastic and acoustic
+    # mass before inversion and the implicit Stacey terms, combined and inverted by finalize() as
+    # prepare_timerun_mass_matrices does (prepare_timerun.F90:246-345)
+    mass_parts: dict = field(default_factory=dict)
     # attenuation (filled by attenuation.setup_attenuation)
     attenuation: object | None = None
     qmu: np.ndarray | None = None           # (nspec,125) float32 or None
@@ -155,12 +159,15 @@ class Database:
             irr = ~reg
             jac[irr] = self.jacobian[self.irregular_element_number[irr] - 1]
         # elastic mass: per-term real(dble(jac)*w*dble(rho)) summed in float32 in element order
+        parts = {}
         if self.nspec_elastic:
             el = self.ispec_is_elastic
             terms = (jac[el].astype(np.float64) * w3[None, :] * self.rhostore[el].astype(np.float64)).astype(np.float32)
             rmass = _ordered_scatter_add(nglob, ib[el], terms)
+            parts["rmass"] = rmass
             if stacey and self.abs_faces.n:
                 sx, sy, sz = self._stacey_mass_elastic(dt)
+                parts["rmassx"], parts["rmassy"], parts["rmassz"] = sx, sy, sz
                 rmassx, rmassy, rmassz = rmass + sx, rmass + sy, rmass + sz
             else:
                 rmassx = rmassy = rmassz = rmass
@@ -177,12 +184,17 @@ class Database:
             ac = self.ispec_is_acoustic
             terms = (jac[ac].astype(np.float64) * w3[None, :] / self.kappastore[ac].astype(np.float64)).astype(np.float32)
             rma = _ordered_scatter_add(nglob, ib[ac], terms)
+            parts["rmass_acoustic"] = rma
             if stacey and self.abs_faces.n:
-                rma = rma + self._stacey_mass_acoustic(dt)
+                parts["rmassz_acoustic"] = self._stacey_mass_acoustic(dt)
+                rma = rma + parts["rmassz_acoustic"]
+            else:
+                rma = rma.copy()
             rma[rma <= 0] = 1.0
             self.rmass_acoustic = (np.float32(1.0) / rma).astype(np.float32)
         else:
             self.rmass_acoustic = np.ones(nglob, np.float32)
+        self.mass_parts = parts
         return self
 
     def _stacey_mass_elastic(self, dt: float):
