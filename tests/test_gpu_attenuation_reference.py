@@ -7,9 +7,10 @@ benchmark lists) and REF_SEIS the reference's own run. The GPU run (FMA build, a
 goes through the whole attenuated path: host SLS set-up, scaled moduli, memory-variable update and
 stress correction in the stiffness kernel, point force, receiver.
 
-Tolerances: against the closed form 1e-2 aggregate rel-L2 over the three components (the
-reference's own run is 5.0e-3 away; the purely elastic closed form is 1.9 away, and a 1 % change of
-1/Q moves the traces by about 1e-2); against the reference's run 1e-3.
+Tolerances: against the closed form 6e-3 aggregate rel-L2 over the three components (the
+reference's own run is 5.08e-3 away, and so is this one; the purely elastic closed form is 0.69
+away over the same window, and a 1 % change of 1/Q moves the traces by about 1e-2); against the
+reference's run 1e-5 (measured on MI355X: 1.0e-6).
 """
 import numpy as np
 import pytest
@@ -46,6 +47,6 @@ def test_attenuation_benchmark_closed_form():
     print(f"[gpu] attenuation benchmark: rel-L2 vs closed form {e_ana:.3e} (reference run "
           f"{rel_l2(fx['ref_seis'][m], ana):.3e}), vs reference run {e_ref:.3e}, vs elastic closed form {e_ela:.3f}",
           flush=True)
-    assert e_ana < 1e-2
-    assert e_ref < 1e-3
+    assert e_ana < 6e-3
+    assert e_ref < 1e-5
     assert e_ela > 0.5

@@ -123,7 +123,8 @@ def test_homogeneous_acoustic_reference_example():
     e = rel_l2(out, ref)
     cc = min(np.corrcoef(out[:, r, 0], ref[:, r, 0])[0, 1] for r in range(ref.shape[1]))
     print(f"[gpu] homogeneous_acoustic: pressure rel-L2 vs REF_SEIS {e:.3e}, min correlation {cc:.6f}", flush=True)
-    assert e < 1e-2 and cc > 0.999
+    # measured on MI355X: 1.2e-5 rel-L2, correlation 1.000000 (REF_SEIS is printed to 7-8 digits)
+    assert e < 1e-4 and cc > 0.99999
 
 
 def test_waterlayered_reference_example():
@@ -141,4 +142,5 @@ def test_waterlayered_reference_example():
         print(f"[gpu] waterlayered {k}: {have.sum()} traces, rel-L2 vs REF_SEIS {e:.3e}, min correlation {cc:.6f}",
               flush=True)
         worst.append((e, cc))
-    assert max(e for e, _ in worst) < 5e-2 and min(cc for _, cc in worst) > 0.99
+    # measured on MI355X: 2.4e-6 (d), 4.5e-6 (v), 2.3e-5 (a) rel-L2; correlation >= 0.999999
+    assert max(e for e, _ in worst) < 2e-4 and min(cc for _, cc in worst) > 0.99999
