@@ -360,12 +360,8 @@ def main():
     ap.add_argument("--element-order", default="none",
                     choices=["morton", "none", "brick1x2", "brick2x1", "brick2x2", "brick1x4", "brick4x4", "brick2x4"],
                     help="launch order of the elastic elements of the single-shot run")
-    ap.add_argument("--elastic-shape", type=int, default=-1,
-                    help="single-shot stiffness kernel: 0 column-per-lane, 1 point-per-lane, 2 pipelined "
-                         "(-1: library default, 1)")
     ap.add_argument("--att-variant", type=int, default=-1,
                     help="attenuation schedule of the single-shot stiffness kernel (-1: library default)")
-    ap.add_argument("--pipe-wgs", type=int, default=0, help="persistent workgroups of schedule 5 (0: default)")
     ap.add_argument("--skip-single", action="store_true",
                     help="skip the single-shot 96x96x48 measurement (profiling runs of the other parts)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU oracle sample (0: skip)")
@@ -402,7 +398,7 @@ def main():
         w = W.gradient_shots_box(scale=args.batch_scale, nshots=32)
         w.shots = [(s_, r_, min(n_, 40)) for s_, r_, n_ in w.shots]
         b = BatchSolver(w.db, w.shots, exact=False, shots_per_group=8, assembly=3)
-        for ov in (1, 3):
+        for ov in (0, 1):
             b.set_tuning("lds_overlay", ov)
             b.reset()
             b.run(4, graph_steps=1)
@@ -451,12 +447,8 @@ def main():
     log(f"[bench] built {w.name}: {db.summary()} in {time.time() - t0:.1f}s")
 
     tuning = {}
-    if args.elastic_shape >= 0:
-        tuning["elastic_shape"] = args.elastic_shape
     if args.att_variant >= 0:
         tuning["att_variant"] = args.att_variant
-    if args.pipe_wgs > 0:
-        tuning["pipe_wgs"] = args.pipe_wgs
     g = GPUSolver(db, w.sources, w.receivers, nstep=nstep_total, attenuation=w.attenuation,
                   exact=bool(args.exact), assembly=args.assembly,
                   element_order=None if args.element_order == "none" else args.element_order, tuning=tuning)

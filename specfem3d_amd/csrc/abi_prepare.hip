@@ -302,7 +302,6 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
   H->nspec = m->nspec;
   if (const char* e = std::getenv("SPECFEM_AMD_EXACT")) m->exact = std::atoi(e) != 0;
   if (const char* e = std::getenv("SPECFEM_AMD_ASSEMBLY")) m->assembly = std::atoi(e);
-  if (const char* e = std::getenv("SPECFEM_AMD_ELASTIC_SHAPE")) m->elastic_shape = std::atoi(e) != 0;
 
   // GLL constants: hp[i*5+l] = hprime_xx(i+1,l+1) (column-major offset i + 5 l)
   realw hp[25], hpw[25], wxy[25];
@@ -1048,24 +1047,15 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
   }
 }
 
-// Tuning switches read by the step calls: "elastic_shape" (0 column-per-lane stiffness kernel,
-// 1 point-per-lane, the default, 2 persistent pipelined point kernels), "att_variant" (0..5:
-// attenuation-state scheduling of the point kernel, same results; 4 is the default, 5 the
-// persistent pipelined kernel), "pipe_wgs" (workgroups of the pipelined kernels, 0 = their
-// default: 2048 with attenuation, 2560 without), "acoustic_kernel_field" (below). Unknown names
+// Tuning switches read by the step calls: "att_variant" (0..4: attenuation-state scheduling of the
+// point kernel, same results; 4 is the default), "acoustic_kernel_field" (below). Unknown names
 // are a fatal error.
 extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
-  if (std::strcmp(name, "elastic_shape") == 0) {
-    if (value < 0 || value > 2) fatal("sf_set_tuning", "elastic_shape must be 0, 1 or 2");
-    m->elastic_shape = value;
-  } else if (std::strcmp(name, "att_variant") == 0) {
-    if (value < 0 || value > 5) fatal("sf_set_tuning", "att_variant must be 0..5");
+  if (std::strcmp(name, "att_variant") == 0) {
+    if (value < 0 || value > 4) fatal("sf_set_tuning", "att_variant must be 0..4");
     m->att_variant = value;
-  } else if (std::strcmp(name, "pipe_wgs") == 0) {
-    if (value < 0) fatal("sf_set_tuning", "pipe_wgs must be >= 0");
-    m->pipe_wgs = value;
   } else if (std::strcmp(name, "acoustic_kernel_field") == 0) {
     // which adjoint array compute_kernels_acoustic_cuda pairs with the reconstructed field:
     // 0 = potential_acoustic (compute_kernels.f90:269 and :325), 1 = potential_dot_dot_acoustic

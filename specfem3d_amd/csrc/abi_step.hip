@@ -77,7 +77,6 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
     a.gamma[s] = f ? m->b_gammaval[s] : m->gammaval[s];
   }
   a.local_out = m->asm_buf;
-  a.pipe_wgs = m->pipe_wgs;
   return a;
 }
 
@@ -91,10 +90,10 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
     for (int c = 0; c < m->el_ncolor; c++) {
       a.elems = m->el_color_elems + m->el_color_off[c];
       a.nelem = m->el_color_off[c + 1] - m->el_color_off[c];
-      launch_elastic(m->stream, a, strain, att, m->exact != 0, 2, m->elastic_shape, m->att_variant);
+      launch_elastic(m->stream, a, strain, att, m->exact != 0, 2, m->att_variant);
     }
   } else {
-    launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->elastic_shape, m->att_variant);
+    launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->att_variant);
   }
   if (timed) {
     SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count + 1], m->stream));

@@ -33,8 +33,7 @@ struct Batch {
   int nshots = 0, S = 0;
   int S0 = 0;                    // padded shot count at creation; S shrinks as shots finish
   int sc = 4, mode = 2;          // shots per workgroup, assembly (0 atomic, 2 coloured, 3 fused)
-  int overlay = 0;               // 1: displacement tile and stress products share LDS; 2: pipelined kernel
-  int pipe_wgs = 0;              // persistent workgroups of the pipelined kernel (0: its default)
+  int overlay = 0;               // 1: displacement tile and stress products share LDS
   int max_nstep = 0;
   realw dt = 0.f, dt2 = 0.f, dtsq2 = 0.f;  // deltat, deltatover2, deltatsqover2
   std::vector<int> nstep_user;   // per user shot
@@ -435,7 +434,6 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
   std::memset(&A, 0, sizeof(A));
   A.ngroups = b->S / b->sc;
   A.S = b->S;
-  A.pipe_wgs = b->pipe_wgs;
   A.ibool = m->ibool;
   A.irregular = m->irregular;
   A.xix_regular = m->xix_regular;
@@ -481,7 +479,7 @@ static void enqueue_stiffness(Batch* b, hipStream_t st) {
   } else {
     A.elems = m->elastic_elems;
     A.nelem = m->nspec_elastic;
-    launch_elastic_batch(st, A, b->sc, 0, b->overlay >= 2 ? 1 : b->overlay, m->exact != 0);  // atomic: plain kernel
+    launch_elastic_batch(st, A, b->sc, 0, b->overlay, m->exact != 0);
   }
 }
 
@@ -749,23 +747,14 @@ extern "C" void sf_batch_set_options(long h, int shots_per_group, int assembly) 
 }
 
 // Kernel variants that give the same results: "lds_overlay" 0/1 (the displacement tile shares LDS
-// with the stress products), 2 (persistent pipelined kernel), 3 (shot-packed tiles; 2 and 3 for
-// coloured elastic assembly only);
-// "pipe_wgs" (workgroups of the pipelined kernel, 0 = its default). Unknown names are fatal.
+// with the stress products). Unknown names are fatal.
 extern "C" void sf_batch_set_tuning(long h, const char* name, int value) {
   Batch* b = batch_of(h);
   if (std::strcmp(name, "lds_overlay") == 0) {
-    if (value < 0 || value > 3) fatal("sf_batch_set_tuning", "lds_overlay must be 0..3");
-    if (value >= 2 && (b->att || b->mode == 0))
-      fatal("sf_batch_set_tuning", "the pipelined and shot-packed kernels need elastic shots and coloured assembly");
+    if (value < 0 || value > 1) fatal("sf_batch_set_tuning", "lds_overlay must be 0 or 1");
     SF_CHECK(hipStreamSynchronize(b->m->stream));
     drop_graph(b);
     b->overlay = value;
-  } else if (std::strcmp(name, "pipe_wgs") == 0) {
-    if (value < 0) fatal("sf_batch_set_tuning", "pipe_wgs must be >= 0");
-    SF_CHECK(hipStreamSynchronize(b->m->stream));
-    drop_graph(b);
-    b->pipe_wgs = value;
   } else {
     fatal("sf_batch_set_tuning", "unknown tuning switch");
   }
@@ -774,7 +763,6 @@ extern "C" void sf_batch_set_tuning(long h, const char* name, int value) {
 extern "C" int sf_batch_get_tuning(long h, const char* name) {
   Batch* b = batch_of(h);
   if (std::strcmp(name, "lds_overlay") == 0) return b->overlay;
-  if (std::strcmp(name, "pipe_wgs") == 0) return b->pipe_wgs;
   fatal("sf_batch_get_tuning", "unknown tuning switch");
   return 0;
 }
@@ -880,10 +868,9 @@ extern "C" double sf_batch_autotune(long h, int trial_steps) {
   int best_sc = b->sc, best_mode = b->mode, best_ov = b->overlay;
   for (int mode : {3, 2, 0})
     for (int sc : {4, 8})
-    for (int ov : {3, 2, 1, 0}) {
+    for (int ov : {1, 0}) {
       if (b->S % sc) continue;
       if (b->att && (mode == 0 || ov != 1)) continue;  // coloured only; no overlay switch
-      if (ov >= 2 && mode == 0) continue;             // the pipelined and packed kernels are coloured only
       b->sc = sc;
       b->mode = mode;
       b->overlay = ov;

@@ -15,8 +15,8 @@ void set_gll_constants(const realw* hprime, const realw* hprimewgll, const realw
 }
 
 // variants compiled from k_elastic.hip / k_acoustic.hip with and without FMA contraction
-void launch_elastic_exact(hipStream_t, const ElasticArgs&, bool, bool, int, int, int);
-void launch_elastic_fast(hipStream_t, const ElasticArgs&, bool, bool, int, int, int);
+void launch_elastic_exact(hipStream_t, const ElasticArgs&, bool, bool, int, int);
+void launch_elastic_fast(hipStream_t, const ElasticArgs&, bool, bool, int, int);
 void launch_element_strain_exact(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
 void launch_element_strain_fast(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
 void launch_elastic_batch_exact(hipStream_t, const BatchArgs&, int, int, int);
@@ -24,12 +24,12 @@ void launch_elastic_batch_fast(hipStream_t, const BatchArgs&, int, int, int);
 void launch_acoustic_exact(hipStream_t, const AcousticArgs&, int);
 void launch_acoustic_fast(hipStream_t, const AcousticArgs&, int);
 
-void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode, int shape,
+void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode,
                     int att_variant) {
   if (exact)
-    launch_elastic_exact(s, a, strain, att, mode, shape, att_variant);
+    launch_elastic_exact(s, a, strain, att, mode, att_variant);
   else
-    launch_elastic_fast(s, a, strain, att, mode, shape, att_variant);
+    launch_elastic_fast(s, a, strain, att, mode, att_variant);
 }
 
 void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,

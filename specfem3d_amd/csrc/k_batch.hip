@@ -223,392 +223,6 @@ k_elastic_batch(BatchArgs B, GLLk G) {
   }
 }
 
-// Shot-packed variant (MODE 2 or 3; "lds_overlay" 3): the tiles hold a lane's four shots of a
-// point and plane as one 16-byte word, value (plane, point, shot) at float4 index
-// (plane * 128 + point) * NQ + quad, so every tile read of the contractions is one 16-byte LDS
-// load serving four shots (a quarter of the LDS round trips of the shot-major rows). The four
-// shots' contractions run side by side, each component with the expressions of tile_gradient and
-// tile_divergence_c, then each shot's mapping, stress and products as in elastic_point.h: the
-// same bits as k_elastic_batch.
-__device__ __forceinline__ float4 f4_mul(realw h, const float4& a) {
-  return make_float4(h * a.x, h * a.y, h * a.z, h * a.w);
-}
-__device__ __forceinline__ void f4_acc(float4& t, realw h, const float4& a) {
-  t.x = t.x + h * a.x; t.y = t.y + h * a.y; t.z = t.z + h * a.z; t.w = t.w + h * a.w;
-}
-// the value is computed here (the empty asm reads and rewrites it), which bounds live ranges
-__device__ __forceinline__ void f4_pin(float4& t) { asm volatile("" : "+v"(t.x), "+v"(t.y), "+v"(t.z), "+v"(t.w)); }
-
-template <int NQ, int MODE>
-__global__ void __launch_bounds__(128 * NQ) k_elastic_batch_packed(BatchArgs B, GLLk G) {
-  constexpr int SC = 4 * NQ;
-  constexpr int PL = 128 * NQ;                    // float4 per plane
-  __shared__ float4 sU4[3 * PL];                  // displacement [c][point][quad]
-  __shared__ float4 sT4[9 * PL];                  // stress products [m][point][quad]
-  __shared__ realw sG[75];
-  stage_gll(sG, G, threadIdx.x);
-  const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int b = xcd_remap_b(blockIdx.x, gridDim.x);
-  const int ie = b / B.ngroups;
-  const int s0 = (b - ie * B.ngroups) * SC;
-  if (B.nstep[s0] < it) return;                   // every shot of this group has finished
-  const int q = threadIdx.x % NQ, p = threadIdx.x / NQ;
-  const int sq = s0 + 4 * q;
-  const bool live0 = B.nstep[sq] >= it;
-  bool live[4];
-#pragma unroll
-  for (int r = 0; r < 4; r++) live[r] = B.nstep[sq + r] >= it;
-  const bool active = p < 125;
-  const int pp = active ? p : 124;
-  const int e = B.elems[ie];
-  const int irr = B.irregular[e];
-  const int64_t S = B.S;
-  const int g = B.ibool[(int64_t)e * 125 + pp];
-  const int64_t row = (int64_t)g * 3 * S + sq;
-  const unsigned fl = MODE == 3 ? (unsigned)B.flags[(int64_t)e * 125 + pp] : 0u;
-  float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;
-  if (live0) {
-    u0 = *reinterpret_cast<const float4*>(B.u + row);
-    u1 = *reinterpret_cast<const float4*>(B.u + row + S);
-    u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
-  }
-  const float2 km = B.kappamu[(int64_t)e * 125 + pp];
-  JacPt J = {};
-  realw jacl = B.jacobian_regular;
-  if (irr) {
-    J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
-    jacl = J.jac;
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
-  const bool last = MODE == 3 && (fl & BF_LAST) != 0;
-  const bool mine = active && live0;
-  float4 A[3], V[3];
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    A[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-    V[c] = A[c];
-    if (mine && (!first || !live[3])) A[c] = *reinterpret_cast<const float4*>(B.a + row + c * S);
-    if (last && mine) V[c] = *reinterpret_cast<const float4*>(B.v + row + c * S);
-  }
-  __builtin_amdgcn_sched_barrier(0);
-  if (active) {
-    sU4[(0 * 128 + p) * NQ + q] = u0;
-    sU4[(1 * 128 + p) * NQ + q] = u1;
-    sU4[(2 * 128 + p) * NQ + q] = u2;
-  }
-  lds_barrier();
-
-  const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
-  {
-    // contractions of the three components for the lane's four shots (tile_gradient's order)
-    const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
-    const float4* ux = sU4 + q;
-    const float4* uy = sU4 + PL + q;
-    const float4* uz = sU4 + 2 * PL + q;
-    const realw h1 = sG[i * 5], h2 = sG[j * 5], h3 = sG[k * 5];
-    float4 tx1 = f4_mul(h1, ux[b1 * NQ]), ty1 = f4_mul(h1, uy[b1 * NQ]), tz1 = f4_mul(h1, uz[b1 * NQ]);
-    float4 tx2 = f4_mul(h2, ux[b2 * NQ]), ty2 = f4_mul(h2, uy[b2 * NQ]), tz2 = f4_mul(h2, uz[b2 * NQ]);
-    float4 tx3 = f4_mul(h3, ux[b3 * NQ]), ty3 = f4_mul(h3, uy[b3 * NQ]), tz3 = f4_mul(h3, uz[b3 * NQ]);
-#pragma unroll
-    for (int l = 1; l < 5; l++) {
-      const realw a1 = sG[i * 5 + l], a2 = sG[j * 5 + l], a3 = sG[k * 5 + l];
-      const int q1 = (b1 + l) * NQ, q2 = (b2 + 5 * l) * NQ, q3 = (b3 + 25 * l) * NQ;
-      f4_acc(tx1, a1, ux[q1]); f4_acc(ty1, a1, uy[q1]); f4_acc(tz1, a1, uz[q1]);
-      f4_acc(tx2, a2, ux[q2]); f4_acc(ty2, a2, uy[q2]); f4_acc(tz2, a2, uz[q2]);
-      f4_acc(tx3, a3, ux[q3]); f4_acc(ty3, a3, uy[q3]); f4_acc(tz3, a3, uz[q3]);
-      // the accumulations of this l are complete before the next tile values are loaded: nine
-      // 16-byte values in flight at a time, not forty-five
-      f4_pin(tx1); f4_pin(ty1); f4_pin(tz1); f4_pin(tx2); f4_pin(ty2); f4_pin(tz2);
-      f4_pin(tx3); f4_pin(ty3); f4_pin(tz3);
-      asm volatile("" ::: "memory");
-    }
-    realw* sTs = reinterpret_cast<realw*>(sT4);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      Grad9 tg;
-      tg.x1 = c4(tx1, r); tg.y1 = c4(ty1, r); tg.z1 = c4(tz1, r);
-      tg.x2 = c4(tx2, r); tg.y2 = c4(ty2, r); tg.z2 = c4(tz2, r);
-      tg.x3 = c4(tx3, r); tg.y3 = c4(ty3, r); tg.z3 = c4(tz3, r);
-      const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
-      const Sigma6 sg = iso_stress(du, km.x, km.y);
-      realw t[9];
-      stress_products(sg, irr != 0, J, jacl, B.xix_regular, t);
-      if (active) {
-#pragma unroll
-        for (int m = 0; m < 9; m++) sTs[((m * 128 + p) * NQ + q) * 4 + r] = t[m];
-      }
-      asm volatile("" ::: "memory");              // one shot's stress at a time
-    }
-  }
-  lds_barrier();
-  if (!mine) return;
-
-  realw* a = B.a + row;
-  const realw* hpw = sG + 25;
-  const realw* wxy = sG + 50;
-  bool cont[4], ends = false;
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    cont[r] = live[r] && B.nstep[sq + r] > it;
-    ends = ends || (live[r] && !cont[r]);
-  }
-  const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    // tile_divergence_c for four shots
-    const float4* s1 = sT4 + c * PL + q;
-    const float4* s2 = sT4 + (3 + c) * PL + q;
-    const float4* s3 = sT4 + (6 + c) * PL + q;
-    float4 n1 = f4_mul(hpw[i], s1[b1 * NQ]);
-    float4 n2 = f4_mul(hpw[j], s2[b2 * NQ]);
-    float4 n3 = f4_mul(hpw[k], s3[b3 * NQ]);
-#pragma unroll
-    for (int l = 1; l < 5; l++) {
-      f4_acc(n1, hpw[l * 5 + i], s1[(b1 + l) * NQ]);
-      f4_acc(n2, hpw[l * 5 + j], s2[(b2 + 5 * l) * NQ]);
-      f4_acc(n3, hpw[l * 5 + k], s3[(b3 + 25 * l) * NQ]);
-      f4_pin(n1); f4_pin(n2); f4_pin(n3);
-      asm volatile("" ::: "memory");
-    }
-    const realw w1 = wxy[j * 5 + k], w2 = wxy[i * 5 + k], w3 = wxy[i * 5 + j];
-    realw f[4];
-#pragma unroll
-    for (int r = 0; r < 4; r++) f[r] = w1 * c4(n1, r) + w2 * c4(n2, r) + w3 * c4(n3, r);
-    float4 Ac = A[c];
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-      if (live[r]) q4(Ac, r) = (first ? 0.f : q4(Ac, r)) + (-f[r]);
-    if (MODE == 2 || !last) {
-      *reinterpret_cast<float4*>(a + c * S) = Ac;
-      continue;
-    }
-    const realw M = B.rmass[(int64_t)g * 3 + c];
-    float4 Vc = V[c];
-    float4 U = c == 0 ? u0 : (c == 1 ? u1 : u2);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      if (!live[r]) continue;
-      newmark_finish(q4(U, r), q4(Vc, r), q4(Ac, r), M, B.dt, B.dt2, B.dtsq2, cont[r]);
-      if (cont[r]) q4(Ac, r) = 0.f;
-    }
-    if (cont[0]) *reinterpret_cast<float4*>(B.u + row + c * S) = U;
-    *reinterpret_cast<float4*>(B.v + row + c * S) = Vc;
-    if ((fl & BF_ZERO) || ends) *reinterpret_cast<float4*>(a + c * S) = Ac;
-  }
-}
-
-// Persistent, pipelined variant (MODE 2 or 3; "lds_overlay" 2): a workgroup walks the pairs
-// (element, shot group) w, w + NG, w + 2 NG, ... (NG = grid size; pair w is element w / ngroups,
-// group w % ngroups, as the one-pair-per-workgroup grid numbers them). While one pair is
-// computed, the next pair's displacement rows, point flags, moduli and Jacobian index are in
-// flight, the ibool row of the pair after it and the list entry of the one after that (so no
-// load in the loop waits for another), and the current pair's accumulator and velocity rows.
-// The displacement tile has LDS planes of its own, so the next pair's tile is staged as soon as
-// every lane is past this pair's gradient; the finisher re-reads this step's u row from global
-// memory (nothing in the colour writes it before). Barriers order LDS only. Per pair and shot the
-// arithmetic is k_elastic_batch's, so the results are the same bits. Uniform indices go through
-// a vector register: loaded from a uniform address they would be moved to a scalar register
-// right after the load, which waits for every memory operation issued before it.
-template <int NQ, int MODE>
-__global__ void __launch_bounds__(128 * NQ) __attribute__((amdgpu_waves_per_eu(3)))
-k_elastic_batch_pipe(BatchArgs B, GLLk G) {
-  constexpr int SC = 4 * NQ;
-  constexpr int PST = 132;
-  constexpr int CPL = SC * PST;
-  __shared__ realw sBuf[12 * CPL];
-  realw* const sT = sBuf;                         // stress products [m][s][p]
-  realw* const sU = sBuf + 9 * CPL;               // displacement tile [c][s][p]
-  __shared__ realw sG[75];
-  stage_gll(sG, G, threadIdx.x);
-  const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int q = threadIdx.x % NQ, p = threadIdx.x / NQ;
-  const bool active = p < 125;
-  const int pp = active ? p : 124;
-  const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
-  const int ts = 4 * q;
-  const int64_t S = B.S;
-  const int NG = gridDim.x;
-  const int ngr = B.ngroups;
-  const int npairs = B.nelem * ngr;               // the launcher keeps npairs + 3 NG below 2^31
-  int w = xcd_remap_b(blockIdx.x, NG);
-  if (w >= npairs) return;                        // the launcher keeps NG <= npairs
-  // live bits of a pair for this lane: bit r = shot sq + r is live, bit 4 = the group's first
-  // shot is live, bit 5 + r = shot sq + r continues after this step
-  auto live_bits = [&](int ww) {
-    const int s0 = (ww % ngr) * SC;
-    const int sq = s0 + 4 * q;
-    int b = B.nstep[s0] >= it ? 16 : 0;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int n = B.nstep[sq + r];
-      b |= (n >= it ? (1 << r) : 0) | (n > it ? (32 << r) : 0);
-    }
-    return b;
-  };
-  auto elem_of = [&](int ww) {
-    int iev = ww / ngr;
-    asm volatile("" : "+v"(iev));
-    return B.elems[iev];
-  };
-  // pairs k, k+1 (next), k+2, k+3 of this walk; a finished walk repeats its last pair in the
-  // look-ahead slots (all loads stay in bounds; nothing is written for them)
-  const int w1 = w + NG < npairs ? w + NG : w;
-  const int w2 = w + 2 * NG < npairs ? w + 2 * NG : w1;
-  int e = elem_of(w);
-  int e_n = elem_of(w1);
-  int e_nn = elem_of(w2);
-  int lv = live_bits(w);
-  int lv_n = live_bits(w1);
-  int g = B.ibool[(int64_t)e * 125 + pp];
-  int g_n = B.ibool[(int64_t)e_n * 125 + pp];
-  unsigned fl = MODE == 3 ? (unsigned)B.flags[(int64_t)e * 125 + pp] : 0u;
-  float2 km = B.kappamu[(int64_t)e * 125 + pp];
-  int irr = B.irregular[e];
-  asm volatile("" : "+v"(irr));
-  {
-    const int64_t row = (int64_t)g * 3 * S + (w % ngr) * SC + 4 * q;
-    float4 u0 = make_float4(0.f, 0.f, 0.f, 0.f), u1 = u0, u2 = u0;
-    if ((lv & 1) && (lv & 16)) {
-      u0 = *reinterpret_cast<const float4*>(B.u + row);
-      u1 = *reinterpret_cast<const float4*>(B.u + row + S);
-      u2 = *reinterpret_cast<const float4*>(B.u + row + 2 * S);
-    }
-    if (active) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        sU[(0 * SC + ts + r) * PST + p] = c4(u0, r);
-        sU[(1 * SC + ts + r) * PST + p] = c4(u1, r);
-        sU[(2 * SC + ts + r) * PST + p] = c4(u2, r);
-      }
-    }
-  }
-  lds_barrier();
-
-  const realw* hpw = sG + 25;
-  const realw* wxy = sG + 50;
-  for (;;) {
-    const bool has_next = w + NG < npairs;        // uniform over the workgroup
-    const int s0 = (w % ngr) * SC;
-    const int sq = s0 + 4 * q;
-    const int64_t row = (int64_t)g * 3 * S + sq;
-    const bool alive = (lv & 16) != 0;            // some shot of the group is live
-    const bool live0 = (lv & 1) != 0;
-    const bool mine = active && live0 && alive;
-    const bool first = MODE == 3 && (fl & BF_FIRST) != 0;
-    const bool last = MODE == 3 && (fl & BF_LAST) != 0;
-    // look-ahead loads
-    const int w3 = w + 3 * NG < npairs ? w + 3 * NG : (w + 2 * NG < npairs ? w + 2 * NG : (has_next ? w + NG : w));
-    const int e_nnn = elem_of(w3);
-    const int w2n = w + 2 * NG < npairs ? w + 2 * NG : (has_next ? w + NG : w);
-    const int lv_nn = live_bits(w2n);
-    const int g_nn = B.ibool[(int64_t)e_nn * 125 + pp];
-    const int wn = has_next ? w + NG : w;
-    const int64_t row_n = (int64_t)g_n * 3 * S + (wn % ngr) * SC + 4 * q;
-    float4 un0 = make_float4(0.f, 0.f, 0.f, 0.f), un1 = un0, un2 = un0;
-    if (has_next && (lv_n & 1) && (lv_n & 16)) {
-      un0 = *reinterpret_cast<const float4*>(B.u + row_n);
-      un1 = *reinterpret_cast<const float4*>(B.u + row_n + S);
-      un2 = *reinterpret_cast<const float4*>(B.u + row_n + 2 * S);
-    }
-    const unsigned fl_n = MODE == 3 ? (unsigned)B.flags[(int64_t)e_n * 125 + pp] : 0u;
-    const float2 km_n = B.kappamu[(int64_t)e_n * 125 + pp];
-    int en_v = e_n;
-    asm volatile("" : "+v"(en_v));
-    const int irr_n = B.irregular[en_v];
-    // this pair: accumulator rows (unless this lane writes first) and velocity rows (finisher)
-    float4 A[3], V[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      A[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-      V[c] = A[c];
-      if (mine && (!first || !(lv & 8))) A[c] = *reinterpret_cast<const float4*>(B.a + row + c * S);
-      if (last && mine) V[c] = *reinterpret_cast<const float4*>(B.v + row + c * S);
-    }
-    JacPt J = {};
-    realw jacl = B.jacobian_regular;
-    if (irr) {
-      J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
-      jacl = J.jac;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-
-    if (alive) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const realw* su = sU + (ts + r) * PST;
-        const Grad9 tg = tile_gradient<1>(su, su + SC * PST, su + 2 * SC * PST, sG, i, j, k);
-        const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
-        const Sigma6 sg = iso_stress(du, km.x, km.y);
-        realw t[9];
-        stress_products(sg, irr != 0, J, jacl, B.xix_regular, t);
-        if (active) {
-#pragma unroll
-          for (int m = 0; m < 9; m++) sT[(m * SC + ts + r) * PST + p] = t[m];
-        }
-      }
-    }
-    lds_barrier();                                // products complete; the tile is free
-    if (has_next && active) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        sU[(0 * SC + ts + r) * PST + p] = c4(un0, r);
-        sU[(1 * SC + ts + r) * PST + p] = c4(un1, r);
-        sU[(2 * SC + ts + r) * PST + p] = c4(un2, r);
-      }
-    }
-    if (mine) {
-      realw* a = B.a + row;
-      bool live[4], cont[4], ends = false;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        live[r] = (lv >> r) & 1;
-        cont[r] = live[r] && ((lv >> (5 + r)) & 1);
-        ends = ends || (live[r] && !cont[r]);
-      }
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        realw f[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) f[r] = tile_divergence_c<1>(sT + (ts + r) * PST, CPL, hpw, wxy, i, j, k, c);
-        float4 Ac = A[c];
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          if (live[r]) q4(Ac, r) = (first ? 0.f : q4(Ac, r)) + (-f[r]);
-        if (MODE == 2 || !last) {
-          *reinterpret_cast<float4*>(a + c * S) = Ac;
-          continue;
-        }
-        const realw M = B.rmass[(int64_t)g * 3 + c];
-        float4 Vc = V[c];
-        float4 U = *reinterpret_cast<const float4*>(B.u + row + c * S);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          if (!live[r]) continue;
-          newmark_finish(q4(U, r), q4(Vc, r), q4(Ac, r), M, B.dt, B.dt2, B.dtsq2, cont[r]);
-          if (cont[r]) q4(Ac, r) = 0.f;
-        }
-        if (cont[0]) *reinterpret_cast<float4*>(B.u + row + c * S) = U;
-        *reinterpret_cast<float4*>(B.v + row + c * S) = Vc;
-        if ((fl & BF_ZERO) || ends) *reinterpret_cast<float4*>(a + c * S) = Ac;
-      }
-    }
-    lds_barrier();                                // products read; the next tile is staged
-    if (!has_next) break;
-    w += NG;
-    e = e_n;
-    e_n = e_nn;
-    e_nn = e_nnn;
-    g = g_n;
-    g_n = g_nn;
-    lv = lv_n;
-    lv_n = lv_nn;
-    fl = fl_n;
-    km = km_n;
-    irr = irr_n;
-  }
-}
-
 // Viscoelastic (3-SLS) variant: each shot also carries its memory variables and stored strain
 // (one block per element and shot: plane, point). The tile holds u and dt*v (6 rows per shot; they share the
 // LDS of the stress products t2, t3, written after a barrier). Per shot, the arithmetic and its
@@ -835,48 +449,6 @@ void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int m
   const int64_t nb = (int64_t)a.nelem * a.ngroups;
   if (nb > 0x7fffffff) fatal("launch_elastic_batch", "grid too large");
   const GLLk G = g_gll_host;
-  if (overlay == 3) {
-    if (mode != 2 && mode != 3) fatal("launch_elastic_batch", "the shot-packed kernel needs coloured assembly");
-    if (mode == 3 && (!a.v || !a.rmass || !a.flags))
-      fatal("launch_elastic_batch", "fused update needs v, rmass and point flags");
-    dim3 pgrid((unsigned)nb);
-#define SF_KK(NQ, M) hipLaunchKernelGGL((k_elastic_batch_packed<NQ, M>), pgrid, dim3(128 * NQ), 0, st, a, G)
-    if (sc == 4) {
-      if (mode == 3) SF_KK(1, 3);
-      else SF_KK(1, 2);
-    } else if (sc == 8) {
-      if (mode == 3) SF_KK(2, 3);
-      else SF_KK(2, 2);
-    } else {
-      fatal("launch_elastic_batch", "shots per workgroup must be 4 or 8");
-    }
-#undef SF_KK
-    SF_LAUNCH_CHECK("k_elastic_batch_packed");
-    return;
-  }
-  if (overlay == 2) {
-    // persistent pipelined kernel: about three workgroups of 8 shots (six of 4) per CU
-    if (mode != 2 && mode != 3) fatal("launch_elastic_batch", "the pipelined kernel needs coloured assembly");
-    if (mode == 3 && (!a.v || !a.rmass || !a.flags))
-      fatal("launch_elastic_batch", "fused update needs v, rmass and point flags");
-    int64_t ng = a.pipe_wgs > 0 ? a.pipe_wgs : (sc == 8 ? 768 : 1536);
-    if (ng > nb) ng = nb;
-    if (nb + 3 * ng > 0x7fffffff) fatal("launch_elastic_batch", "grid too large");
-    dim3 pg((unsigned)ng);
-#define SF_KP(NQ, M) hipLaunchKernelGGL((k_elastic_batch_pipe<NQ, M>), pg, dim3(128 * NQ), 0, st, a, G)
-    if (sc == 4) {
-      if (mode == 3) SF_KP(1, 3);
-      else SF_KP(1, 2);
-    } else if (sc == 8) {
-      if (mode == 3) SF_KP(2, 3);
-      else SF_KP(2, 2);
-    } else {
-      fatal("launch_elastic_batch", "shots per workgroup must be 4 or 8");
-    }
-#undef SF_KP
-    SF_LAUNCH_CHECK("k_elastic_batch_pipe");
-    return;
-  }
   dim3 grid((unsigned)nb);
 #define SF_KB(NQ, M)                                                                           \
   if (overlay) hipLaunchKernelGGL((k_elastic_batch<NQ, M, 1>), grid, dim3(128 * NQ), 0, st, a, G); \

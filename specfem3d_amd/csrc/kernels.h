@@ -53,14 +53,11 @@ struct ElasticArgs {
   int64_t nloc;            // nspec*125
   realw alpha[3], beta[3], gamma[3];
   realw* local_out;        // ordered assembly: 3 planes of nloc
-  int pipe_wgs;            // schedule 5: persistent workgroups (at most nelem are launched)
 };
 // mode: 0 atomic scatter, 1 store local contributions (ordered gather), 2 plain read-modify-write
-// (the element list is one colour: its elements share no global point); shape: 0 five elements
-// per workgroup with a GLL column per lane (k_elastic), 1 one element per workgroup with a GLL
-// point per lane (k_elastic_pt)
+// (the element list is one colour: its elements share no global point)
 // att_variant: scheduling of the attenuation state in the point kernel (k_elastic_pt's AV)
-void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode, int shape,
+void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, bool exact, int mode,
                     int att_variant);
 // deviatoric strain planes 0..4 and trace/3 of A.displ; with_trace also writes plane 5 = 3 * trace/3
 void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,
@@ -100,12 +97,11 @@ struct BatchArgs {
   const realw* fck;
   int64_t nloc;
   realw alpha[3], beta[3], gamma[3];
-  int pipe_wgs;                // persistent workgroups of the pipelined kernel (0: default)
 };
 enum : unsigned char { BF_FIRST = 1, BF_LAST = 2, BF_ZERO = 4 };
 // sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour),
 // 3 one colour with the Newmark update fused (see k_batch.hip); overlay 1 shares the LDS of the
-// displacement tile and the stress products, 2 selects the persistent pipelined kernel (modes 2, 3)
+// displacement tile and the stress products
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact);
 // viscoelastic (3 SLS) batched stiffness; mode 2 or 3
 void launch_visco_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);

@@ -178,9 +178,7 @@ struct Mesh {
   int exact = 0;                   // 1: kernels built without fused multiply-add (oracle order)
   int assembly = -1;               // 0 atomic scatter-add, 1 ordered gather (oracle order), 2 coloured,
                                    // -1 automatic (resolved at prepare_fields_elastic_device)
-  int elastic_shape = 1;           // 0 column-per-lane kernel (5 elements/WG), 1 point-per-lane
-  int att_variant = 4;             // attenuation-state scheduling of the point kernel (0..5)
-  int pipe_wgs = 0;                // persistent workgroups of the pipelined kernels (0: their default)
+  int att_variant = 4;             // attenuation-state scheduling of the point kernel (0..4)
   int ac_kernel_dd = 0;            // 1: fluid kernels read the adjoint potential_dot_dot (misfit gradient)
   // deferred corrector (kernel_3) per field, fused into the next predictor when possible
   int pending_k3_el[2] = {0, 0};

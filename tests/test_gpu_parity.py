@@ -18,13 +18,11 @@ from conftest import GOLDEN, rel_l2, small_elastic_box, smooth_warp
 pytestmark = pytest.mark.gpu
 
 
-def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, shape=0, att_variant=0, pipe_wgs=None):
+def run_gpu(db, src, rec, nstep, exact, assembly, attenuation=None, att_variant=None):
     from specfem3d_amd.solver import GPUSolver
-    print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly} shape={shape} "
-          f"att_variant={att_variant} pipe_wgs={pipe_wgs}", flush=True)
-    tuning = {"elastic_shape": shape, "att_variant": att_variant}
-    if pipe_wgs is not None:
-        tuning["pipe_wgs"] = pipe_wgs
+    print(f"[gpu] nspec={db.nspec} nstep={nstep} exact={exact} assembly={assembly} att_variant={att_variant}",
+          flush=True)
+    tuning = {} if att_variant is None else {"att_variant": att_variant}
     g = GPUSolver(db, src, rec, nstep=nstep, exact=exact, assembly=assembly, attenuation=attenuation,
                   tuning=tuning)
     out = g.run().copy()
@@ -36,40 +34,15 @@ def run_oracle(oracle_mod, db, src, rec, nstep, attenuation=None):
     return oracle_mod.OracleSolver(db, src, rec, "f32", attenuation=attenuation).run(nstep)
 
 
-@pytest.mark.parametrize("shape,pipe_wgs", [(0, None), (1, None), (2, None), (2, 5)])
 @pytest.mark.parametrize("warp", [False, True])
-def test_elastic_exact_ordered_bitwise(oracle_mod, warp, shape, pipe_wgs):
-    """Every stiffness kernel shape equals the oracle bit for bit (exact build, CPU element order);
-    shape 2 with one element per workgroup (default size at 64 elements) and with 5 persistent
-    workgroups walking the elements."""
+def test_elastic_exact_ordered_bitwise(oracle_mod, warp):
+    """The stiffness kernel equals the oracle bit for bit (exact build, CPU element order), on
+    regular and warped (irregular) elements."""
     db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0) if warp else None)
     ref = run_oracle(oracle_mod, db, src, rec, 200)
-    out = run_gpu(db, src, rec, 200, True, 1, shape=shape, pipe_wgs=pipe_wgs)
+    out = run_gpu(db, src, rec, 200, True, 1)
     assert np.abs(ref).max() > 0
     assert np.array_equal(out, ref), rel_l2(out, ref)
-
-
-def test_pipelined_elastic_coloured_and_strain():
-    """Shape 2 (pipelined, no attenuation) with coloured assembly equals shape 1's coloured run bit
-    for bit; with 3 persistent workgroups, the strain-storing variant of a kernel run (the
-    adjoint's forward) gives shape 1's kernels."""
-    from specfem3d_amd import adjoint as A
-    db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0))
-    c1 = run_gpu(db, src, rec, 150, True, 2, shape=1)
-    c2 = run_gpu(db, src, rec, 150, True, 2, shape=2, pipe_wgs=3)
-    assert np.array_equal(c2, c1), rel_l2(c2, c1)
-    out = []
-    for shape, pw in ((1, None), (2, 3)):
-        tuning = {"elastic_shape": shape}
-        if pw is not None:
-            tuning["pipe_wgs"] = pw
-        print(f"[gpu] kernel run, shape {shape}, pipe_wgs {pw}", flush=True)
-        ks = A.KernelSolver(db, src, rec, 120, nt_dump=16, exact=True, assembly=1, tuning=tuning)
-        syn = ks.forward()
-        out.append((syn,) + tuple(ks.adjoint(A.waveform_adjoint_source(syn, 0.5 * syn))))
-        ks.close()
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b), rel_l2(a, b)
 
 
 @pytest.mark.parametrize("exact", [True, False])
@@ -80,15 +53,14 @@ def test_elastic_atomic_within_tolerance(oracle_mod, exact):
     assert rel_l2(out, ref) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("att", [False, True])
-def test_coupled_acoustic_viscoelastic(oracle_mod, att, shape):
+def test_coupled_acoustic_viscoelastic(oracle_mod, att):
     from specfem3d_amd import workloads as W
     w = W.sea_over_visco_box(scale=0.125, nstep=200, with_attenuation=att, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
-    out = run_gpu(w.db, w.sources, w.receivers, 200, True, 1, w.attenuation, shape=shape)
+    out = run_gpu(w.db, w.sources, w.receivers, 200, True, 1, w.attenuation)
     assert np.array_equal(out, ref), rel_l2(out, ref)
-    fast = run_gpu(w.db, w.sources, w.receivers, 200, False, 0, w.attenuation, shape=shape)
+    fast = run_gpu(w.db, w.sources, w.receivers, 200, False, 0, w.attenuation)
     assert rel_l2(fast, ref) < 1e-5
 
 
@@ -100,51 +72,47 @@ def test_large_coupled_workload_reduced(oracle_mod):
     w = W.sea_over_visco_large(scale=0.1, nshots=2, nstep=150, nrec_side=3)
     src, rec, n = w.shots[1]
     ref = run_oracle(oracle_mod, w.db, src, rec, n, w.attenuation)
-    out = run_gpu(w.db, src, rec, n, True, 1, w.attenuation, shape=1)
+    out = run_gpu(w.db, src, rec, n, True, 1, w.attenuation)
     assert np.abs(ref).max() > 0
     assert np.array_equal(out, ref), rel_l2(out, ref)
-    fast = run_gpu(w.db, src, rec, n, False, 0, w.attenuation, shape=1)
+    fast = run_gpu(w.db, src, rec, n, False, 0, w.attenuation)
     assert rel_l2(fast, ref) < 1e-5
 
 
-@pytest.mark.parametrize("att_variant,pipe_wgs", [(1, None), (2, None), (3, None), (4, None), (5, None), (5, 7),
-                                                   (5, 100)])
-def test_attenuation_schedules_bitwise(oracle_mod, att_variant, pipe_wgs):
-    """The point kernel's attenuation-state schedules give the oracle's results bit for bit; the
-    pipelined schedule 5 with one element per workgroup (default size at this mesh) and with
-    7 or 100 persistent workgroups walking 864 elements (uneven walks)."""
+@pytest.mark.parametrize("att_variant", [0, 1, 2, 3, 4])
+def test_attenuation_schedules_bitwise(oracle_mod, att_variant):
+    """The point kernel's attenuation-state schedules give the oracle's results bit for bit."""
     from specfem3d_amd import workloads as W
     w = W.sea_over_visco_box(scale=0.125, nstep=150, with_attenuation=True, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 150, w.attenuation)
-    out = run_gpu(w.db, w.sources, w.receivers, 150, True, 1, w.attenuation, shape=1, att_variant=att_variant,
-                  pipe_wgs=pipe_wgs)
+    out = run_gpu(w.db, w.sources, w.receivers, 150, True, 1, w.attenuation, att_variant=att_variant)
     assert np.array_equal(out, ref), rel_l2(out, ref)
 
 
-def test_pipelined_schedule_irregular_and_coloured(oracle_mod):
-    """Schedule 5 on a warped (irregular) viscoelastic box: ordered assembly equals the oracle bit
-    for bit with 3 persistent workgroups, and coloured assembly equals schedule 4's coloured run."""
+def test_attenuation_irregular_ordered_and_coloured(oracle_mod):
+    """The default schedule on a warped (irregular) viscoelastic box: ordered assembly equals the
+    oracle bit for bit, coloured assembly stays within 1e-5 and is deterministic."""
     from specfem3d_amd.attenuation import setup_attenuation
     db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0))
     att = setup_attenuation(db, qmu=60.0, qkappa=9000.0)
     ref = run_oracle(oracle_mod, db, src, rec, 120, att)
-    out = run_gpu(db, src, rec, 120, True, 1, att, shape=1, att_variant=5, pipe_wgs=3)
+    out = run_gpu(db, src, rec, 120, True, 1, att)
     assert np.abs(ref).max() > 0
     assert np.array_equal(out, ref), rel_l2(out, ref)
-    c4 = run_gpu(db, src, rec, 120, True, 2, att, shape=1, att_variant=4)
-    c5 = run_gpu(db, src, rec, 120, True, 2, att, shape=1, att_variant=5, pipe_wgs=5)
-    assert np.array_equal(c5, c4), rel_l2(c5, c4)
+    c1 = run_gpu(db, src, rec, 120, True, 2, att)
+    c2 = run_gpu(db, src, rec, 120, True, 2, att)
+    assert np.array_equal(c1, c2)
+    assert rel_l2(c1, ref) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("exact", [True, False])
-def test_colored_assembly_deterministic_within_tolerance(oracle_mod, exact, shape):
+def test_colored_assembly_deterministic_within_tolerance(oracle_mod, exact):
     """Coloured scatter (no atomics): same answer on every run, within 1e-5 of the oracle."""
     from specfem3d_amd import workloads as W
     w = W.sea_over_visco_box(scale=0.125, nstep=200, with_attenuation=True, nrec_side=3)
     ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
-    a = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation, shape=shape)
-    b = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation, shape=shape)
+    a = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation)
+    b = run_gpu(w.db, w.sources, w.receivers, 200, exact, 2, w.attenuation)
     assert np.array_equal(a, b)
     assert rel_l2(a, ref) < 1e-5
 
