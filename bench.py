@@ -255,6 +255,122 @@ def batched_visco_measurement(args) -> dict:
                          "bytes_per_launch": kbytes, "avg_pass_ms": round(kms, 4), "step_ms": round(sms, 4)}}
 
 
+def homogeneous_box_measurement(args) -> dict:
+    """The 20^3 homogeneous elastic box (531,441 points, 2000 steps, the CPU-runnable case) through the
+    ABI call sequence with the library's defaults (FMA kernels, coloured assembly without attenuation),
+    the FP32 oracle on one host core over a sample of its steps, and the reference's own Fortran
+    solver figure for the same case (measured on another host)."""
+    import numpy as np
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.solver import GPUSolver
+    w = W.homogeneous_box()
+    db = w.db
+    n = w.nstep
+    g = GPUSolver(db, w.sources, w.receivers, nstep=n, exact=False, assembly=-1)
+    for _ in range(20):
+        g.step()
+    g.synchronize()
+    t = time.perf_counter()
+    for _ in range(n - 20):
+        g.step()
+    g.synchronize()
+    wall = time.perf_counter() - t
+    finite = bool(np.isfinite(g.seis_d[: g.seismo_current]).all())
+    g.close()
+    value = db.nglob * (n - 20) / wall
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        ora = oracle.OracleSolver(db, w.sources, w.receivers, "f32")
+        ns = 100
+        tc = time.perf_counter()
+        for _ in range(ns):
+            ora.step()
+        cpu = {"value": db.nglob * ns / (time.perf_counter() - tc), "unit": "GLL-point updates/s", "cores": 1,
+               "kind": "port", "sample": f"first {ns} of the {n} steps (FP32 C oracle, one thread)"}
+    log(f"[bench] 20^3 homogeneous box: {value:.3e} GLL-pt updates/s ({wall / (n - 20) * 1e3:.3f} ms/step), "
+        f"finite={finite}")
+    return {"workload": w.name, "nglob": db.nglob, "value": value, "unit": "GLL-point updates/s",
+            "ms_per_step": wall / (n - 20) * 1e3, "steps": n - 20, "assembly": "library default (coloured)",
+            "seismograms_finite": finite, "cpu_baseline": cpu,
+            "reference_fortran_cpu": {"value": 7.16e6, "unit": "GLL-point updates/s", "cores": 1,
+                                      "kind": "reference",
+                                      "sample": "the reference's own FP32 solver (flang -O3, serial), whole "
+                                                "2000-step run, measured on another host (8-core Xeon @ 2.70 GHz)"}}
+
+
+def coupled_batch_measurement(args) -> dict:
+    """The fluid-over-viscoelastic many-shot workload (the 160 x 160 x 80 builder: 10 % of the element
+    layers fluid, 3-SLS solid with Q_mu = 100, seeded random CMT shots below the sea floor, one
+    sea-floor receiver grid) at ``--coupled-scale``: ``--coupled-shots`` shots of ``--coupled-steps``
+    steps batched (autotuned, graph-replayed), the single-shot path on the same mesh, and the
+    checkpointed adjoint (forward with snapshots + adjoint with recomputation) of the first shot
+    against synthetics of a 3 % Gaussian mu perturbation."""
+    import numpy as np
+    from specfem3d_amd import adjoint as A
+    from specfem3d_amd import workloads as W
+    from specfem3d_amd.autograd import perturbed_database
+    from specfem3d_amd.batch import BatchSolver
+    from specfem3d_amd.solver import GPUSolver
+    n = args.coupled_steps
+    t0 = time.time()
+    w = W.sea_over_visco_large(scale=args.coupled_scale, nshots=args.coupled_shots, nstep=n)
+    db = w.db
+    shots = w.shots
+    log(f"[bench] coupled batch workload {w.name}: NGLOB {db.nglob}, built in {time.time() - t0:.1f}s")
+    b = BatchSolver(db, shots, exact=False, attenuation=w.attenuation)
+    choice = b.autotune(trial_steps=3)
+    log(f"[bench] coupled autotuner: {choice}")
+    kms, sms = b.time(10)
+    b.reset()
+    b.synchronize()
+    t = time.perf_counter()
+    b.run(graph_steps=10)
+    b.synchronize()
+    wall = time.perf_counter() - t
+    finite = all(bool(np.isfinite(b.seismograms(i)).all()) for i in range(len(shots)))
+    b.close()
+    value = db.nglob * float(n * len(shots)) / wall
+    src, rec, _ = shots[0]
+    ps = min(args.plain_steps, n)
+    pw = min(10, ps // 2)
+    g = GPUSolver(db, src, rec, nstep=ps, attenuation=w.attenuation, exact=False)
+    for _ in range(pw):
+        g.step()
+    g.synchronize()
+    tp = time.perf_counter()
+    for _ in range(ps - pw):
+        g.step()
+    g.synchronize()
+    plain = db.nglob * (ps - pw) / (time.perf_counter() - tp)
+    g.close()
+    log(f"[bench] coupled batch {value:.3e} GLL-pt updates/s in {wall:.2f}s (step {sms:.3f} ms), single shot "
+        f"{plain:.3e}, ratio {value / plain:.2f}, finite={finite}")
+    adj = None
+    if args.coupled_adjoint:
+        g = GPUSolver(perturbed_database(db, dlnmu=0.03 * W.gaussian_bump(db)), src, rec, nstep=n,
+                      attenuation=w.attenuation, exact=False)
+        obs = g.run().copy()
+        g.close()
+        ks = A.KernelSolver(db, src, rec, n, nt_dump=64, attenuation=w.attenuation)
+        ks.synchronize()
+        t = time.perf_counter()
+        syn = ks.forward()
+        tf = time.perf_counter() - t
+        rk, mk, kk = ks.adjoint(A.waveform_adjoint_source(syn, obs))
+        aw = time.perf_counter() - t
+        ks.close()
+        kfin = bool(np.isfinite(mk).all() and np.abs(mk).max() > 0)
+        log(f"[bench] coupled kernel run {aw:.2f}s (forward with snapshots {tf:.2f}s), kernels finite={kfin}")
+        adj = {"wall_s": round(aw, 3), "forward_s": round(tf, 3), "steps": n, "nt_dump": 64,
+               "kernels_finite": kfin, "unit": "s (forward with checkpoints + adjoint with recompute, one shot)"}
+    return {"workload": w.name, "scale": args.coupled_scale, "nglob": db.nglob, "value": value,
+            "unit": "GLL-point updates/s", "wall_s": round(wall, 3), "step_ms_all_live": round(sms, 4),
+            "plain_single_shot": plain, "ratio_vs_plain": value / plain, "autotuned": choice,
+            "seismograms_finite": finite, "adjoint": adj}
+
+
 def adjoint_measurement(args) -> dict:
     """Wall time of a checkpointed sensitivity-kernel run: forward with a snapshot every 64 steps +
     adjoint with recomputation (kernels copied back to the host included), on the 48^3 elastic
@@ -360,6 +476,8 @@ def main():
     ap.add_argument("--element-order", default="none",
                     choices=["morton", "none", "brick1x2", "brick2x1", "brick2x2", "brick1x4", "brick4x4", "brick2x4"],
                     help="launch order of the elastic elements of the single-shot run")
+    ap.add_argument("--interior-store", type=int, default=-1,
+                    help="1: atomic stiffness stores the element-interior points (-1: library default)")
     ap.add_argument("--att-variant", type=int, default=-1,
                     help="attenuation schedule of the single-shot stiffness kernel (-1: library default)")
     ap.add_argument("--skip-single", action="store_true",
@@ -379,6 +497,15 @@ def main():
     ap.add_argument("--adjoint-assembly", type=int, default=2,
                     help="assembly of the kernel run: 0 atomic, 1 ordered, 2 coloured (default: the fastest here)")
     ap.add_argument("--adjoint-scale", type=float, default=1.0, help="shrink the adjoint mesh (tests only)")
+    ap.add_argument("--homogeneous-box", type=int, default=1,
+                    help="1: also run the 20^3 homogeneous box single-shot figures")
+    ap.add_argument("--coupled-shots", type=int, default=8,
+                    help="shots of the batched fluid-over-viscoelastic workload (0: skip)")
+    ap.add_argument("--coupled-steps", type=int, default=300, help="steps of each batched coupled shot")
+    ap.add_argument("--coupled-scale", type=float, default=0.6,
+                    help="scale of the 160x160x80 coupled builder for the batched run (0.6: 96x96x48)")
+    ap.add_argument("--coupled-adjoint", type=int, default=1,
+                    help="1: also time the checkpointed adjoint of the first coupled shot")
     ap.add_argument("--large-steps", type=int, default=0,
                     help="steps of one shot of the 160x160x80 viscoelastic workload (0: skip; the mesh build "
                          "takes minutes)")
@@ -436,6 +563,7 @@ def main():
                "batched": batched_measurement(args) if args.batch_shots > 0 else None,
                "batched_viscoelastic": batched_visco_measurement(args) if args.visco_shots > 0 else None,
                "adjoint": adjoint_measurement(args) if args.adjoint_steps > 0 else None,
+               "coupled_batch": coupled_batch_measurement(args) if args.coupled_shots > 0 else None,
                "large": large_measurement(args) if args.large_steps > 0 else None}
         print(json.dumps(out), flush=True)
         return
@@ -449,6 +577,8 @@ def main():
     tuning = {}
     if args.att_variant >= 0:
         tuning["att_variant"] = args.att_variant
+    if args.interior_store >= 0:
+        tuning["interior_store"] = args.interior_store
     g = GPUSolver(db, w.sources, w.receivers, nstep=nstep_total, attenuation=w.attenuation,
                   exact=bool(args.exact), assembly=args.assembly,
                   element_order=None if args.element_order == "none" else args.element_order, tuning=tuning)
@@ -517,6 +647,8 @@ def main():
     adj = adjoint_measurement(args) if args.adjoint_steps > 0 else None
     batched = batched_measurement(args) if args.batch_shots > 0 else None
     batched_visco = batched_visco_measurement(args) if args.visco_shots > 0 else None
+    coupled = coupled_batch_measurement(args) if args.coupled_shots > 0 else None
+    box = homogeneous_box_measurement(args) if args.homogeneous_box else None
     large = large_measurement(args) if args.large_steps > 0 else None
 
     out = {
@@ -538,11 +670,14 @@ def main():
                    "receivers": w.receivers.n, "exact": args.exact,
                    "assembly": ["atomic", "ordered", "coloured"][args.assembly],
                    "element_order": args.element_order,
-                   "att_variant": "default" if args.att_variant < 0 else args.att_variant},
+                   "att_variant": "default" if args.att_variant < 0 else args.att_variant,
+                   "interior_store": "default" if args.interior_store < 0 else args.interior_store},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
         "batched_viscoelastic": batched_visco,
+        "coupled_batch": coupled,
+        "homogeneous_box": box,
         "adjoint": adj,
     }
     if large is not None:

@@ -239,6 +239,99 @@ void sync_copy_from_device_(long* Mesh_pointer, int* iphase, realw* send_buffer)
 void register_host_array_(int* size, realw* h_array);
 void unregister_host_array_(realw* h_array);
 
+/* ---- outside the single-partition hot path: these stop the run with a message (noise
+   tomography, faults, C-PML, gravity, ocean load, Hessian kernels, kernel smoothing, debug checks),
+   as the reference's own stubs do when it is built without a GPU; pause_for_debug does nothing.
+   Parameter names are placeholders. */
+/* src/gpu/compute_add_sources_viscoelastic_cuda.cu (stubs.c:186) */
+void add_source_main_rec_noise_cu_(long* p1, int* p2, int* p3, int* p4);
+/* src/gpu/check_fields_cuda.cu (stubs.c:145) */
+void check_error_vectors_(int* p1, realw* p2, realw* p3);
+/* src/gpu/check_fields_cuda.cu (stubs.c:142) */
+void check_max_norm_b_accel_(int* p1, realw* p2, int* p3);
+/* src/gpu/check_fields_cuda.cu (stubs.c:133) */
+void check_max_norm_b_accel_gpu_(int* p1, realw* p2, long* p3, int* p4);
+/* src/gpu/check_fields_cuda.cu (stubs.c:139) */
+void check_max_norm_b_displ_(int* p1, realw* p2, int* p3);
+/* src/gpu/check_fields_cuda.cu (stubs.c:130) */
+void check_max_norm_b_displ_gpu_(int* p1, realw* p2, long* p3, int* p4);
+/* src/gpu/check_fields_cuda.cu (stubs.c:136) */
+void check_max_norm_b_veloc_gpu_(int* p1, realw* p2, long* p3, int* p4);
+/* src/gpu/check_fields_cuda.cu (stubs.c:127) */
+void check_max_norm_displ_(int* p1, realw* p2, int* p3);
+/* src/gpu/check_fields_cuda.cu (stubs.c:121) */
+void check_max_norm_displ_gpu_(int* p1, realw* p2, long* p3, int* p4);
+/* src/gpu/check_fields_cuda.cu (stubs.c:124) */
+void check_max_norm_vector_(int* p1, realw* p2, int* p3);
+/* src/gpu/compute_coupling_cuda.cu (stubs.c:217) */
+void compute_coupling_ocean_cuda_(long* p1, int* p2);
+/* src/gpu/compute_kernels_cuda.cu (stubs.c:271) */
+void compute_kernels_hess_cuda_(long* p1, realw* p2, int* p3, int* p4);
+/* src/gpu/compute_kernels_cuda.cu (stubs.c:262) */
+void compute_kernels_strgth_noise_cu_(long* p1, realw* p2, realw* p3);
+/* src/gpu/smooth_cuda.cu (stubs.c:665) */
+void compute_smooth_gpu_(long* p1, realw* p2, realw* p3, realw* p4, realw* p5, realw* p6, int* p7);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:427) */
+void fault_solver_gpu_(long* p1, long* p2, realw* p3, int* p4);
+/* src/gpu/check_fields_cuda.cu (stubs.c:118) */
+void get_max_accel_(int* p1, int* p2, long* p3);
+/* src/gpu/smooth_cuda.cu (stubs.c:674) */
+void get_smooth_gpu_(long* p1, realw* p2);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:330) */
+void initialize_fault_data_gpu_(long* p1, int* p2, int* p3, int* p4, int* p5, int* p6);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:323) */
+void initialize_fault_solver_gpu_(long* p1, int* p2, realw* p3, realw* p4, int* p5);
+/* src/gpu/noise_tomography_cuda.cu (stubs.c:457) */
+void make_displ_rand_(long* p1, realw* p2);
+/* src/gpu/noise_tomography_cuda.cu (stubs.c:463) */
+void noise_read_add_surface_movie_cu_(long* p1, realw* p2, int* p3);
+/* src/gpu/helper_functions.cu (stubs.c:438) */
+void pause_for_debug_(void);
+/* src/gpu/pml_compute_accel_cuda.cu (stubs.c:473) */
+void pml_impose_boundary_condition_elastic_cuda_(long* p1);
+/* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:637) */
+void prepare_fault_device_(long* p1, int* p2, realw* p3);
+/* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:598) */
+void prepare_fields_elastic_pml_(long* p1, int* p2, int* p3, int* p4, int* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, realw* p11);
+/* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:629) */
+void prepare_fields_gravity_device_(long* p1, int* p2, realw* p3, realw* p4, realw* p5, realw* p6);
+/* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:616) */
+void prepare_fields_noise_device_(long* p1, int* p2, int* p3, int* p4, int* p5, int* p6, int* p7, int* p8, realw* p9, realw* p10, realw* p11, realw* p12, realw* p13, realw* p14);
+/* src/gpu/smooth_cuda.cu (stubs.c:653) */
+void prepare_smooth_gpu_(long* p1, realw* p2, realw* p3, realw* p4, realw* p5, realw* p6, realw* p7, realw* p8, int* p9, int* p10);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:861) */
+void transfer_compute_kernel_answers_from_device_(long* p1, realw* p2, int* p3, realw* p4, int* p5, realw* p6, int* p7);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:867) */
+void transfer_compute_kernel_fields_from_device_(long* p1, realw* p2, int* p3, realw* p4, int* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, int* p11, realw* p12, realw* p13, realw* p14, realw* p15, realw* p16, int* p17, realw* p18, int* p19, realw* p20, int* p21, realw* p22, int* p23, realw* p24, realw* p25, int* p26);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:365) */
+void transfer_datat_to_host_(long* p1, int* p2, realw* p3, int* p4);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:338) */
+void transfer_fault_data_to_device_(long* p1, int* p2, int* p3, int* p4, realw* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, realw* p11, realw* p12, realw* p13, int* p14, int* p15, int* p16);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:356) */
+void transfer_fault_data_to_host_(long* p1, int* p2, int* p3, int* p4, realw* p5, realw* p6, realw* p7);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:854) */
+void transfer_kernels_hess_ac_tohost_(long* p1, realw* p2, realw* p3, realw* p4, int* p5);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:846) */
+void transfer_kernels_hess_el_tohost_(long* p1, realw* p2, realw* p3, realw* p4, realw* p5, int* p6);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:776) */
+void transfer_kernels_noise_to_host_(long* p1, realw* p2, int* p3);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:735) */
+void transfer_pml_displ_from_device_(int* p1, realw* p2, realw* p3, long* p4);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:738) */
+void transfer_pml_displ_to_device_(int* p1, realw* p2, realw* p3, long* p4);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:371) */
+void transfer_rsf_data_todevice_(long* p1, int* p2, int* p3, realw* p4, realw* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, realw* p11, realw* p12, realw* p13, realw* p14, realw* p15, int* p16);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:400) */
+void transfer_rsf_data_tohost_(long* p1, int* p2, int* p3, realw* p4, realw* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, realw* p11, realw* p12, realw* p13, realw* p14);
+/* src/gpu/transfer_fields_cuda.cu (stubs.c:720) */
+void transfer_sigma_from_device_(int* p1, realw* p2, long* p3);
+/* src/gpu/noise_tomography_cuda.cu (stubs.c:459) */
+void transfer_surface_to_host_(long* p1, realw* p2);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:389) */
+void transfer_swf_data_todevice_(long* p1, int* p2, int* p3, realw* p4, realw* p5, realw* p6, realw* p7, realw* p8, realw* p9);
+/* src/gpu/fault_solver_dynamics.cu (stubs.c:416) */
+void transfer_swf_data_tohost_(long* p1, int* p2, int* p3, realw* p4, realw* p5, realw* p6, realw* p7, realw* p8, realw* p9);
+
 /* ---- library extensions (no counterpart in the reference interface) ---------------------- */
 /* exact = 1: element kernels without fused multiply-add (the CPU solver's rounding);
    assembly = 0: float atomic scatter-add; 1: element contributions summed per point in element
@@ -269,8 +362,10 @@ void sf_checkpoints_free(long* Mesh_pointer);
 void sf_zero_kernels(long* Mesh_pointer);
 /* batched many-shot forward engine (no counterpart in the reference, which runs one shot per
    process): shots with their own sources, receivers and record lengths stepped together on one
-   prepared elastic mesh (isotropic, optionally 3-SLS viscoelastic: each shot then carries its own
-   memory variables) in shot-interleaved wavefields; element ids 0-based */
+   prepared mesh (isotropic elastic, optionally 3-SLS viscoelastic: each shot then carries its own
+   memory variables; fluid elements coupled to the solid: each shot then carries its own
+   potentials) in shot-interleaved wavefields; sources and receivers lie in elastic elements;
+   element ids 0-based */
 long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep, realw deltat, realw deltatover2,
                      realw deltatsqover2);
 void sf_batch_add_source(long batch, int shot, int ispec, const realw* sourcearray, const realw* stf);
@@ -283,9 +378,7 @@ void sf_batch_finalize(long batch);
 void sf_batch_set_options(long batch, int shots_per_group, int assembly);
 void sf_batch_get_options(long batch, int* shots_per_group, int* assembly);
 /* kernel variants with the same results: "lds_overlay" 0/1 (displacement tile and stress products
-   share LDS), 2 (persistent pipelined kernel), 3 (shot-packed tiles: one 16-byte LDS read serves
-   four shots; 2 and 3 for elastic shots with coloured assembly); "pipe_wgs" >= 0
-   (workgroups of the pipelined kernel, 0 = its default); unknown names are fatal */
+   share LDS); unknown names are fatal */
 void sf_batch_set_tuning(long batch, const char* name, int value);
 int sf_batch_get_tuning(long batch, const char* name);
 double sf_batch_autotune(long batch, int trial_steps);
@@ -293,6 +386,8 @@ void sf_batch_run(long batch, int nsteps, int graph_steps);
 int sf_batch_current_step(long batch);
 void sf_batch_seismograms(long batch, int shot, realw* out);
 void sf_batch_displ(long batch, int shot, realw* out);
+/* potential of one shot at every global point (zero outside the fluid elements) */
+void sf_batch_potential(long batch, int shot, realw* out);
 void sf_batch_reset(long batch);
 void sf_batch_time(long batch, int nsteps, double* stiffness_ms, double* step_ms);
 void sf_batch_destroy(long batch);

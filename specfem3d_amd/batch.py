@@ -6,7 +6,8 @@ re-runs the whole loop per shot. ``BatchSolver`` prepares the mesh once through 
 wavefields (``csrc/batch.hip``). The shots have their own sources, receivers and record lengths.
 Shots-per-workgroup and the assembly mode come from the library's autotuner, and time steps are
 replayed from captured hipGraphs. Each shot's seismograms are bit-identical to a single-shot
-``GPUSolver`` run with the same options.
+``GPUSolver`` run with the same options. Meshes with fluid elements (coupled to the solid) step each
+shot's potentials too; sources and receivers lie in the solid.
 """
 from __future__ import annotations
 
@@ -107,6 +108,12 @@ class BatchSolver:
     def displ(self, shot: int) -> np.ndarray:
         out = np.zeros(3 * self.db.nglob, np.float32)
         self.L.sf_batch_displ(self.h, C.c_int(shot), abi.ptr(out))
+        return out
+
+    def potential(self, shot: int) -> np.ndarray:
+        """Potential of one shot at every global point (zero outside the fluid elements)."""
+        out = np.zeros(self.db.nglob, np.float32)
+        self.L.sf_batch_potential(self.h, C.c_int(shot), abi.ptr(out))
         return out
 
     def close(self):

@@ -7,6 +7,11 @@
 //   * layout: value (g, c, s) at ((int64_t)g * 3 + c) * S + s (shot-fastest rows, see k_batch.hip);
 //   * shots are placed in decreasing order of record length, so the live shots at any step are
 //     a prefix of the row and finished shots cost no memory traffic;
+//   * a mesh with fluid elements also steps S interleaved potentials over its acoustic range of
+//     global points (value (g, s) at (g - g0) * S + s): per step the potential predictor, free
+//     surface, acoustic stiffness, fluid Stacey and fluid side of the coupling, the corrector and
+//     the free surface again come first, then the solid side of the coupling with the elastic
+//     boundary terms (the single-shot call order of iterate_time.F90);
 //   * one step = fused corrector/predictor, Stacey rounds, source rounds, stiffness (one launch per
 //     element colour with plain read-modify-writes, or one atomic launch), receivers, and a
 //     one-lane kernel that advances the device step counter. Assembly mode 3 instead records the
@@ -27,6 +32,8 @@
 
 namespace sf {
 void ensure_element_colors(Mesh* m);
+void ensure_acoustic_colors(Mesh* m);
+bool element_is_elastic(Mesh* m, int e);
 
 struct Batch {
   Mesh* m = nullptr;
@@ -45,6 +52,10 @@ struct Batch {
   realw* u_last = nullptr;       // final displacement of slots dropped by compaction (stride S0)
   std::vector<int> h_nstep;      // per slot (S0 entries, decreasing, padding 0)
   bool att = false;              // 3-SLS attenuation: per-shot memory variables and strain
+  bool ac = false;               // fluid elements: per-shot potentials over the acoustic range
+  int64_t ac_g0 = 0, ac_n = 0;   // acoustic range [ac_g0, ac_g0 + ac_n) of global points
+  realw *pot = nullptr, *potd = nullptr, *potdd = nullptr;
+  realw* p_last = nullptr;       // final potential of slots dropped by compaction (stride S0)
   realw *R = nullptr, *eps = nullptr;
   unsigned char* flags = nullptr;  // fused mode: BF_* per local point (first/last colour, cleared)
   int flags_version = -1;        // Mesh::color_version the flags were built for (-1: stale)
@@ -205,6 +216,121 @@ __global__ void __launch_bounds__(64) k_batch_seismo(int nrec, int S, const int*
   }
 }
 
+// ---- fluid domain: potentials over the acoustic range, value (g, s) at (g - g0) * S + s
+// Newmark on the potential rows of live shots (k_update.hip arithmetic): correct = 0 the
+// predictor chi += dt chi' + dt^2/2 chi'', chi' += dt/2 chi'', chi'' = 0; correct = 1 the corrector
+// chi'' *= rmass, chi' += dt/2 chi''. A lane takes four consecutive shots of one row.
+__global__ void __launch_bounds__(256) k_batch_ac_newmark(int64_t nrow, int S, int rows_per_block,
+                                                          const int* __restrict__ nstep, const int* __restrict__ itp,
+                                                          realw* __restrict__ pot, realw* __restrict__ potd,
+                                                          realw* __restrict__ potdd, const realw* __restrict__ rm,
+                                                          realw dt, realw dt2, realw dtsq2, int correct) {
+  const int nq = S >> 2;
+  const int rl = (int)threadIdx.x / nq;
+  const int qd = (int)threadIdx.x - rl * nq;
+  if (rl >= rows_per_block) return;
+  const int64_t row = (int64_t)blockIdx.x * rows_per_block + rl;
+  if (row >= nrow) return;
+  const int it = load_it(itp);
+  const int s = 4 * qd;
+  if (nstep[s] < it) return;
+  const bool l1 = nstep[s + 1] >= it, l2 = nstep[s + 2] >= it, l3 = nstep[s + 3] >= it;
+  const int64_t o = row * S + s;
+  const float4 V0 = *reinterpret_cast<const float4*>(potd + o);
+  const float4 A0 = *reinterpret_cast<const float4*>(potdd + o);
+  float4 V = V0, A = A0;
+  if (correct) {
+    const realw M = rm[row];
+#define SF_AC(c) A.c = A.c * M; V.c = V.c + dt2 * A.c;
+    SF_AC(x) SF_AC(y) SF_AC(z) SF_AC(w)
+#undef SF_AC
+    if (!l1) { V.y = V0.y; A.y = A0.y; }
+    if (!l2) { V.z = V0.z; A.z = A0.z; }
+    if (!l3) { V.w = V0.w; A.w = A0.w; }
+    *reinterpret_cast<float4*>(potd + o) = V;
+    *reinterpret_cast<float4*>(potdd + o) = A;
+    return;
+  }
+  const float4 U0 = *reinterpret_cast<const float4*>(pot + o);
+  float4 U = U0;
+#define SF_AP(c) U.c = U.c + dt * V.c + dtsq2 * A.c; V.c = V.c + dt2 * A.c; A.c = 0.f;
+  SF_AP(x) SF_AP(y) SF_AP(z) SF_AP(w)
+#undef SF_AP
+  if (!l1) { U.y = U0.y; V.y = V0.y; A.y = A0.y; }
+  if (!l2) { U.z = U0.z; V.z = V0.z; A.z = A0.z; }
+  if (!l3) { U.w = U0.w; V.w = V0.w; A.w = A0.w; }
+  *reinterpret_cast<float4*>(pot + o) = U;
+  *reinterpret_cast<float4*>(potd + o) = V;
+  *reinterpret_cast<float4*>(potdd + o) = A;
+}
+
+// free surface (k_free_surface): zero chi, chi', chi'' of every (free-surface point, live shot)
+__global__ void k_batch_free_surface(int n, int S, int64_t g0, const int* __restrict__ ig,
+                                     const int* __restrict__ nstep, const int* __restrict__ itp, realw* pot,
+                                     realw* potd, realw* potdd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // n * S < 2^31 (checked by the caller)
+  const int q = t / S;
+  const int s = t - q * S;
+  if (q >= n) return;
+  if (nstep[s] < load_it(itp)) return;
+  const int64_t b = ((int64_t)ig[q] - g0) * S + s;
+  pot[b] = 0.f;
+  potd[b] = 0.f;
+  potdd[b] = 0.f;
+}
+
+// fluid Stacey (k_stacey_ac arithmetic) for every (face point of one round, live shot)
+__global__ void k_batch_stacey_ac(int n, int S, int64_t g0, const int* __restrict__ ig, const realw* __restrict__ jw,
+                                  const realw* __restrict__ rho, const realw* __restrict__ cp,
+                                  const int* __restrict__ nstep, const int* __restrict__ itp,
+                                  const realw* __restrict__ potd, realw* potdd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = t / S;
+  const int s = t - q * S;
+  if (q >= n) return;
+  if (nstep[s] < load_it(itp)) return;
+  const int64_t b = ((int64_t)ig[q] - g0) * S + s;
+  const realw absorbl = potd[b] * jw[q] / cp[q] / rho[q];
+  potdd[b] = potdd[b] - absorbl;
+}
+
+// fluid side of the coupling (k_coupling_ac_el arithmetic, forward field): chi'' += w (u . n)
+__global__ void k_batch_coupling_ac_el(int n, int S, int64_t g0, const int* __restrict__ ig,
+                                       const realw* __restrict__ nrm, const realw* __restrict__ jw,
+                                       const int* __restrict__ nstep, const int* __restrict__ itp,
+                                       const realw* __restrict__ u, realw* potdd) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = t / S;
+  const int s = t - q * S;
+  if (q >= n) return;
+  if (nstep[s] < load_it(itp)) return;
+  const int64_t g = ig[q];
+  const int64_t e = g * 3 * S + s;
+  const realw ux = u[e], uy = u[e + S], uz = u[e + 2 * S];
+  const realw un = ux * nrm[3 * q] + uy * nrm[3 * q + 1] + uz * nrm[3 * q + 2];
+  const int64_t b = (g - g0) * S + s;
+  potdd[b] = potdd[b] + jw[q] * un;
+}
+
+// solid side of the coupling (k_coupling_el_ac arithmetic, forward field): a += w n (-chi'')
+__global__ void k_batch_coupling_el_ac(int n, int S, int64_t g0, const int* __restrict__ ig,
+                                       const realw* __restrict__ nrm, const realw* __restrict__ jw,
+                                       const int* __restrict__ nstep, const int* __restrict__ itp,
+                                       const realw* __restrict__ potdd, realw* accel) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = t / S;
+  const int s = t - q * S;
+  if (q >= n) return;
+  if (nstep[s] < load_it(itp)) return;
+  const int64_t g = ig[q];
+  const realw pr = -potdd[(g - g0) * S + s];
+  const realw w = jw[q];
+  const int64_t e = g * 3 * S + s;
+  accel[e] = accel[e] + w * nrm[3 * q] * pr;
+  accel[e + S] = accel[e + S] + w * nrm[3 * q + 1] * pr;
+  accel[e + 2 * S] = accel[e + 2 * S] + w * nrm[3 * q + 2] * pr;
+}
+
 __global__ void k_batch_tick(int* itp) {
   if (threadIdx.x == 0) itp[0] = itp[0] + 1;
 }
@@ -310,6 +436,21 @@ static void compact(Batch* b, int S2) {
     bfree(*f);
     *f = d;
   }
+  if (b->ac) {
+    const int64_t na = b->ac_n;
+    if (!b->p_last) b->p_last = balloc<realw>(na * b->S0);
+    hipLaunchKernelGGL(k_batch_archive, stride_grid(na * (b->S - S2)), dim3(256), 0, st, na, b->S, S2, b->S0,
+                       b->pot, b->p_last);
+    SF_LAUNCH_CHECK("k_batch_archive");
+    for (realw** f : {&b->pot, &b->potd, &b->potdd}) {
+      realw* d = balloc<realw>(na * S2);
+      hipLaunchKernelGGL(k_batch_restride, stride_grid(na * (S2 / 4)), dim3(256), 0, st, na, b->S, S2, *f, d);
+      SF_LAUNCH_CHECK("k_batch_restride");
+      SF_CHECK(hipStreamSynchronize(st));
+      bfree(*f);
+      *f = d;
+    }
+  }
   if (b->att) {
     const int64_t ne = m->nspec;
     for (auto fb : {std::make_pair(&b->R, 18 * 125), std::make_pair(&b->eps, 6 * 125)}) {
@@ -343,6 +484,12 @@ static void restore_width(Batch* b) {
     bfree(*f);
     *f = balloc<realw>(n);
   }
+  if (b->ac) {
+    for (realw** f : {&b->pot, &b->potd, &b->potdd}) {
+      bfree(*f);
+      *f = balloc<realw>(b->ac_n * b->S0);
+    }
+  }
   if (b->att) {
     bfree(b->R);
     bfree(b->eps);
@@ -358,6 +505,11 @@ static void zero_state(Batch* b) {
   const int64_t n = 3 * (int64_t)m->nglob * b->S;
   for (realw* p : {b->u, b->v, b->a})
     hipLaunchKernelGGL(k_batch_fill, sf_grid(n, 256), dim3(256), 0, m->stream, n, p);
+  if (b->ac) {
+    const int64_t na = b->ac_n * b->S;
+    for (realw* p : {b->pot, b->potd, b->potdd})
+      hipLaunchKernelGGL(k_batch_fill, sf_grid(na, 256), dim3(256), 0, m->stream, na, p);
+  }
   if (b->att) {
     const int64_t nr = 18 * (int64_t)m->nspec * 125 * b->S, ne = 6 * (int64_t)m->nspec * 125 * b->S;
     hipLaunchKernelGGL(k_batch_fill, sf_grid(nr, 256), dim3(256), 0, m->stream, nr, b->R);
@@ -416,6 +568,12 @@ static void build_point_flags(Batch* b) {
   if (m->stacey && m->abs_el.n > 0) {
     std::vector<int> ig(m->abs_el.n);
     SF_CHECK(hipMemcpy(ig.data(), m->abs_el.iglob, sizeof(int) * ig.size(), hipMemcpyDeviceToHost));
+    for (int g : ig) forced[g] = 1;
+  }
+  if (b->ac && m->coupling.n > 0) {
+    // the solid side of the coupling adds to these points before the colours
+    std::vector<int> ig(m->coupling.n);
+    SF_CHECK(hipMemcpy(ig.data(), m->coupling.iglob, sizeof(int) * ig.size(), hipMemcpyDeviceToHost));
     for (int g : ig) forced[g] = 1;
   }
   for (const Batch::Src& q : b->srcs)
@@ -491,6 +649,85 @@ static void enqueue_seismo(Batch* b, hipStream_t st) {
   SF_LAUNCH_CHECK("k_batch_seismo");
 }
 
+// launches over (face point of one round, live shot) pairs
+static dim3 pair_grid(int cnt, int S, const char* what) {
+  if ((int64_t)cnt * S >= 0x7fffffff) fatal("sf_batch", what);
+  return sf_grid((int64_t)cnt * S, 256);
+}
+
+// the fluid part of a step (the acoustic calls of iterate_time.F90 before the elastic ones):
+// predictor, free surface, Stacey, coupling (fluid side), stiffness (all elements are inner at
+// NPROC = 1, so the stiffness runs in phase 2, after the boundary terms of phase 1), corrector,
+// free surface
+static void enqueue_fluid(Batch* b, hipStream_t st) {
+  Mesh* m = b->m;
+  const int S = b->S;
+  const int rpb = 256 / (S / 4);
+  const int64_t na = b->ac_n;
+  const dim3 rows = sf_grid((na + rpb - 1) / rpb, 1);
+  hipLaunchKernelGGL(k_batch_ac_newmark, rows, dim3(256), 0, st, na, S, rpb, b->d_nstep, b->d_it, b->pot, b->potd,
+                     b->potdd, m->rmass_ac + b->ac_g0, b->dt, b->dt2, b->dtsq2, 0);
+  SF_LAUNCH_CHECK("k_batch_ac_newmark");
+  const FacePoints& fs = m->free_ac;
+  if (fs.n > 0)
+    hipLaunchKernelGGL(k_batch_free_surface, pair_grid(fs.n, S, "too many free-surface points x shots"), dim3(256),
+                       0, st, fs.n, S, b->ac_g0, fs.iglob, b->d_nstep, b->d_it, b->pot, b->potd, b->potdd);
+  if (m->stacey) {
+    const FacePoints& f = m->abs_ac;
+    for (int r = 0; r < f.nrounds; r++) {
+      const int o = f.round_off[r], cnt = f.round_off[r + 1] - o;
+      if (cnt <= 0) continue;
+      hipLaunchKernelGGL(k_batch_stacey_ac, pair_grid(cnt, S, "too many fluid Stacey points x shots"), dim3(256), 0,
+                         st, cnt, S, b->ac_g0, f.iglob + o, f.jacw + o, f.imp_p + o, f.imp_s + o, b->d_nstep,
+                         b->d_it, b->potd, b->potdd);
+    }
+    SF_LAUNCH_CHECK("k_batch_stacey_ac");
+  }
+  const FacePoints& cf = m->coupling;
+  for (int r = 0; r < cf.nrounds; r++) {
+    const int o = cf.round_off[r], cnt = cf.round_off[r + 1] - o;
+    if (cnt <= 0) continue;
+    hipLaunchKernelGGL(k_batch_coupling_ac_el, pair_grid(cnt, S, "too many coupling points x shots"), dim3(256), 0,
+                       st, cnt, S, b->ac_g0, cf.iglob + o, cf.normal + 3 * o, cf.jacw + o, b->d_nstep, b->d_it, b->u,
+                       b->potdd);
+  }
+  SF_LAUNCH_CHECK("k_batch_coupling_ac_el");
+  BatchAcArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.ngroups = S / b->sc;
+  A.S = S;
+  A.g0 = b->ac_g0;
+  A.ibool = m->ibool;
+  A.irregular = m->irregular;
+  A.xix_regular = m->xix_regular;
+  A.jacobian_regular = m->jacobian_regular;
+  A.jac = m->jac;
+  A.jac_plane = (int64_t)m->nspec_irregular * 125;
+  A.rho = m->rho;
+  A.pot = b->pot;
+  A.potdd = b->potdd;
+  A.nstep = b->d_nstep;
+  A.it = b->d_it;
+  if (b->mode == 0) {
+    A.elems = m->acoustic_elems;
+    A.nelem = m->nspec_acoustic;
+    launch_acoustic_batch(st, A, b->sc, 0, m->exact != 0);
+  } else {
+    for (int c = 0; c < m->ac_ncolor; c++) {
+      A.elems = m->ac_color_elems + m->ac_color_off[c];
+      A.nelem = m->ac_color_off[c + 1] - m->ac_color_off[c];
+      launch_acoustic_batch(st, A, b->sc, 2, m->exact != 0);
+    }
+  }
+  hipLaunchKernelGGL(k_batch_ac_newmark, rows, dim3(256), 0, st, na, S, rpb, b->d_nstep, b->d_it, b->pot, b->potd,
+                     b->potdd, m->rmass_ac + b->ac_g0, b->dt, b->dt2, b->dtsq2, 1);
+  SF_LAUNCH_CHECK("k_batch_ac_newmark");
+  if (fs.n > 0)
+    hipLaunchKernelGGL(k_batch_free_surface, pair_grid(fs.n, S, "too many free-surface points x shots"), dim3(256),
+                       0, st, fs.n, S, b->ac_g0, fs.iglob, b->d_nstep, b->d_it, b->pot, b->potd, b->potdd);
+  SF_LAUNCH_CHECK("k_batch_free_surface");
+}
+
 static void enqueue_step(Batch* b, hipStream_t st, bool first) {
   Mesh* m = b->m;
   const bool fused = b->mode == 3;
@@ -504,6 +741,7 @@ static void enqueue_step(Batch* b, hipStream_t st, bool first) {
                        b->d_nstep, b->d_it, b->u, b->v, b->a, m->rmass, dt2, dt, dt2, dtsq2, first ? 1 : 0);
     SF_LAUNCH_CHECK("k_batch_update");
   }
+  if (b->ac) enqueue_fluid(b, st);
   if (m->stacey) {
     const FacePoints& f = m->abs_el;
     for (int r = 0; r < f.nrounds; r++) {
@@ -515,6 +753,17 @@ static void enqueue_step(Batch* b, hipStream_t st, bool first) {
                          b->v, b->a);
     }
     SF_LAUNCH_CHECK("k_batch_stacey");
+  }
+  if (b->ac) {
+    const FacePoints& cf = m->coupling;
+    for (int r = 0; r < cf.nrounds; r++) {
+      const int o = cf.round_off[r], cnt = cf.round_off[r + 1] - o;
+      if (cnt <= 0) continue;
+      hipLaunchKernelGGL(k_batch_coupling_el_ac, pair_grid(cnt, b->S, "too many coupling points x shots"), dim3(256),
+                         0, st, cnt, b->S, b->ac_g0, cf.iglob + o, cf.normal + 3 * o, cf.jacw + o, b->d_nstep,
+                         b->d_it, b->potdd, b->a);
+    }
+    SF_LAUNCH_CHECK("k_batch_coupling_el_ac");
   }
   for (int r = 0; r < b->src_nrounds; r++) {
     const int o = b->src_round_off[r], cnt = b->src_round_off[r + 1] - o;
@@ -582,7 +831,8 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   Mesh* m = from_handle(Mesh_pointer);
   if (nshots <= 0) fatal("sf_batch_create", "need at least one shot");
   if (!m->rmass || !m->kappamu || m->nspec_elastic <= 0) fatal("sf_batch_create", "mesh has no prepared elastic domain");
-  if (m->nspec_acoustic > 0) fatal("sf_batch_create", "batched shots cover elastic meshes only");
+  if (m->nspec_acoustic > 0 && (!m->ac[0].pot || !m->rmass_ac))
+    fatal("sf_batch_create", "mesh has fluid elements but prepare_fields_acoustic_device was not called");
   if (m->attenuation && !m->fc_index && !m->factor_common)
     fatal("sf_batch_create", "attenuation factors not prepared");
   Batch* b = new Batch();
@@ -592,6 +842,12 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   b->dtsq2 = deltatsqover2;
   b->nshots = nshots;
   ensure_element_colors(m);
+  if (m->nspec_acoustic > 0) {
+    ensure_acoustic_colors(m);
+    b->ac = true;
+    b->ac_g0 = m->ac_g0;
+    b->ac_n = m->ac_n;
+  }
   b->S = (nshots + 7) / 8 * 8;
   if (b->S > 1024) fatal("sf_batch_create", "at most 1024 shots per batch");
   b->nstep_user.assign(nstep, nstep + nshots);
@@ -615,6 +871,11 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   b->u = balloc<realw>(n);
   b->v = balloc<realw>(n);
   b->a = balloc<realw>(n);
+  if (b->ac) {
+    b->pot = balloc<realw>(b->ac_n * b->S);
+    b->potd = balloc<realw>(b->ac_n * b->S);
+    b->potdd = balloc<realw>(b->ac_n * b->S);
+  }
   if (m->attenuation) {
     b->att = true;
     b->mode = 3;
@@ -631,6 +892,8 @@ extern "C" void sf_batch_add_source(long h, int shot, int ispec, const realw* so
   Batch* b = batch_of(h);
   if (shot < 0 || shot >= b->nshots) fatal("sf_batch_add_source", "shot index out of range");
   if (ispec < 0 || ispec >= b->m->nspec) fatal("sf_batch_add_source", "element index out of range");
+  if (!element_is_elastic(b->m, ispec))
+    fatal("sf_batch_add_source", "batched sources must lie in elastic elements (single-shot runs take fluid ones)");
   Batch::Src s;
   s.shot = shot;
   s.ispec = ispec;
@@ -647,6 +910,9 @@ extern "C" void sf_batch_add_receiver(long h, int shot, int ispec, const double*
   Batch* b = batch_of(h);
   if (shot < 0 || shot >= b->nshots) fatal("sf_batch_add_receiver", "shot index out of range");
   if (ispec < 0 || ispec >= b->m->nspec) fatal("sf_batch_add_receiver", "element index out of range");
+  if (!element_is_elastic(b->m, ispec))
+    fatal("sf_batch_add_receiver",
+          "batched receivers must lie in elastic elements (single-shot runs record fluid ones)");
   Batch::Rec r;
   r.shot = shot;
   r.ispec = ispec;
@@ -816,6 +1082,23 @@ extern "C" void sf_batch_displ(long h, int shot, realw* out) {
   for (int64_t q = 0; q < n; q++) out[q] = all[(size_t)q * stride + s];
 }
 
+// Potential of one shot over all global points (zero outside the acoustic range; all zero on a mesh
+// without fluid elements). Synchronises the stream.
+extern "C" void sf_batch_potential(long h, int shot, realw* out) {
+  Batch* b = batch_of(h);
+  if (shot < 0 || shot >= b->nshots) fatal("sf_batch_potential", "shot index out of range");
+  SF_CHECK(hipStreamSynchronize(b->m->stream));
+  const int64_t n = b->m->nglob;
+  for (int64_t q = 0; q < n; q++) out[q] = 0.f;
+  if (!b->ac) return;
+  const int s = b->slot_of[shot];
+  const bool dropped = s >= b->S;
+  const int stride = dropped ? b->S0 : b->S;
+  std::vector<realw> all((size_t)b->ac_n * stride);
+  SF_CHECK(hipMemcpy(all.data(), dropped ? b->p_last : b->pot, sizeof(realw) * all.size(), hipMemcpyDeviceToHost));
+  for (int64_t q = 0; q < b->ac_n; q++) out[b->ac_g0 + q] = all[(size_t)q * stride + s];
+}
+
 // Rewind to step 1 with zero wavefields and seismograms (keeps sources, receivers and options).
 extern "C" void sf_batch_reset(long h) {
   Batch* b = batch_of(h);
@@ -902,6 +1185,7 @@ extern "C" void sf_batch_destroy(long h) {
   drop_graph(b);
   bfree(b->d_nstep); bfree(b->d_it); bfree(b->u); bfree(b->v); bfree(b->a); bfree(b->R); bfree(b->eps);
   bfree(b->u_last);
+  bfree(b->pot); bfree(b->potd); bfree(b->potdd); bfree(b->p_last);
   bfree(b->flags);
   bfree(b->src_ig); bfree(b->src_slot); bfree(b->src_id); bfree(b->src_ploc); bfree(b->src_arr); bfree(b->src_stf);
   bfree(b->rec_slot); bfree(b->rec_ig); bfree(b->rec_h); bfree(b->rec_nu); bfree(b->seis);

@@ -56,6 +56,15 @@ void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, i
     launch_elastic_batch_fast(s, a, sc, mode, overlay);
 }
 
+void launch_acoustic_batch_exact(hipStream_t, const BatchAcArgs&, int, int);
+void launch_acoustic_batch_fast(hipStream_t, const BatchAcArgs&, int, int);
+void launch_acoustic_batch(hipStream_t s, const BatchAcArgs& a, int sc, int mode, bool exact) {
+  if (exact)
+    launch_acoustic_batch_exact(s, a, sc, mode);
+  else
+    launch_acoustic_batch_fast(s, a, sc, mode);
+}
+
 void launch_acoustic(hipStream_t s, const AcousticArgs& a, bool exact, int mode) {
   if (exact)
     launch_acoustic_exact(s, a, mode);

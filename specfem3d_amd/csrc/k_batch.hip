@@ -441,6 +441,113 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
   }
 }
 
+// Batched acoustic stiffness (compute_forces_acoustic.F90:35-726; the arithmetic of k_acoustic per
+// point and shot): chi'' -= div((1/rho) grad chi) for SC = 4 NQ shots of one element per
+// workgroup. Lane = (GLL point p, quad of 4 consecutive shots) as in k_elastic_batch; the
+// potential tile and the three products (rho^-1 jacobian grad chi . grad xi, eta, gamma) go through
+// LDS, one row of PST floats per (plane, shot). Every sum runs in k_acoustic's order (the gamma
+// contraction reads the same values the column kernel holds in registers), so a batched shot
+// reproduces the single-shot run with the same assembly. MODE 0 atomic scatter, 2 plain
+// read-modify-write (the launch holds one colour).
+template <int NQ, int MODE>
+__global__ void __launch_bounds__(128 * NQ) k_acoustic_batch(BatchAcArgs B, GLLk G) {
+  constexpr int SC = 4 * NQ;
+  constexpr int PST = 132;
+  constexpr int CPL = SC * PST;
+  __shared__ realw sBuf[4 * CPL];                 // [0] potential tile, [1..3] products t1, t2, t3
+  __shared__ realw sG[75];
+  stage_gll(sG, G, threadIdx.x);
+  const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int b = xcd_remap_b(blockIdx.x, gridDim.x);
+  const int ie = b / B.ngroups;
+  const int s0 = (b - ie * B.ngroups) * SC;
+  if (B.nstep[s0] < it) return;
+  const int q = threadIdx.x % NQ, p = threadIdx.x / NQ;
+  const int sq = s0 + 4 * q;
+  bool live[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) live[r] = B.nstep[sq + r] >= it;
+  const bool active = p < 125;
+  const int pp = active ? p : 124;
+  const int e = B.elems[ie];
+  const int irr = B.irregular[e];
+  const int64_t S = B.S;
+  const int g = B.ibool[(int64_t)e * 125 + pp];
+  const int64_t row = ((int64_t)g - B.g0) * S + sq;
+  const int ts = 4 * q;
+  float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (live[0]) c0 = *reinterpret_cast<const float4*>(B.pot + row);
+  const realw rho_invl = (realw)1 / B.rho[(int64_t)e * 125 + pp];
+  JacPt J = {};
+  if (irr) J = load_jac(B.jac, B.jac_plane, (int64_t)(irr - 1) * 125 + pp);
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) sBuf[(ts + r) * PST + p] = c4(c0, r);
+  }
+  lds_barrier();
+  const int k = pp / 25, j = (pp / 5) % 5, i = pp % 5;
+  const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
+  const realw* hp = sG;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const realw* s = sBuf + (ts + r) * PST;
+    realw a1 = hp[i * 5] * s[b1];
+    realw a2 = hp[j * 5] * s[b2];
+    realw a3 = hp[k * 5] * s[b3];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      a1 = a1 + hp[i * 5 + l] * s[b1 + l];
+      a2 = a2 + hp[j * 5 + l] * s[b2 + 5 * l];
+      a3 = a3 + hp[k * 5 + l] * s[b3 + 25 * l];
+    }
+    realw t1, t2, t3;
+    if (irr) {
+      const realw dpdx = J.xix * a1 + J.etax * a2 + J.gax * a3;
+      const realw dpdy = J.xiy * a1 + J.etay * a2 + J.gay * a3;
+      const realw dpdz = J.xiz * a1 + J.etaz * a2 + J.gaz * a3;
+      t1 = rho_invl * J.jac * (J.xix * dpdx + J.xiy * dpdy + J.xiz * dpdz);
+      t2 = rho_invl * J.jac * (J.etax * dpdx + J.etay * dpdy + J.etaz * dpdz);
+      t3 = rho_invl * J.jac * (J.gax * dpdx + J.gay * dpdy + J.gaz * dpdz);
+    } else {
+      const realw jl = B.jacobian_regular, xr = B.xix_regular;
+      t1 = rho_invl * jl * xr * xr * a1;
+      t2 = rho_invl * jl * xr * xr * a2;
+      t3 = rho_invl * jl * xr * xr * a3;
+    }
+    if (active) {
+      sBuf[(1 * SC + ts + r) * PST + p] = t1;
+      sBuf[(2 * SC + ts + r) * PST + p] = t2;
+      sBuf[(3 * SC + ts + r) * PST + p] = t3;
+    }
+  }
+  lds_barrier();
+  if (!active || !live[0]) return;
+  const realw* hpw = sG + 25;
+  const realw* wxy = sG + 50;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (MODE == 2) acc = *reinterpret_cast<const float4*>(B.potdd + row);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const realw* s1 = sBuf + (1 * SC + ts + r) * PST;
+    const realw* s2 = sBuf + (2 * SC + ts + r) * PST;
+    const realw* s3 = sBuf + (3 * SC + ts + r) * PST;
+    realw n1 = hpw[i] * s1[b1];
+    realw n2 = hpw[j] * s2[b2];
+    realw n3 = hpw[k] * s3[b3];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      n1 = n1 + hpw[l * 5 + i] * s1[b1 + l];
+      n2 = n2 + hpw[l * 5 + j] * s2[b2 + 5 * l];
+      n3 = n3 + hpw[l * 5 + k] * s3[b3 + 25 * l];
+    }
+    const realw f = wxy[j * 5 + k] * n1 + wxy[i * 5 + k] * n2 + wxy[i * 5 + j] * n3;
+    if (!live[r]) continue;
+    if (MODE == 0) atomicAdd(B.potdd + row + r, -f);
+    else q4(acc, r) = q4(acc, r) + (-f);
+  }
+  if (MODE == 2) *reinterpret_cast<float4*>(B.potdd + row) = acc;
+}
+
 }  // namespace SF_VNS
 using namespace SF_VNS;
 
@@ -492,6 +599,25 @@ void SFX(launch_visco_batch)(hipStream_t st, const BatchArgs& a, int sc, int mod
   }
 #undef SF_KV
   SF_LAUNCH_CHECK("k_visco_batch");
+}
+
+void SFX(launch_acoustic_batch)(hipStream_t st, const BatchAcArgs& a, int sc, int mode) {
+  if (a.nelem <= 0 || a.ngroups <= 0) return;
+  const int64_t nb = (int64_t)a.nelem * a.ngroups;
+  if (nb > 0x7fffffff) fatal("launch_acoustic_batch", "grid too large");
+  dim3 grid((unsigned)nb);
+  const GLLk G = g_gll_host;
+#define SF_KA(NQ, M) hipLaunchKernelGGL((k_acoustic_batch<NQ, M>), grid, dim3(128 * NQ), 0, st, a, G)
+  if (sc != 4 && sc != 8) fatal("launch_acoustic_batch", "shots per workgroup must be 4 or 8");
+  if (mode == 0) {
+    if (sc == 4) SF_KA(1, 0);
+    else SF_KA(2, 0);
+  } else {
+    if (sc == 4) SF_KA(1, 2);
+    else SF_KA(2, 2);
+  }
+#undef SF_KA
+  SF_LAUNCH_CHECK("k_acoustic_batch");
 }
 
 }  // namespace sf

@@ -53,6 +53,11 @@ struct ElasticArgs {
   int64_t nloc;            // nspec*125
   realw alpha[3], beta[3], gamma[3];
   realw* local_out;        // ordered assembly: 3 planes of nloc
+  // atomic assembly: 1 when `accel` is zero at every element-interior point before this launch
+  // except in the elements flagged in interior_busy (sources, adjoint sources): the 27 interior
+  // points of the other elements are then stored instead of added (they belong to one element)
+  int interior_free;
+  const unsigned char* interior_busy;  // per element
 };
 // mode: 0 atomic scatter, 1 store local contributions (ordered gather), 2 plain read-modify-write
 // (the element list is one colour: its elements share no global point)
@@ -105,6 +110,28 @@ enum : unsigned char { BF_FIRST = 1, BF_LAST = 2, BF_ZERO = 4 };
 void launch_elastic_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, int overlay, bool exact);
 // viscoelastic (3 SLS) batched stiffness; mode 2 or 3
 void launch_visco_batch(hipStream_t s, const BatchArgs& a, int sc, int mode, bool exact);
+
+// batched acoustic stiffness (k_batch.hip): potentials interleaved shot-fastest over the acoustic
+// range of global points, value (g, s) at (g - g0) * S + s
+struct BatchAcArgs {
+  int nelem;
+  const int* elems;
+  int ngroups;             // S / shots-per-workgroup
+  int S;                   // padded shot count (row stride)
+  int64_t g0;              // first global point of the acoustic range
+  const int* ibool;
+  const int* irregular;
+  realw xix_regular, jacobian_regular;
+  const realw* jac;
+  int64_t jac_plane;
+  const realw* rho;        // rhostore, nspec*125
+  const realw* pot;
+  realw* potdd;
+  const int* nstep;        // per shot slot, decreasing (padding slots 0)
+  const int* it;           // device step counter, 1-based current step
+};
+// sc = shots per workgroup (4, 8); mode 0 atomic scatter, 2 plain read-modify-write (one colour)
+void launch_acoustic_batch(hipStream_t s, const BatchAcArgs& a, int sc, int mode, bool exact);
 
 struct AcousticArgs {
   int nelem;

@@ -40,38 +40,49 @@ extern "C" void FC_FUNC_(register_host_array, X)(int*, realw*) {}
 extern "C" void FC_FUNC_(unregister_host_array, X)(realw*) {}
 extern "C" void FC_FUNC_(pause_for_debug, X)() {}
 
-// ---- out of scope: stop with a message
-#define SF_UNSUPPORTED(NAME, WHAT) \
-  extern "C" void FC_FUNC_(NAME, X)() { fatal(#NAME, WHAT " is not supported by this library"); }
-SF_UNSUPPORTED(compute_coupling_ocean_cuda, "ocean load")
-SF_UNSUPPORTED(compute_kernels_hess_cuda, "Hessian kernels")
-SF_UNSUPPORTED(compute_kernels_strgth_noise_cu, "noise tomography")
-SF_UNSUPPORTED(add_source_main_rec_noise_cu, "noise tomography")
-SF_UNSUPPORTED(make_displ_rand, "noise tomography")
-SF_UNSUPPORTED(transfer_surface_to_host, "noise tomography")
-SF_UNSUPPORTED(noise_read_add_surface_movie_cu, "noise tomography")
-SF_UNSUPPORTED(prepare_fields_noise_device, "noise tomography")
-SF_UNSUPPORTED(transfer_kernels_noise_to_host, "noise tomography")
-SF_UNSUPPORTED(transfer_sigma_from_device, "noise tomography")
-SF_UNSUPPORTED(prepare_fields_gravity_device, "gravity")
-SF_UNSUPPORTED(prepare_fields_elastic_pml, "C-PML")
-SF_UNSUPPORTED(pml_impose_boundary_condition_elastic_cuda, "C-PML")
-SF_UNSUPPORTED(transfer_pml_displ_from_device, "C-PML")
-SF_UNSUPPORTED(transfer_pml_displ_to_device, "C-PML")
-SF_UNSUPPORTED(prepare_fault_device, "fault simulation")
-SF_UNSUPPORTED(initialize_fault_solver_gpu, "fault simulation")
-SF_UNSUPPORTED(initialize_fault_data_gpu, "fault simulation")
-SF_UNSUPPORTED(transfer_fault_data_to_device, "fault simulation")
-SF_UNSUPPORTED(transfer_fault_data_to_host, "fault simulation")
-SF_UNSUPPORTED(transfer_datat_to_host, "fault simulation")
-SF_UNSUPPORTED(transfer_rsf_data_todevice, "fault simulation")
-SF_UNSUPPORTED(transfer_swf_data_todevice, "fault simulation")
-SF_UNSUPPORTED(transfer_rsf_data_tohost, "fault simulation")
-SF_UNSUPPORTED(transfer_swf_data_tohost, "fault simulation")
-SF_UNSUPPORTED(fault_solver_gpu, "fault simulation")
-SF_UNSUPPORTED(prepare_smooth_gpu, "kernel smoothing")
-SF_UNSUPPORTED(compute_smooth_gpu, "kernel smoothing")
-SF_UNSUPPORTED(get_smooth_gpu, "kernel smoothing")
-SF_UNSUPPORTED(transfer_kernels_hess_el_tohost, "Hessian kernels")
-SF_UNSUPPORTED(transfer_kernels_hess_ac_tohost, "Hessian kernels")
-SF_UNSUPPORTED(get_max_accel, "get_max_accel (debug)")
+// ---- out of scope: stop with a message (parameter lists as the reference declares them)
+#define SF_UNSUPPORTED(NAME, WHAT, ...) \
+  extern "C" void FC_FUNC_(NAME, X)(__VA_ARGS__) { fatal(#NAME, WHAT " is not supported by this library"); }
+SF_UNSUPPORTED(add_source_main_rec_noise_cu, "noise tomography", long*, int*, int*, int*)
+SF_UNSUPPORTED(check_error_vectors, "debug check routine", int*, realw*, realw*)
+SF_UNSUPPORTED(check_max_norm_b_accel, "debug check routine", int*, realw*, int*)
+SF_UNSUPPORTED(check_max_norm_b_accel_gpu, "debug check routine", int*, realw*, long*, int*)
+SF_UNSUPPORTED(check_max_norm_b_displ, "debug check routine", int*, realw*, int*)
+SF_UNSUPPORTED(check_max_norm_b_displ_gpu, "debug check routine", int*, realw*, long*, int*)
+SF_UNSUPPORTED(check_max_norm_b_veloc_gpu, "debug check routine", int*, realw*, long*, int*)
+SF_UNSUPPORTED(check_max_norm_displ, "debug check routine", int*, realw*, int*)
+SF_UNSUPPORTED(check_max_norm_displ_gpu, "debug check routine", int*, realw*, long*, int*)
+SF_UNSUPPORTED(check_max_norm_vector, "debug check routine", int*, realw*, int*)
+SF_UNSUPPORTED(compute_coupling_ocean_cuda, "ocean load", long*, int*)
+SF_UNSUPPORTED(compute_kernels_hess_cuda, "Hessian kernels", long*, realw*, int*, int*)
+SF_UNSUPPORTED(compute_kernels_strgth_noise_cu, "noise tomography", long*, realw*, realw*)
+SF_UNSUPPORTED(compute_smooth_gpu, "kernel smoothing", long*, realw*, realw*, realw*, realw*, realw*, int*)
+SF_UNSUPPORTED(fault_solver_gpu, "fault simulation", long*, long*, realw*, int*)
+SF_UNSUPPORTED(get_max_accel, "debug check routine", int*, int*, long*)
+SF_UNSUPPORTED(get_smooth_gpu, "kernel smoothing", long*, realw*)
+SF_UNSUPPORTED(initialize_fault_data_gpu, "fault simulation", long*, int*, int*, int*, int*, int*)
+SF_UNSUPPORTED(initialize_fault_solver_gpu, "fault simulation", long*, int*, realw*, realw*, int*)
+SF_UNSUPPORTED(make_displ_rand, "noise tomography", long*, realw*)
+SF_UNSUPPORTED(noise_read_add_surface_movie_cu, "noise tomography", long*, realw*, int*)
+SF_UNSUPPORTED(pml_impose_boundary_condition_elastic_cuda, "C-PML", long*)
+SF_UNSUPPORTED(prepare_fault_device, "fault simulation", long*, int*, realw*)
+SF_UNSUPPORTED(prepare_fields_elastic_pml, "C-PML", long*, int*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*)
+SF_UNSUPPORTED(prepare_fields_gravity_device, "gravity", long*, int*, realw*, realw*, realw*, realw*)
+SF_UNSUPPORTED(prepare_fields_noise_device, "noise tomography", long*, int*, int*, int*, int*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*)
+SF_UNSUPPORTED(prepare_smooth_gpu, "kernel smoothing", long*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*)
+SF_UNSUPPORTED(transfer_compute_kernel_answers_from_device, "debug check routine", long*, realw*, int*, realw*, int*, realw*, int*)
+SF_UNSUPPORTED(transfer_compute_kernel_fields_from_device, "debug check routine", long*, realw*, int*, realw*, int*, realw*, realw*, realw*, realw*, realw*, int*, realw*, realw*, realw*, realw*, realw*, int*, realw*, int*, realw*, int*, realw*, int*, realw*, realw*, int*)
+SF_UNSUPPORTED(transfer_datat_to_host, "fault simulation", long*, int*, realw*, int*)
+SF_UNSUPPORTED(transfer_fault_data_to_device, "fault simulation", long*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*, int*)
+SF_UNSUPPORTED(transfer_fault_data_to_host, "fault simulation", long*, int*, int*, int*, realw*, realw*, realw*)
+SF_UNSUPPORTED(transfer_kernels_hess_ac_tohost, "Hessian kernels", long*, realw*, realw*, realw*, int*)
+SF_UNSUPPORTED(transfer_kernels_hess_el_tohost, "Hessian kernels", long*, realw*, realw*, realw*, realw*, int*)
+SF_UNSUPPORTED(transfer_kernels_noise_to_host, "noise tomography", long*, realw*, int*)
+SF_UNSUPPORTED(transfer_pml_displ_from_device, "C-PML", int*, realw*, realw*, long*)
+SF_UNSUPPORTED(transfer_pml_displ_to_device, "C-PML", int*, realw*, realw*, long*)
+SF_UNSUPPORTED(transfer_rsf_data_todevice, "fault simulation", long*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*)
+SF_UNSUPPORTED(transfer_rsf_data_tohost, "fault simulation", long*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*)
+SF_UNSUPPORTED(transfer_sigma_from_device, "noise tomography", int*, realw*, long*)
+SF_UNSUPPORTED(transfer_surface_to_host, "noise tomography", long*, realw*)
+SF_UNSUPPORTED(transfer_swf_data_todevice, "fault simulation", long*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*)
+SF_UNSUPPORTED(transfer_swf_data_tohost, "fault simulation", long*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*)

@@ -335,6 +335,14 @@ class GPUSolver:
                                                abi.ptr(self.accel), self.MP)
         return self.displ.copy(), self.veloc.copy(), self.accel.copy()
 
+    def potentials(self):
+        """(potential, first and second time derivatives) of the fluid copied back from the device."""
+        A = abi.Args()
+        n = self.db.nglob
+        out = [np.zeros(n, np.float32) for _ in range(3)]
+        self.L.transfer_fields_ac_from_device_(A.i(n), abi.ptr(out[0]), abi.ptr(out[1]), abi.ptr(out[2]), self.MP)
+        return tuple(out)
+
     def max_norm(self) -> float:
         out = C.c_float(0)
         self.L.get_norm_elastic_from_device_(C.byref(out), self.MP, self._ref(1))
