@@ -388,6 +388,15 @@ void sf_batch_seismograms(long batch, int shot, realw* out);
 void sf_batch_displ(long batch, int shot, realw* out);
 /* potential of one shot at every global point (zero outside the fluid elements) */
 void sf_batch_potential(long batch, int shot, realw* out);
+/* No counterpart in the reference (its get_norm_*_from_device allocate a partial-maximum buffer on
+   every call); for callers that cannot allocate (code under HIP graph capture, serving loops): the
+   stability norm of get_norm_elastic_from_device_ / get_norm_acoustic_from_device_ (largest |u|
+   for ncomp = 3, |chi| for ncomp = 1, NaN counted as +inf) of the device field `field` (npts points)
+   into the device word `out`, enqueued on `stream` (a hipStream_t, or null) with no allocation, no
+   synchronisation and no memset. Its scratch is a fixed 512 KiB device array per GPU, shared by
+   every call on that GPU: calls that overlap on one GPU race. Returns 0, or -1 with the reason in
+   sf_last_error() (npts above 268435456, ncomp not 1 or 3, null pointers). */
+int sf_norm_noalloc(const realw* field, long long npts, int ncomp, realw* out, void* stream);
 void sf_batch_reset(long batch);
 void sf_batch_time(long batch, int nsteps, double* stiffness_ms, double* step_ms);
 void sf_batch_destroy(long batch);

@@ -50,7 +50,7 @@ EXPORTED = [
     "sf_history_reserve", "sf_history_store", "sf_history_load", "sf_checkpoints_free", "sf_zero_kernels",
     "sf_batch_create", "sf_batch_add_source", "sf_batch_add_receiver", "sf_batch_finalize", "sf_batch_set_options",
     "sf_batch_get_options", "sf_batch_set_tuning", "sf_batch_get_tuning", "sf_batch_autotune", "sf_batch_run", "sf_batch_current_step", "sf_batch_seismograms",
-    "sf_batch_displ", "sf_batch_potential", "sf_batch_reset", "sf_batch_time", "sf_batch_destroy", "sf_set_jacobian", "sf_synchronize", "sf_last_error",
+    "sf_batch_displ", "sf_batch_potential", "sf_batch_reset", "sf_batch_time", "sf_batch_destroy", "sf_set_jacobian", "sf_synchronize", "sf_last_error", "sf_norm_noalloc",
 ]
 
 
