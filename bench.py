@@ -349,21 +349,31 @@ def coupled_batch_measurement(args) -> dict:
         f"{plain:.3e}, ratio {value / plain:.2f}, finite={finite}")
     adj = None
     if args.coupled_adjoint:
-        g = GPUSolver(perturbed_database(db, dlnmu=0.03 * W.gaussian_bump(db)), src, rec, nstep=n,
-                      attenuation=w.attenuation, exact=False)
+        na = args.coupled_adjoint_steps
+        from specfem3d_amd.attenuation import estimate_min_resolved_period, setup_attenuation
+        dbp = perturbed_database(db, dlnmu=0.03 * W.gaussian_bump(db))
+        # the attenuated moduli of the observed model come from its own (perturbed) mu; same SLS band
+        att_obs = setup_attenuation(dbp, qmu=100.0, qkappa=9000.0,
+                                    min_resolved_period=estimate_min_resolved_period(db))
+        g = GPUSolver(dbp, src, rec, nstep=na, attenuation=att_obs, exact=False)
         obs = g.run().copy()
         g.close()
-        ks = A.KernelSolver(db, src, rec, n, nt_dump=64, attenuation=w.attenuation)
+        ks = A.KernelSolver(db, src, rec, na, nt_dump=64, attenuation=w.attenuation)
         ks.synchronize()
         t = time.perf_counter()
         syn = ks.forward()
         tf = time.perf_counter() - t
         rk, mk, kk = ks.adjoint(A.waveform_adjoint_source(syn, obs))
         aw = time.perf_counter() - t
+        ka = ks.acoustic_kernels
         ks.close()
+        chi = A.waveform_misfit(syn, obs, db.dt)
         kfin = bool(np.isfinite(mk).all() and np.abs(mk).max() > 0)
-        log(f"[bench] coupled kernel run {aw:.2f}s (forward with snapshots {tf:.2f}s), kernels finite={kfin}")
-        adj = {"wall_s": round(aw, 3), "forward_s": round(tf, 3), "steps": n, "nt_dump": 64,
+        log(f"[bench] coupled kernel run {aw:.2f}s (forward with snapshots {tf:.2f}s), misfit {chi:.4e}, "
+            f"max |mu_kl| {float(np.abs(mk).max()):.3e}, non-finite mu/rho/kappa "
+            f"{int((~np.isfinite(mk)).sum())}/{int((~np.isfinite(rk)).sum())}/{int((~np.isfinite(kk)).sum())}, "
+            f"fluid kernels finite={bool(np.isfinite(ka[0]).all() and np.isfinite(ka[1]).all())}, kernels finite={kfin}")
+        adj = {"wall_s": round(aw, 3), "forward_s": round(tf, 3), "steps": na, "nt_dump": 64, "misfit": chi,
                "kernels_finite": kfin, "unit": "s (forward with checkpoints + adjoint with recompute, one shot)"}
     return {"workload": w.name, "scale": args.coupled_scale, "nglob": db.nglob, "value": value,
             "unit": "GLL-point updates/s", "wall_s": round(wall, 3), "step_ms_all_live": round(sms, 4),
@@ -476,8 +486,6 @@ def main():
     ap.add_argument("--element-order", default="none",
                     choices=["morton", "none", "brick1x2", "brick2x1", "brick2x2", "brick1x4", "brick4x4", "brick2x4"],
                     help="launch order of the elastic elements of the single-shot run")
-    ap.add_argument("--interior-store", type=int, default=-1,
-                    help="1: atomic stiffness stores the element-interior points (-1: library default)")
     ap.add_argument("--att-variant", type=int, default=-1,
                     help="attenuation schedule of the single-shot stiffness kernel (-1: library default)")
     ap.add_argument("--skip-single", action="store_true",
@@ -506,6 +514,8 @@ def main():
                     help="scale of the 160x160x80 coupled builder for the batched run (0.6: 96x96x48)")
     ap.add_argument("--coupled-adjoint", type=int, default=1,
                     help="1: also time the checkpointed adjoint of the first coupled shot")
+    ap.add_argument("--coupled-adjoint-steps", type=int, default=1500,
+                    help="steps of the coupled checkpointed adjoint run")
     ap.add_argument("--large-steps", type=int, default=0,
                     help="steps of one shot of the 160x160x80 viscoelastic workload (0: skip; the mesh build "
                          "takes minutes)")
@@ -577,8 +587,6 @@ def main():
     tuning = {}
     if args.att_variant >= 0:
         tuning["att_variant"] = args.att_variant
-    if args.interior_store >= 0:
-        tuning["interior_store"] = args.interior_store
     g = GPUSolver(db, w.sources, w.receivers, nstep=nstep_total, attenuation=w.attenuation,
                   exact=bool(args.exact), assembly=args.assembly,
                   element_order=None if args.element_order == "none" else args.element_order, tuning=tuning)
@@ -670,8 +678,7 @@ def main():
                    "receivers": w.receivers.n, "exact": args.exact,
                    "assembly": ["atomic", "ordered", "coloured"][args.assembly],
                    "element_order": args.element_order,
-                   "att_variant": "default" if args.att_variant < 0 else args.att_variant,
-                   "interior_store": "default" if args.interior_store < 0 else args.interior_store},
+                   "att_variant": "default" if args.att_variant < 0 else args.att_variant},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

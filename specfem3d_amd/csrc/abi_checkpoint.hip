@@ -97,7 +97,6 @@ extern "C" void sf_snapshot_load(long* Mesh_pointer, int FORWARD_OR_ADJOINT, int
     ck_copy(m, m->el[f].displ, s.el, 3 * n);
     ck_copy(m, m->el[f].veloc, s.el + 3 * n, 3 * n);
     ck_copy(m, m->el[f].accel, s.el + 6 * n, 3 * n);
-    m->accel_clean[f] = 0;
   }
   if (m->ac[f].pot) {
     if (!s.ac) fatal("sf_snapshot_load", "snapshot holds no acoustic field");

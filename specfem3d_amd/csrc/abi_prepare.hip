@@ -374,12 +374,7 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
   }
   m->nsources = (int)src_ispec.size();
   H->src_ispec = src_ispec;
-  {
-    // elements whose interior points receive source terms before the stiffness
-    std::vector<unsigned char> busy(m->nspec, 0);
-    for (int e : src_ispec) busy[e] = 1;
-    m->interior_busy = dupload(busy.data(), (int64_t)m->nspec);
-  }
+
   (void)nsources_local_f;
   // sources are added in the domain of their element: elastic sources to accel
   // (compute_add_sources_viscoelastic.F90:143), acoustic ones to potential_dot_dot_acoustic
@@ -959,18 +954,7 @@ extern "C" void FC_FUNC_(prepare_sim2_or_3_const_device, PREPARE_SIM2_OR_3_CONST
   std::vector<realw> h((size_t)m->nadj_rec_local * 125);
   int ir = 0;
   m->h_adj_elastic = new unsigned char[m->nadj_rec_local];
-  {
-    // adjoint sources add to the interior points of their receiver elements before the stiffness
-    std::vector<unsigned char> busy(m->nspec, 0);
-    SF_CHECK(hipMemcpy(busy.data(), m->interior_busy, busy.size(), hipMemcpyDeviceToHost));
-    int r2 = 0;
-    for (int r = 0; r < *nrec && r2 < m->nadj_rec_local; r++) {
-      if (h_islice_selected_rec[r] != m->myrank) continue;
-      busy[h_ispec_selected_rec[r] - 1] = 1;
-      r2++;
-    }
-    SF_CHECK(hipMemcpy(m->interior_busy, busy.data(), busy.size(), hipMemcpyHostToDevice));
-  }
+
   for (int r = 0; r < *nrec && ir < m->nadj_rec_local; r++) {
     if (h_islice_selected_rec[r] != m->myrank) continue;
     const int e = h_ispec_selected_rec[r] - 1;
@@ -1036,7 +1020,6 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   dfree(m->rho_kl); dfree(m->mu_kl); dfree(m->kappa_kl); dfree(m->rho_ac_kl); dfree(m->kappa_ac_kl);
   dfree(m->asm_ptr); dfree(m->asm_idx); dfree(m->asm_buf);
   dfree(m->asm_ac_ptr); dfree(m->asm_ac_idx); dfree(m->asm_ac_buf);
-  dfree(m->interior_busy);
   dfree(m->d_norm);
   if (m->h_norm) (void)hipHostFree(m->h_norm);
   if (m->timing_ev) {
@@ -1082,18 +1065,14 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
 }
 
 // Tuning switches read by the step calls: "att_variant" (0..4: attenuation-state scheduling of the
-// point kernel, same results; 4 is the default), "interior_store" (atomic assembly: 1 stores the
-// element-interior points of the step's first stiffness launch instead of adding them, same
-// values), "acoustic_kernel_field" (below). Unknown names are a fatal error.
+// point kernel, same results; 4 is the default), "acoustic_kernel_field" (below). Unknown names
+// are a fatal error.
 extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
   Mesh* m = from_handle(Mesh_pointer);
   SF_CHECK(hipStreamSynchronize(m->stream));
   if (std::strcmp(name, "att_variant") == 0) {
     if (value < 0 || value > 4) fatal("sf_set_tuning", "att_variant must be 0..4");
     m->att_variant = value;
-  } else if (std::strcmp(name, "interior_store") == 0) {
-    if (value < 0 || value > 1) fatal("sf_set_tuning", "interior_store must be 0 or 1");
-    m->interior_store = value;
   } else if (std::strcmp(name, "acoustic_kernel_field") == 0) {
     // which adjoint array compute_kernels_acoustic_cuda pairs with the reconstructed field:
     // 0 = potential_acoustic (compute_kernels.f90:269 and :325), 1 = potential_dot_dot_acoustic

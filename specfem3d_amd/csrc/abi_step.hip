@@ -77,14 +77,11 @@ static ElasticArgs elastic_args(Mesh* m, int f, realw deltat) {
     a.gamma[s] = f ? m->b_gammaval[s] : m->gammaval[s];
   }
   a.local_out = m->asm_buf;
-  a.interior_busy = m->interior_busy;
-  a.interior_free = (m->interior_store && m->interior_busy && m->accel_clean[f]) ? 1 : 0;
   return a;
 }
 
 void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
   ElasticArgs a = elastic_args(m, f, deltat);
-  m->accel_clean[f] = 0;  // accel holds this launch's sums from here on
   const bool ordered = m->assembly == 1 && m->asm_ready;
   const bool colored = m->assembly == 2 && m->el_color_elems;
   const bool timed = m->timing_count < m->timing_max;
@@ -132,7 +129,6 @@ extern "C" void FC_FUNC_(update_displacement_cuda, UPDATE_DISPLACMENT_CUDA)(long
     launch_predictor(m->stream, n, w.displ + o, w.veloc + o, w.accel + o, *deltat_F, *deltatover2_F,
                      *deltatsqover2_F);
   }
-  m->accel_clean[f] = 1;  // both predictors leave accel = 0 on the elastic range
 }
 
 extern "C" void FC_FUNC_(update_displacement_ac_cuda, UPDATE_DISPLACEMENT_AC_CUDA)(long* Mesh_pointer,
@@ -166,7 +162,6 @@ extern "C" void FC_FUNC_(kernel_3_a_cuda, KERNEL_3_A_CUDA)(long* Mesh_pointer, r
     if (foa == 3 && f == 0) continue;
     if (!m->el[f].displ) continue;
     flush_el(m, f);
-    m->accel_clean[f] = 0;
     m->pending_k3_el[f] = 1;
     m->pending_dt2_el[f] = f ? *b_deltatover2_F : *deltatover2_F;
   }
@@ -650,8 +645,7 @@ static void d2h(Mesh* m, realw* h, const realw* d, int64_t n) {
     flush_el(m, F);                                                                      \
     DIR(m, m->el[F].MEMBER, arr, *size);                                                 \
   }
-// a host write of an elastic field may leave accel non-zero at element-interior points
-#define H2D(m, d, h, n) (h2d(m, d, h, n), (m)->accel_clean[0] = (m)->accel_clean[1] = 0)
+#define H2D(m, d, h, n) h2d(m, d, h, n)
 #define D2H(m, d, h, n) d2h(m, h, d, n)
 
 SF_XFER3(transfer_fields_el_to_device, 0, H2D)

@@ -14,6 +14,15 @@
 
 #include "kernels.h"
 
+#ifndef SF_EXACT
+#define SF_EXACT 1
+#endif
+// FMA builds fold, on regular elements, xix_regular into hprime and jacobian_regular *
+// xix_regular into hprimewgll (the tables at sG + 75 and sG + 100): the nine-term mapping of the
+// gradient and the eighteen multiplies of the test-function products become copies. The exact
+// builds keep the reference's operation order.
+#define SF_FOLD (!SF_EXACT)
+
 namespace sf {
 
 // Workgroup barrier that orders LDS accesses only: this wave's LDS operations complete
@@ -87,6 +96,10 @@ __device__ __forceinline__ Du9 map_gradient(const Grad9& t, bool irr, const JacP
     d.zx = J.xix * t.z1 + J.etax * t.z2 + J.gax * t.z3;
     d.zy = J.xiy * t.z1 + J.etay * t.z2 + J.gay * t.z3;
     d.zz = J.xiz * t.z1 + J.etaz * t.z2 + J.gaz * t.z3;
+  } else if (SF_FOLD) {
+    d.xx = t.x1; d.xy = t.x2; d.xz = t.x3;
+    d.yx = t.y1; d.yy = t.y2; d.yz = t.y3;
+    d.zx = t.z1; d.zy = t.z2; d.zz = t.z3;
   } else {
     d.xx = xr * t.x1; d.xy = xr * t.x2; d.xz = xr * t.x3;
     d.yx = xr * t.y1; d.yy = xr * t.y2; d.yz = xr * t.y3;
@@ -128,6 +141,10 @@ __device__ __forceinline__ void stress_products(const Sigma6& s, bool irr, const
     t[6] = jacl * (s.xx * J.gax + syx * J.gay + szx * J.gaz);
     t[7] = jacl * (s.xy * J.gax + s.yy * J.gay + szy * J.gaz);
     t[8] = jacl * (s.xz * J.gax + s.yz * J.gay + s.zz * J.gaz);
+  } else if (SF_FOLD) {
+    t[0] = s.xx; t[1] = s.xy; t[2] = s.xz;
+    t[3] = syx;  t[4] = s.yy; t[5] = s.yz;
+    t[6] = szx;  t[7] = szy;  t[8] = s.zz;
   } else {
     t[0] = jacl * s.xx * xr;
     t[1] = jacl * s.xy * xr;
@@ -181,11 +198,20 @@ __device__ __forceinline__ realw tile_divergence_c(const realw* sT, int plane, c
   return wxy[j * 5 + k] * n1 + wxy[i * 5 + k] * n2 + wxy[i * 5 + j] * n3;
 }
 
-// stage the 75 GLL coefficients of G in LDS (lanes 0..74 of the workgroup; needs a barrier before use)
+// stage the GLL coefficients of G in LDS (lanes 0..124 of the workgroup; needs a barrier before
+// use): hp, hpw, wxy, and the folded hpr, hpwr at sG + 75, sG + 100
+constexpr int SF_NGLL_TABLE = 125;
 __device__ __forceinline__ void stage_gll(realw* sG, const GLLk& G, int lane) {
   if (lane < 25) sG[lane] = G.hp[lane];
   else if (lane < 50) sG[lane] = G.hpw[lane - 25];
   else if (lane < 75) sG[lane] = G.wxy[lane - 50];
+  else if (lane < 100) sG[lane] = G.hpr[lane - 75];
+  else if (lane < 125) sG[lane] = G.hpwr[lane - 100];
+}
+// the hprime / hprimewgll table an element uses (folded ones on regular elements in FMA builds)
+__device__ __forceinline__ const realw* gll_hp(const realw* sG, bool irr) { return (SF_FOLD && !irr) ? sG + 75 : sG; }
+__device__ __forceinline__ const realw* gll_hpw(const realw* sG, bool irr) {
+  return (SF_FOLD && !irr) ? sG + 100 : sG + 25;
 }
 
 __device__ __forceinline__ JacPt load_jac(const realw* J, int64_t P, int64_t q) {

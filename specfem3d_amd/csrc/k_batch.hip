@@ -81,7 +81,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
   // displacement tile [c][s][p]; with OV it shares the planes of the products t3 (m = 6..8),
   // which are written last
   realw* const sU = OV ? sBuf + 6 * CPL : sBuf + 9 * CPL;
-  __shared__ realw sG[75];                        // hprime, hprimewgll, wgllwgll
+  __shared__ realw sG[SF_NGLL_TABLE];                        // hprime, hprimewgll, wgllwgll
   stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int b = xcd_remap_b(blockIdx.x, gridDim.x);
@@ -148,7 +148,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const realw* su = sU + (ts + r) * PST;
-    const Grad9 tg = tile_gradient<1>(su, su + SC * PST, su + 2 * SC * PST, sG, i, j, k);
+    const Grad9 tg = tile_gradient<1>(su, su + SC * PST, su + 2 * SC * PST, gll_hp(sG, irr != 0), i, j, k);
     const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
     const Sigma6 sg = iso_stress(du, km.x, km.y);
     realw t[9];
@@ -173,7 +173,7 @@ k_elastic_batch(BatchArgs B, GLLk G) {
 
   // one force component at a time for the lane's four shots (keeps few values live)
   realw* a = B.a + row;
-  const realw* hpw = sG + 25;
+  const realw* hpw = gll_hpw(sG, irr != 0);
   const realw* wxy = sG + 50;
   bool cont[4], ends = false;
 #pragma unroll
@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
   __shared__ realw sBuf[9 * CPL];
   realw* const sT = sBuf;                         // stress products [m][s][p]
   realw* const sU = sBuf + 3 * CPL;               // u (rows 0..2) and dt*v (rows 3..5), [c][s][p]
-  __shared__ realw sG[75];
+  __shared__ realw sG[SF_NGLL_TABLE];
   stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int b = xcd_remap_b(blockIdx.x, gridDim.x);
@@ -339,9 +339,9 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
     }
     __builtin_amdgcn_sched_barrier(0);            // the next shot's loads are issued first
     const realw* su = sU + (ts + r) * PST;
-    const Grad9 tg = tile_gradient<1>(su, su + CPL, su + 2 * CPL, sG, i, j, k);
+    const Grad9 tg = tile_gradient<1>(su, su + CPL, su + 2 * CPL, gll_hp(sG, irr != 0), i, j, k);
     const Du9 du = map_gradient(tg, irr != 0, J, B.xix_regular);
-    const Du9 st = map_gradient(tile_gradient_continue<1>(tg, su + 3 * CPL, su + 4 * CPL, su + 5 * CPL, sG, i, j, k),
+    const Du9 st = map_gradient(tile_gradient_continue<1>(tg, su + 3 * CPL, su + 4 * CPL, su + 5 * CPL, gll_hp(sG, irr != 0), i, j, k),
                                 irr != 0, J, B.xix_regular);
     const realw e_tr = (st.xx + st.yy + st.zz);
     const realw templ = ONE_THIRD * (st.xx + st.yy + st.zz);
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(128 * NQ) k_visco_batch(BatchArgs B, GLLk G) {
   if (!mine) return;
 
   realw* a = B.a + row;
-  const realw* hpw = sG + 25;
+  const realw* hpw = gll_hpw(sG, irr != 0);
   const realw* wxy = sG + 50;
   bool cont[4], ends = false;
 #pragma unroll
@@ -455,7 +455,7 @@ __global__ void __launch_bounds__(128 * NQ) k_acoustic_batch(BatchAcArgs B, GLLk
   constexpr int PST = 132;
   constexpr int CPL = SC * PST;
   __shared__ realw sBuf[4 * CPL];                 // [0] potential tile, [1..3] products t1, t2, t3
-  __shared__ realw sG[75];
+  __shared__ realw sG[SF_NGLL_TABLE];
   stage_gll(sG, G, threadIdx.x);
   const int it = __hip_atomic_load(B.it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int b = xcd_remap_b(blockIdx.x, gridDim.x);
@@ -555,7 +555,8 @@ void SFX(launch_elastic_batch)(hipStream_t st, const BatchArgs& a, int sc, int m
   if (a.nelem <= 0 || a.ngroups <= 0) return;
   const int64_t nb = (int64_t)a.nelem * a.ngroups;
   if (nb > 0x7fffffff) fatal("launch_elastic_batch", "grid too large");
-  const GLLk G = g_gll_host;
+  GLLk G = g_gll_host;
+  if (SF_FOLD) fold_gll(G, a.xix_regular, a.jacobian_regular);
   dim3 grid((unsigned)nb);
 #define SF_KB(NQ, M)                                                                           \
   if (overlay) hipLaunchKernelGGL((k_elastic_batch<NQ, M, 1>), grid, dim3(128 * NQ), 0, st, a, G); \
@@ -584,7 +585,8 @@ void SFX(launch_visco_batch)(hipStream_t st, const BatchArgs& a, int sc, int mod
   if (!a.R || !a.eps || !a.v) fatal("launch_visco_batch", "memory variables, strain and velocity are needed");
   if (mode == 3 && (!a.rmass || !a.flags)) fatal("launch_visco_batch", "fused update needs rmass and point flags");
   dim3 grid((unsigned)nb);
-  const GLLk G = g_gll_host;
+  GLLk G = g_gll_host;
+  if (SF_FOLD) fold_gll(G, a.xix_regular, a.jacobian_regular);
 #define SF_KV(NQ, M) hipLaunchKernelGGL((k_visco_batch<NQ, M>), grid, dim3(128 * NQ), 0, st, a, G)
   if (mode == 3) {
     if (sc == 4) SF_KV(1, 3);

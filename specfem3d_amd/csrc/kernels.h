@@ -18,7 +18,18 @@ struct GLLk {
   realw hp[25];
   realw hpw[25];
   realw wxy[25];
+  // FMA builds, regular elements: hp * xix_regular and hpw * (jacobian_regular * xix_regular), which
+  // fold the regular mapping and the jacobian weight into the contractions (fold_gll)
+  realw hpr[25];
+  realw hpwr[25];
 };
+inline void fold_gll(GLLk& G, realw xix_regular, realw jacobian_regular) {
+  const realw jx = jacobian_regular * xix_regular;
+  for (int q = 0; q < 25; q++) {
+    G.hpr[q] = G.hp[q] * xix_regular;
+    G.hpwr[q] = G.hpw[q] * jx;
+  }
+}
 extern GLLk g_gll_host;
 void set_gll_constants(const realw* hprime, const realw* hprimewgll, const realw* wgllwgll);
 
@@ -53,11 +64,6 @@ struct ElasticArgs {
   int64_t nloc;            // nspec*125
   realw alpha[3], beta[3], gamma[3];
   realw* local_out;        // ordered assembly: 3 planes of nloc
-  // atomic assembly: 1 when `accel` is zero at every element-interior point before this launch
-  // except in the elements flagged in interior_busy (sources, adjoint sources): the 27 interior
-  // points of the other elements are then stored instead of added (they belong to one element)
-  int interior_free;
-  const unsigned char* interior_busy;  // per element
 };
 // mode: 0 atomic scatter, 1 store local contributions (ordered gather), 2 plain read-modify-write
 // (the element list is one colour: its elements share no global point)

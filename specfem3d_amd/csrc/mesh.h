@@ -179,12 +179,6 @@ struct Mesh {
   int assembly = -1;               // 0 atomic scatter-add, 1 ordered gather (oracle order), 2 coloured,
                                    // -1 automatic (resolved at prepare_fields_elastic_device)
   int att_variant = 4;             // attenuation-state scheduling of the point kernel (0..4)
-  // atomic assembly: store (instead of add) the element-interior points when `accel` is known to
-  // be zero there (right after the predictor), except in elements that sources or adjoint sources
-  // add to before the stiffness (interior_busy, device, per element)
-  int interior_store = 0;
-  int accel_clean[2] = {0, 0};     // per field: accel zeroed by the predictor, nothing added since
-  unsigned char* interior_busy = nullptr;
   int ac_kernel_dd = 0;            // 1: fluid kernels read the adjoint potential_dot_dot (misfit gradient)
   // deferred corrector (kernel_3) per field, fused into the next predictor when possible
   int pending_k3_el[2] = {0, 0};

@@ -45,30 +45,11 @@ def test_elastic_exact_ordered_bitwise(oracle_mod, warp):
     assert np.array_equal(out, ref), rel_l2(out, ref)
 
 
-@pytest.mark.parametrize("interior_store", [0, 1])
 @pytest.mark.parametrize("exact", [True, False])
-def test_elastic_atomic_within_tolerance(oracle_mod, exact, interior_store):
-    """Float atomics (and, with interior_store, plain stores at element-interior points) within 1e-5
-    of the oracle; the source element keeps atomics at its interior points."""
-    from specfem3d_amd.solver import GPUSolver
+def test_elastic_atomic_within_tolerance(oracle_mod, exact):
     db, src, rec = small_elastic_box(n=4, warp=smooth_warp(2000.0))
     ref = run_oracle(oracle_mod, db, src, rec, 300)
-    g = GPUSolver(db, src, rec, nstep=300, exact=exact, assembly=0, tuning={"interior_store": interior_store})
-    out = g.run().copy()
-    g.close()
-    assert rel_l2(out, ref) < 1e-5
-
-
-@pytest.mark.parametrize("interior_store", [0, 1])
-def test_coupled_viscoelastic_atomic_interior_store(oracle_mod, interior_store):
-    from specfem3d_amd import workloads as W
-    from specfem3d_amd.solver import GPUSolver
-    w = W.sea_over_visco_box(scale=0.125, nstep=200, with_attenuation=True, nrec_side=3)
-    ref = run_oracle(oracle_mod, w.db, w.sources, w.receivers, 200, w.attenuation)
-    g = GPUSolver(w.db, w.sources, w.receivers, nstep=200, attenuation=w.attenuation, exact=False, assembly=0,
-                  tuning={"interior_store": interior_store})
-    out = g.run().copy()
-    g.close()
+    out = run_gpu(db, src, rec, 300, exact, 0)
     assert rel_l2(out, ref) < 1e-5
 
 
