@@ -300,6 +300,41 @@ def homogeneous_box_measurement(args) -> dict:
                                                 "2000-step run, measured on another host (8-core Xeon @ 2.70 GHz)"}}
 
 
+def cpml_example_measurement(args) -> dict:
+    """The reference example CPML_examples/homogeneous_halfspace_HEX8_elastic_absorbing_CPML_5sides
+    (36 x 36 x 16 irregular elements, 1,366,625 points, 9036 C-PML elements, 2400 steps) with the
+    library defaults, and its seismograms against the example's REF_SEIS."""
+    import numpy as np
+    from specfem3d_amd.examples import cpml5_example
+    from specfem3d_amd.solver import GPUSolver
+    db, src, rec, ref, dt, _ = cpml5_example()
+    n = ref.shape[0]
+    g = GPUSolver(db, src, rec, nstep=n)
+    warm = 20
+    for _ in range(warm):
+        g.step()
+    g.synchronize()
+    g.L.sf_timing_start(g.MP, C.c_int(n - warm))
+    t = time.perf_counter()
+    for _ in range(n - warm):
+        g.step()
+    g.synchronize()
+    wall = time.perf_counter() - t
+    tot = C.c_double(0)
+    cnt = C.c_int(0)
+    g.L.sf_timing_read(g.MP, C.byref(tot), C.byref(cnt))
+    out = g.seismograms("d").copy()
+    g.close()
+    e = float(np.sqrt(((out.astype(np.float64) - ref) ** 2).sum() / (ref.astype(np.float64) ** 2).sum()))
+    value = db.nglob * (n - warm) / wall
+    log(f"[bench] C-PML example: {value:.3e} GLL-pt updates/s ({wall / (n - warm) * 1e3:.3f} ms/step, stiffness "
+        f"incl. C-PML {tot.value / max(cnt.value, 1):.3f} ms), rel-L2 vs REF_SEIS {e:.2e}")
+    return {"workload": "CPML_examples/homogeneous_halfspace_HEX8_elastic_absorbing_CPML_5sides", "nglob": db.nglob,
+            "nspec": db.nspec, "nspec_cpml": db.pml.n, "value": value, "unit": "GLL-point updates/s",
+            "ms_per_step": wall / (n - warm) * 1e3, "steps": n - warm,
+            "stiffness_ms": tot.value / max(cnt.value, 1), "rel_l2_vs_ref_seis": e}
+
+
 def coupled_batch_measurement(args) -> dict:
     """The fluid-over-viscoelastic many-shot workload (the 160 x 160 x 80 builder: 10 % of the element
     layers fluid, 3-SLS solid with Q_mu = 100, seeded random CMT shots below the sea floor, one
@@ -507,6 +542,8 @@ def main():
     ap.add_argument("--adjoint-scale", type=float, default=1.0, help="shrink the adjoint mesh (tests only)")
     ap.add_argument("--homogeneous-box", type=int, default=1,
                     help="1: also run the 20^3 homogeneous box single-shot figures")
+    ap.add_argument("--cpml-example", type=int, default=1,
+                    help="1: also run the reference's C-PML example (2400 steps) and compare with its REF_SEIS")
     ap.add_argument("--coupled-shots", type=int, default=8,
                     help="shots of the batched fluid-over-viscoelastic workload (0: skip)")
     ap.add_argument("--coupled-steps", type=int, default=300, help="steps of each batched coupled shot")
@@ -657,6 +694,7 @@ def main():
     batched_visco = batched_visco_measurement(args) if args.visco_shots > 0 else None
     coupled = coupled_batch_measurement(args) if args.coupled_shots > 0 else None
     box = homogeneous_box_measurement(args) if args.homogeneous_box else None
+    cpml = cpml_example_measurement(args) if args.cpml_example else None
     large = large_measurement(args) if args.large_steps > 0 else None
 
     out = {
@@ -685,6 +723,7 @@ def main():
         "batched_viscoelastic": batched_visco,
         "coupled_batch": coupled,
         "homogeneous_box": box,
+        "cpml_example": cpml,
         "adjoint": adj,
     }
     if large is not None:

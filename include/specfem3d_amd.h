@@ -239,8 +239,23 @@ void sync_copy_from_device_(long* Mesh_pointer, int* iphase, realw* send_buffer)
 void register_host_array_(int* size, realw* h_array);
 void unregister_host_array_(realw* h_array);
 
+/* ---- C-PML absorbing layers (elastic elements, forward runs) ---- */
+/* prepare_mesh_constants_cuda.cu:1176 -- C-PML element lists, convolution coefficients
+   pml_convolution_coef_alpha/beta(9,NGLL3,NSPEC_CPML), _abar(5,..), _strain(18,..) as computed by
+   prepare_timerun_pml (prepare_timerun.F90:592-905), wgll_cube(NGLL3) and rhostore(NGLL3,NSPEC_AB);
+   call after prepare_fields_elastic_device_ (PML_CONDITIONS set in prepare_constants_device_) */
+void prepare_fields_elastic_pml_(long* Mesh_pointer, int* NSPEC_CPML, int* is_CPML, int* CPML_to_spec,
+                                 int* spec_to_CPML, realw* pml_convolution_coef_alpha,
+                                 realw* pml_convolution_coef_beta, realw* pml_convolution_coef_abar,
+                                 realw* pml_convolution_coef_strain, realw* h_wgll_cube, realw* rhostore);
+/* pml_compute_accel_cuda.cu:245 -- Dirichlet condition on the outer faces of the layers */
+void pml_impose_boundary_condition_elastic_cuda_(long* Mesh_pointer);
+/* transfer_fields_cuda.cu (stubs.c:735-739) -- PML_displ_old/new(NDIM,NGLL3,NSPEC_CPML) */
+void transfer_pml_displ_from_device_(int* size, realw* PML_displ_old, realw* PML_displ_new, long* Mesh_pointer);
+void transfer_pml_displ_to_device_(int* size, realw* PML_displ_old, realw* PML_displ_new, long* Mesh_pointer);
+
 /* ---- outside the single-partition hot path: these stop the run with a message (noise
-   tomography, faults, C-PML, gravity, ocean load, Hessian kernels, kernel smoothing, debug checks),
+   tomography, faults, gravity, ocean load, Hessian kernels, kernel smoothing, debug checks),
    as the reference's own stubs do when it is built without a GPU; pause_for_debug does nothing.
    Parameter names are placeholders. */
 /* src/gpu/compute_add_sources_viscoelastic_cuda.cu (stubs.c:186) */
@@ -287,12 +302,8 @@ void make_displ_rand_(long* p1, realw* p2);
 void noise_read_add_surface_movie_cu_(long* p1, realw* p2, int* p3);
 /* src/gpu/helper_functions.cu (stubs.c:438) */
 void pause_for_debug_(void);
-/* src/gpu/pml_compute_accel_cuda.cu (stubs.c:473) */
-void pml_impose_boundary_condition_elastic_cuda_(long* p1);
 /* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:637) */
 void prepare_fault_device_(long* p1, int* p2, realw* p3);
-/* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:598) */
-void prepare_fields_elastic_pml_(long* p1, int* p2, int* p3, int* p4, int* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, realw* p11);
 /* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:629) */
 void prepare_fields_gravity_device_(long* p1, int* p2, realw* p3, realw* p4, realw* p5, realw* p6);
 /* src/gpu/prepare_mesh_constants_cuda.cu (stubs.c:616) */
@@ -315,10 +326,6 @@ void transfer_kernels_hess_ac_tohost_(long* p1, realw* p2, realw* p3, realw* p4,
 void transfer_kernels_hess_el_tohost_(long* p1, realw* p2, realw* p3, realw* p4, realw* p5, int* p6);
 /* src/gpu/transfer_fields_cuda.cu (stubs.c:776) */
 void transfer_kernels_noise_to_host_(long* p1, realw* p2, int* p3);
-/* src/gpu/transfer_fields_cuda.cu (stubs.c:735) */
-void transfer_pml_displ_from_device_(int* p1, realw* p2, realw* p3, long* p4);
-/* src/gpu/transfer_fields_cuda.cu (stubs.c:738) */
-void transfer_pml_displ_to_device_(int* p1, realw* p2, realw* p3, long* p4);
 /* src/gpu/fault_solver_dynamics.cu (stubs.c:371) */
 void transfer_rsf_data_todevice_(long* p1, int* p2, int* p3, realw* p4, realw* p5, realw* p6, realw* p7, realw* p8, realw* p9, realw* p10, realw* p11, realw* p12, realw* p13, realw* p14, realw* p15, int* p16);
 /* src/gpu/fault_solver_dynamics.cu (stubs.c:400) */

@@ -843,3 +843,234 @@ R FN(orc_max_norm)(int64_t nglob, const R *f)
     }
     return m;
 }
+
+/* ------------------------------------------------------------------------- */
+/* C-PML (elastic). Arrays of the C-PML elements in the reference's Fortran layouts:
+ *   pml_displ[(ic*125 + p)*3 + c]             PML_displ_old/new(NDIM,NGLLX,NGLLY,NGLLZ,NSPEC_CPML)
+ *   rm3[a][(s*ncpml + ic)*125 + p]            rmemory_du?_d?l_?(NGLLX,NGLLY,NGLLZ,NSPEC_CPML,3), a = 0..8:
+ *       dux_dxl_x, duy_dxl_y, duz_dxl_z, dux_dyl_x, duy_dyl_y, duz_dyl_z, dux_dzl_x, duy_dzl_y, duz_dzl_z
+ *   rm1[b][ic*125 + p]                        rmemory_*(NGLLX,NGLLY,NGLLZ,NSPEC_CPML), b = 0..11:
+ *       duz_dzl_y, duz_dyl_y, duy_dzl_z, duy_dyl_z, duz_dzl_x, duz_dxl_x, dux_dzl_z, dux_dxl_z,
+ *       duy_dyl_x, duy_dxl_x, dux_dyl_y, dux_dxl_y
+ *   rdisp[((s*ncpml + ic)*125 + p)*3 + c]     rmemory_displ_elastic(NDIM,NGLLX,NGLLY,NGLLZ,NSPEC_CPML,3)
+ *   coef_alpha/beta[(ic*125 + p)*9 + t], coef_abar[(..)*5 + t], coef_strain[(..)*18 + t]
+ *       pml_convolution_coef_*(t,i,j,k,ispec_CPML)
+ * Each array of rm3 / rm1 is passed as its own pointer (a pointer table). */
+
+/* update_displ_elastic_PML (update_displacement_scheme.f90:417-455):
+ * PML_displ = displ + deltatover2 (1 - 2 theta) veloc + deltatsqover2 (1 - theta) accel, theta = 1/8 */
+void FN(orc_pml_store)(int ncpml, const int *cpml_to_spec, const int *ibool, const R *displ, const R *veloc,
+                       const R *accel, R deltatover2, R deltatsqover2, R *pml_displ)
+{
+    const R theta = (R)0.125;
+    const R c1 = deltatover2 * ((R)1 - (R)2 * theta);
+    const R c2 = deltatsqover2 * ((R)1 - theta);
+    for (int ic = 0; ic < ncpml; ic++) {
+        const int e = cpml_to_spec[ic];
+        for (int p = 0; p < 125; p++) {
+            const int64_t g = ibool[(int64_t)e * 125 + p];
+            for (int c = 0; c < 3; c++)
+                pml_displ[((int64_t)ic * 125 + p) * 3 + c] =
+                    displ[3 * g + c] + c1 * veloc[3 * g + c] + c2 * accel[3 * g + c];
+        }
+    }
+}
+
+/* pml_impose_boundary_condition_elastic (pml_compute_accel_contribution.f90:654-698): Dirichlet
+ * condition on the outer faces of the C-PML layers (absorbing faces of elastic C-PML elements). */
+void FN(orc_pml_dirichlet)(int nfaces, const int *face_ispec, const int *face_ijk, const int *ibool,
+                           const unsigned char *is_elastic, const int *spec_to_cpml, R *displ, R *veloc, R *accel,
+                           R *pml_old, R *pml_new)
+{
+    for (int f = 0; f < nfaces; f++) {
+        const int e = face_ispec[f];
+        if (!is_elastic[e] || !spec_to_cpml[e]) continue;
+        const int ic = spec_to_cpml[e] - 1;
+        for (int q = 0; q < 25; q++) {
+            const int *ijk = face_ijk + ((int64_t)f * 25 + q) * 3;
+            const int p = ijk[0] + 5 * ijk[1] + 25 * ijk[2];
+            const int64_t g = ibool[(int64_t)e * 125 + p];
+            for (int c = 0; c < 3; c++) {
+                accel[3 * g + c] = 0;
+                veloc[3 * g + c] = 0;
+                displ[3 * g + c] = 0;
+                pml_old[((int64_t)ic * 125 + p) * 3 + c] = 0;
+                pml_new[((int64_t)ic * 125 + p) * 3 + c] = 0;
+            }
+        }
+    }
+}
+
+/* r <- coef0 r + new coef1 + old coef2 (second-order recursive convolution, the expressions of
+ * pml_compute_memory_variables.f90:152-296 and pml_compute_accel_contribution.f90:88-121) */
+static inline R pml_conv(R r, R nw, R od, const R *c) { return c[0] * r + nw * c[1] + od * c[2]; }
+
+/* C-PML element forces: compute_forces_viscoelastic_PML (compute_forces_viscoelastic.F90:969-1748)
+ * with pml_compute_memory_variables_elastic (pml_compute_memory_variables.f90:29-415) and
+ * pml_compute_accel_contribution_elastic (pml_compute_accel_contribution.f90:29-146). elems lists the
+ * elastic elements in processing order; those with spec_to_cpml[e] == 0 are skipped. */
+void FN(orc_forces_elastic_pml)(int nelem, const int *elems, const int *spec_to_cpml, int ncpml, const int *ibool,
+                                const int *irregular, R xix_regular, R jacobian_regular, const R *jac9,
+                                const R *jacobian, const R *hprime, const R *hprimewgll, const R *wgllwgll,
+                                const R *wgll_cube, const R *kappastore, const R *mustore, const R *rhostore,
+                                const R *displ, const R *veloc, R *accel, const R *pml_old, const R *pml_new,
+                                R *const *rm3, R *const *rm1, R *rdisp, const R *coef_alpha, const R *coef_beta,
+                                const R *coef_abar, const R *coef_strain)
+{
+    const R FOUR_THIRDS = (R)4 / (R)3;
+    for (int ie = 0; ie < nelem; ie++) {
+        const int e = elems[ie];
+        if (!spec_to_cpml[e]) continue;
+        const int ic = spec_to_cpml[e] - 1;
+        const int *ib = ibool + (int64_t)e * 125;
+        R f[9][125], t[27][125];  /* fields u, old, new (x,y,z each); their reference gradients */
+        for (int p = 0; p < 125; p++) {
+            const int64_t g = ib[p];
+            for (int c = 0; c < 3; c++) {
+                f[c][p] = displ[3 * g + c];
+                f[3 + c][p] = pml_old[((int64_t)ic * 125 + p) * 3 + c];
+                f[6 + c][p] = pml_new[((int64_t)ic * 125 + p) * 3 + c];
+            }
+        }
+        for (int q = 0; q < 9; q++) grad_ref(hprime, f[q], t[3 * q], t[3 * q + 1], t[3 * q + 2]);
+        const int irr = irregular[e];
+        const R *J = irr ? jac9 + (int64_t)(irr - 1) * 9 * 125 : 0;
+        R s1[9][125];  /* stress products tempx1, tempy1, tempz1, tempx2, ..., tempz3 */
+        R cpml[3][125];
+        for (int p = 0; p < 125; p++) {
+            /* derivatives D[field][comp][dir]: d u_comp / d x_dir of u (0), old (1), new (2) */
+            R D[3][3][3];
+            R xixl = 0, xiyl = 0, xizl = 0, etaxl = 0, etayl = 0, etazl = 0, gaxl = 0, gayl = 0, gazl = 0;
+            if (irr) {
+                xixl = J[0 * 125 + p]; xiyl = J[1 * 125 + p]; xizl = J[2 * 125 + p];
+                etaxl = J[3 * 125 + p]; etayl = J[4 * 125 + p]; etazl = J[5 * 125 + p];
+                gaxl = J[6 * 125 + p]; gayl = J[7 * 125 + p]; gazl = J[8 * 125 + p];
+            }
+            for (int fl = 0; fl < 3; fl++)
+                for (int c = 0; c < 3; c++) {
+                    const R a1 = t[3 * (3 * fl + c)][p], a2 = t[3 * (3 * fl + c) + 1][p], a3 = t[3 * (3 * fl + c) + 2][p];
+                    if (irr) {
+                        D[fl][c][0] = xixl * a1 + etaxl * a2 + gaxl * a3;
+                        D[fl][c][1] = xiyl * a1 + etayl * a2 + gayl * a3;
+                        D[fl][c][2] = xizl * a1 + etazl * a2 + gazl * a3;
+                    } else {
+                        D[fl][c][0] = xix_regular * a1;
+                        D[fl][c][1] = xix_regular * a2;
+                        D[fl][c][2] = xix_regular * a3;
+                    }
+                }
+            const int64_t lp = (int64_t)ic * 125 + p;
+            const R *ca = coef_alpha + lp * 9, *cb = coef_beta + lp * 9, *A = coef_strain + lp * 18;
+            const R *cax = ca, *cay = ca + 3, *caz = ca + 6;
+            const R *cbx = cb, *cby = cb + 3, *cbz = cb + 6;
+            /* the three-coefficient families: derivative direction d, coefficient sets per sls index */
+            const R *fam[3][3] = {{caz, cay, cbx}, {cax, caz, cby}, {cax, cay, cbz}};
+            R r3[9][3];
+            for (int d = 0; d < 3; d++)
+                for (int c = 0; c < 3; c++) {
+                    R *arr = rm3[3 * d + c];
+                    for (int s = 0; s < 3; s++) {
+                        R *r = arr + ((int64_t)s * ncpml + ic) * 125 + p;
+                        *r = pml_conv(*r, D[2][c][d], D[1][c][d], fam[d][s]);
+                        r3[3 * d + c][s] = *r;
+                    }
+                }
+            /* single-coefficient memory variables: (array, comp, dir, coefficient set) */
+            static const int one[12][2] = {{2, 2}, {2, 1}, {1, 2}, {1, 1}, {2, 2}, {2, 0},
+                                           {0, 2}, {0, 0}, {1, 1}, {1, 0}, {0, 1}, {0, 0}};
+            R r1[12];
+            for (int b = 0; b < 12; b++) {
+                const R *cs = b < 4 ? cax : (b < 8 ? cay : caz);
+                R *r = rm1[b] + lp;
+                *r = pml_conv(*r, D[2][one[b][0]][one[b][1]], D[1][one[b][0]][one[b][1]], cs);
+                r1[b] = *r;
+            }
+            /* A6..A23 = A[0..17] */
+            const R (*C)[3] = D[0];
+            const R duxdxl_x = A[0] * C[0][0] + A[1] * r3[0][0] + A[2] * r3[0][1] + A[3] * r3[0][2];
+            const R duxdyl_x = A[4] * C[0][1] + A[5] * r3[3][0] + A[6] * r3[3][1] + A[7] * r3[3][2];
+            const R duxdzl_x = A[8] * C[0][2] + A[9] * r3[6][0] + A[10] * r3[6][1] + A[11] * r3[6][2];
+            const R duzdzl_x = A[14] * C[2][2] + A[15] * r1[4];
+            const R duzdxl_x = A[14] * C[2][0] + A[15] * r1[5];
+            const R duydyl_x = A[16] * C[1][1] + A[17] * r1[8];
+            const R duydxl_x = A[16] * C[1][0] + A[17] * r1[9];
+            const R duydxl_y = A[0] * C[1][0] + A[1] * r3[1][0] + A[2] * r3[1][1] + A[3] * r3[1][2];
+            const R duydyl_y = A[4] * C[1][1] + A[5] * r3[4][0] + A[6] * r3[4][1] + A[7] * r3[4][2];
+            const R duydzl_y = A[8] * C[1][2] + A[9] * r3[7][0] + A[10] * r3[7][1] + A[11] * r3[7][2];
+            const R duzdzl_y = A[12] * C[2][2] + A[13] * r1[0];
+            const R duzdyl_y = A[12] * C[2][1] + A[13] * r1[1];
+            const R duxdyl_y = A[16] * C[0][1] + A[17] * r1[10];
+            const R duxdxl_y = A[16] * C[0][0] + A[17] * r1[11];
+            const R duzdxl_z = A[0] * C[2][0] + A[1] * r3[2][0] + A[2] * r3[2][1] + A[3] * r3[2][2];
+            const R duzdyl_z = A[4] * C[2][1] + A[5] * r3[5][0] + A[6] * r3[5][1] + A[7] * r3[5][2];
+            const R duzdzl_z = A[8] * C[2][2] + A[9] * r3[8][0] + A[10] * r3[8][1] + A[11] * r3[8][2];
+            const R duydzl_z = A[12] * C[1][2] + A[13] * r1[2];
+            const R duydyl_z = A[12] * C[1][1] + A[13] * r1[3];
+            const R duxdzl_z = A[14] * C[0][2] + A[15] * r1[6];
+            const R duxdxl_z = A[14] * C[0][0] + A[15] * r1[7];
+            const R kappal = kappastore[(int64_t)e * 125 + p], mul = mustore[(int64_t)e * 125 + p];
+            const R l2m = kappal + FOUR_THIRDS * mul;
+            const R lam = l2m - (R)2 * mul;
+            const R sxx = l2m * duxdxl_x + lam * duydyl_x + lam * duzdzl_x;
+            const R syx = mul * duxdyl_x + mul * duydxl_x;
+            const R szx = mul * duzdxl_x + mul * duxdzl_x;
+            const R sxy = mul * duxdyl_y + mul * duydxl_y;
+            const R syy = lam * duxdxl_y + l2m * duydyl_y + lam * duzdzl_y;
+            const R szy = mul * duzdyl_y + mul * duydzl_y;
+            const R sxz = mul * duzdxl_z + mul * duxdzl_z;
+            const R syz = mul * duzdyl_z + mul * duydzl_z;
+            const R szz = lam * duxdxl_z + lam * duydyl_z + l2m * duzdzl_z;
+            R jl;
+            if (irr) {
+                jl = jacobian[(int64_t)(irr - 1) * 125 + p];
+                s1[0][p] = jl * (sxx * xixl + syx * xiyl + szx * xizl);
+                s1[1][p] = jl * (sxy * xixl + syy * xiyl + szy * xizl);
+                s1[2][p] = jl * (sxz * xixl + syz * xiyl + szz * xizl);
+                s1[3][p] = jl * (sxx * etaxl + syx * etayl + szx * etazl);
+                s1[4][p] = jl * (sxy * etaxl + syy * etayl + szy * etazl);
+                s1[5][p] = jl * (sxz * etaxl + syz * etayl + szz * etazl);
+                s1[6][p] = jl * (sxx * gaxl + syx * gayl + szx * gazl);
+                s1[7][p] = jl * (sxy * gaxl + syy * gayl + szy * gazl);
+                s1[8][p] = jl * (sxz * gaxl + syz * gayl + szz * gazl);
+            } else {
+                jl = jacobian_regular;
+                s1[0][p] = jl * sxx * xix_regular;
+                s1[1][p] = jl * sxy * xix_regular;
+                s1[2][p] = jl * sxz * xix_regular;
+                s1[3][p] = jl * syx * xix_regular;
+                s1[4][p] = jl * syy * xix_regular;
+                s1[5][p] = jl * syz * xix_regular;
+                s1[6][p] = jl * szx * xix_regular;
+                s1[7][p] = jl * szy * xix_regular;
+                s1[8][p] = jl * szz * xix_regular;
+            }
+            /* pml_compute_accel_contribution_elastic */
+            const R *ab = coef_abar + lp * 5;
+            const R *cs3[3] = {cax, cay, caz};
+            const int64_t g = ib[p];
+            const R w = wgll_cube[p] * rhostore[(int64_t)e * 125 + p] * jl;
+            for (int c = 0; c < 3; c++) {
+                R rr[3];
+                for (int s = 0; s < 3; s++) {
+                    R *r = rdisp + (((int64_t)s * ncpml + ic) * 125 + p) * 3 + c;
+                    *r = pml_conv(*r, f[6 + c][p], f[3 + c][p], cs3[s]);
+                    rr[s] = *r;
+                }
+                cpml[c][p] = w * (ab[0] * veloc[3 * g + c] + ab[1] * displ[3 * g + c] + ab[2] * rr[0] + ab[3] * rr[1] +
+                                  ab[4] * rr[2]);
+            }
+        }
+        R n1[125], n2[125], n3[125];
+        for (int c = 0; c < 3; c++) {
+            div_ref(hprimewgll, s1[c], s1[3 + c], s1[6 + c], n1, n2, n3);
+            for (int k = 0; k < 5; k++)
+                for (int j = 0; j < 5; j++)
+                    for (int i = 0; i < 5; i++) {
+                        const int p = i + 5 * j + 25 * k;
+                        const R fac1 = wgllwgll[j * 5 + k], fac2 = wgllwgll[i * 5 + k], fac3 = wgllwgll[i * 5 + j];
+                        const int64_t g = ib[p];
+                        accel[3 * g + c] = accel[3 * g + c] - (fac1 * n1[p] + fac2 * n2[p] + fac3 * n3[p] + cpml[c][p]);
+                    }
+        }
+    }
+}

@@ -46,6 +46,8 @@ EXPORTED = [
     "transfer_boundary_from_device_a_", "transfer_boundary_to_device_a_", "transfer_asmbl_accel_to_device_",
     "transfer_sync_accel_to_device_", "sync_copy_from_device_", "get_free_device_memory_",
     "output_free_device_memory_", "register_host_array_", "unregister_host_array_",
+    "prepare_fields_elastic_pml_", "pml_impose_boundary_condition_elastic_cuda_", "transfer_pml_displ_from_device_",
+    "transfer_pml_displ_to_device_",
     "sf_set_options", "sf_set_tuning", "sf_set_element_order", "sf_color_elements", "sf_batch_point_flags", "sf_snapshot_save", "sf_snapshot_load",
     "sf_history_reserve", "sf_history_store", "sf_history_load", "sf_checkpoints_free", "sf_zero_kernels",
     "sf_batch_create", "sf_batch_add_source", "sf_batch_add_receiver", "sf_batch_finalize", "sf_batch_set_options",

@@ -70,6 +70,7 @@ struct HostSide {
   RawFaces abs;
   std::vector<int> ibool;      // 0-based
   std::vector<unsigned char> is_el, is_ac;
+  std::vector<unsigned char> is_cpml;  // C-PML flag per element (prepare_fields_elastic_pml)
   int nspec = 0;
   // pinned staging ring for per-step host values (source time functions, adjoint sources)
   static const int RING = 64;
@@ -281,7 +282,6 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
   if (h_NB_RUNS_ACOUSTIC_GPU && *h_NB_RUNS_ACOUSTIC_GPU != 1)
     fatal("prepare_constants_device", "NB_RUNS_ACOUSTIC_GPU > 1 is not supported (use the batched engine)");
   if (FAULT_SIMULATION && *FAULT_SIMULATION) fatal("prepare_constants_device", "fault simulations are not supported");
-  if (PML_CONDITIONS && *PML_CONDITIONS) fatal("prepare_constants_device", "C-PML is not supported");
   if (num_interfaces_ext_mesh && *num_interfaces_ext_mesh != 0)
     fatal("prepare_constants_device", "MPI interfaces present: this library runs one partition (NPROC=1)");
   (void)max_nibool_interfaces_ext_mesh; (void)h_nibool_interfaces_ext_mesh; (void)h_ibool_interfaces_ext_mesh;
@@ -348,10 +348,13 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
   m->nspec_elastic = *nspec_elastic;
   m->nspec_acoustic = *nspec_acoustic;
 
-  // absorbing faces (kept on the host until the material arrays arrive)
+  // absorbing faces (kept on the host until the material arrays arrive); with C-PML they are the
+  // outer faces of the layers, which carry its Dirichlet condition
   m->stacey = *STACEY_ABSORBING_CONDITIONS;
+  m->pml_conditions = PML_CONDITIONS ? *PML_CONDITIONS : 0;
+  if (m->pml_conditions && m->stacey) fatal("prepare_constants_device", "C-PML and Stacey conditions exclude each other");
   m->num_abs_boundary_faces = *h_num_abs_boundary_faces;
-  if (m->stacey) {
+  if (m->stacey || m->pml_conditions) {
     const int nf = m->num_abs_boundary_faces;
     H->abs.ispec.resize(nf);
     H->abs.ijk.resize((size_t)nf * 75);
@@ -464,6 +467,7 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
 
 namespace sf {
 static void build_elastic_assembly(Mesh* m);
+static int color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* col);
 
 // Distinct (factor_common, factor_common_kappa) rows, compared bit for bit, and a 16-bit row index
 // per local point; gives up (planes only) beyond 65535 distinct rows.
@@ -767,11 +771,43 @@ static void build_csr(Mesh* m, const std::vector<int>& elems, int*& ptr, int*& i
 }
 
 static void build_elastic_assembly(Mesh* m) {
-  std::vector<int> lst(m->nspec_elastic);
-  SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+  // the stiffness elements, then the C-PML elements (compute_forces_viscoelastic, then
+  // compute_forces_viscoelastic_PML: that is the order their contributions are summed in)
+  std::vector<int> lst(m->nspec_elastic + m->pml_n);
+  SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * m->nspec_elastic, hipMemcpyDeviceToHost));
+  if (m->pml_n)
+    SF_CHECK(hipMemcpy(lst.data() + m->nspec_elastic, m->pml_elems, sizeof(int) * m->pml_n, hipMemcpyDeviceToHost));
+  dfree(m->asm_ptr);
+  dfree(m->asm_idx);
   build_csr(m, lst, m->asm_ptr, m->asm_idx);
-  m->asm_buf = dmalloc<realw>(3 * (int64_t)m->nspec * 125);
+  if (!m->asm_buf) m->asm_buf = dmalloc<realw>(3 * (int64_t)m->nspec * 125);
   m->asm_ready = 1;
+}
+
+// C-PML list regrouped by colour (coloured assembly): same greedy colouring as the stiffness list
+void build_pml_colors(Mesh* m) {
+  if (m->pml_n <= 0) return;
+  HostSide* H = host_of(m);
+  std::vector<int> el(m->pml_n), cp(m->pml_n), col(m->pml_n);
+  SF_CHECK(hipMemcpy(el.data(), m->pml_elems, sizeof(int) * el.size(), hipMemcpyDeviceToHost));
+  SF_CHECK(hipMemcpy(cp.data(), m->pml_cpml, sizeof(int) * cp.size(), hipMemcpyDeviceToHost));
+  const int nc = color_elements(m->nglob, m->pml_n, H->ibool.data(), el.data(), col.data());
+  if (nc < 0) fatal("build_pml_colors", "element colouring needs more than 32 colours");
+  int* off = m->pml_color_off;
+  for (int c = 0; c <= 32; c++) off[c] = 0;
+  for (int q = 0; q < m->pml_n; q++) off[col[q] + 1]++;
+  for (int c = 0; c < 32; c++) off[c + 1] += off[c];
+  std::vector<int> pos(off, off + 32), le(m->pml_n), lc(m->pml_n);
+  for (int q = 0; q < m->pml_n; q++) {
+    const int d = pos[col[q]]++;
+    le[d] = el[q];
+    lc[d] = cp[q];
+  }
+  dfree(m->pml_color_elems);
+  dfree(m->pml_color_cpml);
+  m->pml_color_elems = dupload(le.data(), (int64_t)le.size());
+  m->pml_color_cpml = dupload(lc.data(), (int64_t)lc.size());
+  m->pml_ncolor = nc;
 }
 
 }  // namespace sf
@@ -871,6 +907,132 @@ extern "C" void FC_FUNC_(prepare_fields_acoustic_device, PREPARE_FIELDS_ACOUSTIC
     if (m->save_forward || m->simulation_type == 3)
       m->b_absorb_potential = dmalloc<realw>(25 * (int64_t)std::max(1, m->num_abs_boundary_faces));
   }
+}
+
+// prepare_mesh_constants_cuda.cu:1176 -- C-PML elements of the elastic domain. The convolution
+// coefficients arrive in the Fortran layouts pml_convolution_coef_alpha/beta(9,NGLL3,NSPEC_CPML),
+// _abar(5,..), _strain(18,..); they are regrouped into per-element blocks of 125-point planes with
+// wgll_cube * rhostore appended. The stiffness launches drop the C-PML elements, which a second
+// kernel computes (as compute_forces_viscoelastic_cuda does through pml_kernel_2_impl).
+extern "C" void FC_FUNC_(prepare_fields_elastic_pml, PREPARE_FIELDS_ELASTIC_PML)(
+    long* Mesh_pointer, int* NSPEC_CPML, int* is_CPML, int* CPML_to_spec, int* spec_to_CPML,
+    realw* pml_convolution_coef_alpha, realw* pml_convolution_coef_beta, realw* pml_convolution_coef_abar,
+    realw* pml_convolution_coef_strain, realw* h_wgll_cube, realw* rhostore) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (!m->pml_conditions) return;
+  HostSide* H = host_of(m);
+  const char* where = "prepare_fields_elastic_pml";
+  if (!m->kappamu || !m->elastic_elems) fatal(where, "prepare_fields_elastic_device must come first");
+  if (m->pml_state || m->pml_n) fatal(where, "C-PML arrays are already prepared");
+  if (m->simulation_type != 1)
+    fatal(where, "C-PML runs are forward only (SIMULATION_TYPE = 1), as the reference's GPU C-PML kernel");
+  const int ncp = *NSPEC_CPML;
+  m->nspec_cpml = ncp;
+  H->is_cpml.assign(m->nspec, 0);
+  for (int e = 0; e < m->nspec; e++) {
+    const bool a = is_CPML[e] != 0, b = spec_to_CPML[e] > 0;
+    if (a != b || spec_to_CPML[e] > ncp || (b && CPML_to_spec[spec_to_CPML[e] - 1] != e + 1))
+      fatal(where, "is_CPML, spec_to_CPML and CPML_to_spec disagree");
+    H->is_cpml[e] = a;
+  }
+  for (int ic = 0; ic < ncp; ic++) {
+    const int e = CPML_to_spec[ic] - 1;
+    if (e < 0 || e >= m->nspec) fatal(where, "CPML_to_spec out of range");
+    if (!H->is_el[e])
+      fatal(where, "C-PML elements in acoustic domains are not supported (elastic C-PML only, as the reference's GPU path)");
+  }
+  if (ncp == 0) return;
+  SF_CHECK(hipStreamSynchronize(m->stream));
+
+  // stiffness list without the C-PML elements; C-PML list in the same processing order
+  std::vector<int> lst(m->nspec_elastic), keep, pe, pc;
+  SF_CHECK(hipMemcpy(lst.data(), m->elastic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+  for (int e : lst) {
+    if (H->is_cpml[e]) {
+      pe.push_back(e);
+      pc.push_back(spec_to_CPML[e] - 1);
+    } else {
+      keep.push_back(e);
+    }
+  }
+  dfree(m->elastic_elems);
+  m->nspec_elastic = (int)keep.size();
+  m->elastic_elems = keep.empty() ? dmalloc<int>(1) : dupload(keep.data(), (int64_t)keep.size());
+  if (m->elastic_launch) {
+    std::vector<int> lo(lst.size()), lk;
+    SF_CHECK(hipMemcpy(lo.data(), m->elastic_launch, sizeof(int) * lo.size(), hipMemcpyDeviceToHost));
+    for (int e : lo)
+      if (!H->is_cpml[e]) lk.push_back(e);
+    dfree(m->elastic_launch);
+    m->elastic_launch = lk.empty() ? nullptr : dupload(lk.data(), (int64_t)lk.size());
+  }
+  m->pml_n = (int)pe.size();
+  m->pml_elems = dupload(pe.data(), (int64_t)pe.size());
+  m->pml_cpml = dupload(pc.data(), (int64_t)pc.size());
+
+  // coefficient planes
+  {
+    const size_t blk = (size_t)SF_PML_NCOEF * 125;
+    std::vector<realw> cf((size_t)ncp * blk);
+    for (int ic = 0; ic < ncp; ic++) {
+      const int64_t e = CPML_to_spec[ic] - 1;
+      for (int p = 0; p < 125; p++) {
+        realw* b = &cf[(size_t)ic * blk + p];
+        const int64_t q = p + 125 * (int64_t)ic;
+        for (int t = 0; t < 9; t++) {
+          b[t * 125] = pml_convolution_coef_alpha[t + 9 * q];
+          b[(9 + t) * 125] = pml_convolution_coef_beta[t + 9 * q];
+        }
+        for (int t = 0; t < 5; t++) b[(18 + t) * 125] = pml_convolution_coef_abar[t + 5 * q];
+        for (int t = 0; t < 18; t++) b[(23 + t) * 125] = pml_convolution_coef_strain[t + 18 * q];
+        b[41 * 125] = h_wgll_cube[p] * rhostore[e * 125 + p];
+      }
+    }
+    m->pml_coef = dupload(cf.data(), (int64_t)cf.size());
+  }
+  const int64_t nst = (int64_t)ncp * SF_PML_NSTATE * 125;
+  m->pml_state = dmalloc<realw>(nst);
+  SF_CHECK(hipMemset(m->pml_state, 0, sizeof(realw) * nst));
+  m->pml_old = dmalloc<realw>(3 * (int64_t)m->nglob);
+  SF_CHECK(hipMemset(m->pml_old, 0, sizeof(realw) * 3 * (size_t)m->nglob));
+  // global points of the C-PML elements, and those on the outer faces of the layers
+  {
+    std::vector<unsigned char> mark((size_t)m->nglob, 0);
+    for (int e : pe)
+      for (int p = 0; p < 125; p++) mark[H->ibool[(int64_t)e * 125 + p]] = 1;
+    std::vector<int> pts;
+    for (int g = 0; g < m->nglob; g++)
+      if (mark[g]) pts.push_back(g);
+    m->pml_npts = (int)pts.size();
+    m->pml_pts = dupload(pts.data(), (int64_t)pts.size());
+    std::fill(mark.begin(), mark.end(), 0);
+    const RawFaces& rf = H->abs;
+    for (size_t f = 0; f < rf.ispec.size(); f++) {
+      const int e = rf.ispec[f];
+      if (!H->is_el[e] || !H->is_cpml[e]) continue;
+      for (int q = 0; q < 25; q++) {
+        const int* ijk = &rf.ijk[(f * 25 + q) * 3];
+        mark[H->ibool[(int64_t)e * 125 + ijk[0] + 5 * ijk[1] + 25 * ijk[2]]] = 1;
+      }
+    }
+    std::vector<int> bc;
+    for (int g = 0; g < m->nglob; g++)
+      if (mark[g]) bc.push_back(g);
+    m->pml_nbc = (int)bc.size();
+    m->pml_bc_pts = bc.empty() ? nullptr : dupload(bc.data(), (int64_t)bc.size());
+  }
+  // the assembly structures follow the new lists
+  if (m->assembly == 1) build_elastic_assembly(m);
+  if (m->assembly == 2) {
+    std::vector<int> order = keep;
+    if (m->elastic_launch) {
+      order.resize(m->nspec_elastic);
+      SF_CHECK(hipMemcpy(order.data(), m->elastic_launch, sizeof(int) * order.size(), hipMemcpyDeviceToHost));
+    }
+    build_colors(m, order, m->el_color_elems, m->el_color_off, m->el_ncolor);
+    build_pml_colors(m);
+  }
+  prep_log("C-PML arrays prepared");
 }
 
 // prepare_mesh_constants_cuda.cu:999
@@ -1019,6 +1181,8 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   dfree(m->seismo_d); dfree(m->seismo_v); dfree(m->seismo_a); dfree(m->seismo_p);
   dfree(m->rho_kl); dfree(m->mu_kl); dfree(m->kappa_kl); dfree(m->rho_ac_kl); dfree(m->kappa_ac_kl);
   dfree(m->asm_ptr); dfree(m->asm_idx); dfree(m->asm_buf);
+  dfree(m->pml_elems); dfree(m->pml_cpml); dfree(m->pml_color_elems); dfree(m->pml_color_cpml);
+  dfree(m->pml_state); dfree(m->pml_coef); dfree(m->pml_old); dfree(m->pml_pts); dfree(m->pml_bc_pts);
   dfree(m->asm_ac_ptr); dfree(m->asm_ac_idx); dfree(m->asm_ac_buf);
   dfree(m->d_norm);
   if (m->h_norm) (void)hipHostFree(m->h_norm);
@@ -1056,6 +1220,7 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
   }
   if (assembly == 2) {
     if (!m->el_color_elems) ensure_element_colors(m);
+    if (!m->pml_color_elems && m->pml_n > 0) build_pml_colors(m);
     if (!m->ac_color_elems && m->nspec_acoustic > 0 && m->acoustic_elems) {
       std::vector<int> lst(m->nspec_acoustic);
       SF_CHECK(hipMemcpy(lst.data(), m->acoustic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
@@ -1090,20 +1255,23 @@ extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
 extern "C" void sf_set_element_order(long* Mesh_pointer, int n, const int* order) {
   Mesh* m = from_handle(Mesh_pointer);
   HostSide* H = host_of(m);
-  if (n != m->nspec_elastic) fatal("sf_set_element_order", "the order must list every elastic element once");
+  // with C-PML the order may list the C-PML elements too (they are launched separately)
+  if (n != m->nspec_elastic && n != m->nspec_elastic + m->pml_n)
+    fatal("sf_set_element_order", "the order must list every elastic element once");
   std::vector<unsigned char> seen((size_t)m->nspec, 0);
+  std::vector<int> lst;
+  lst.reserve(n);
   for (int q = 0; q < n; q++) {
     const int e = order[q];
     if (e < 0 || e >= m->nspec || !H->is_el[e] || seen[e]) fatal("sf_set_element_order", "not a permutation of the elastic elements");
     seen[e] = 1;
+    if (H->is_cpml.empty() || !H->is_cpml[e]) lst.push_back(e);
   }
+  if ((int)lst.size() != m->nspec_elastic) fatal("sf_set_element_order", "not a permutation of the elastic elements");
   SF_CHECK(hipStreamSynchronize(m->stream));
   dfree(m->elastic_launch);
-  m->elastic_launch = dupload(order, (int64_t)n);
-  if (m->el_color_elems) {
-    std::vector<int> lst(order, order + n);
-    build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
-  }
+  m->elastic_launch = dupload(lst.data(), (int64_t)lst.size());
+  if (m->el_color_elems) build_colors(m, lst, m->el_color_elems, m->el_color_off, m->el_ncolor);
 }
 
 // Greedy element colouring used by assembly = 2 (host only, no device needed): colour[q] of

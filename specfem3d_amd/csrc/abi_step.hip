@@ -18,8 +18,13 @@
 namespace sf {
 realw* stage_to_device(Mesh* m, const realw* vals, int n, realw* dst);
 realw* stf_device_buffer(Mesh* m, int n);
+const std::vector<int>& host_ibool(Mesh* m);
 
 static inline int field_of(int foa) { return foa == 3 ? 1 : 0; }
+
+// face-point terms in one atomic launch when the element stiffness scatters with atomics (its
+// sums have no fixed order then), else one launch per round in the reference's face order
+static inline bool face_atomics(const Mesh* m) { return m->assembly == 0; }
 
 static void flush_el(Mesh* m, int f) {
   if (m->pending_k3_el[f]) {
@@ -95,6 +100,42 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
   } else {
     launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->att_variant);
   }
+  if (f == 0 && m->pml_n > 0) {
+    // C-PML elements (pml_kernel_2_impl of compute_forces_viscoelastic_cuda), after the others: their
+    // contributions are summed after those of the stiffness elements
+    PmlArgs P;
+    std::memset(&P, 0, sizeof(P));
+    P.nelem = m->pml_n;
+    P.elems = m->pml_elems;
+    P.cpml = m->pml_cpml;
+    P.ibool = m->ibool;
+    P.irregular = m->irregular;
+    P.xix_regular = m->xix_regular;
+    P.jacobian_regular = m->jacobian_regular;
+    P.jac = m->jac;
+    P.jac_plane = (int64_t)m->nspec_irregular * 125;
+    P.kappamu = m->kappamu;
+    P.displ = m->el[0].displ;
+    P.veloc = m->el[0].veloc;
+    P.accel = m->el[0].accel;
+    P.pml_old = m->pml_old;
+    P.state = m->pml_state;
+    P.coef = m->pml_coef;
+    P.t1 = m->pml_t1;
+    P.t2 = m->pml_t2;
+    P.local_out = m->asm_buf;
+    P.nloc = a.nloc;
+    if (m->assembly == 2 && m->pml_color_elems) {
+      for (int c = 0; c < m->pml_ncolor; c++) {
+        P.elems = m->pml_color_elems + m->pml_color_off[c];
+        P.cpml = m->pml_color_cpml + m->pml_color_off[c];
+        P.nelem = m->pml_color_off[c + 1] - m->pml_color_off[c];
+        launch_elastic_pml(m->stream, P, m->exact != 0, 2);
+      }
+    } else {
+      launch_elastic_pml(m->stream, P, m->exact != 0, ordered ? 1 : 0);
+    }
+  }
   if (timed) {
     SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count + 1], m->stream));
     m->timing_count++;
@@ -121,6 +162,14 @@ extern "C" void FC_FUNC_(update_displacement_cuda, UPDATE_DISPLACMENT_CUDA)(long
   check_field(m, f, "update_displacement_cuda");
   const int64_t n = 3 * m->el_n, o = 3 * m->el_g0;  // only the elastic range changes
   Wavefield& w = m->el[f];
+  if (f == 0 && m->pml_n > 0) {
+    // update_displacement_cuda.cu:92-124: PML_displ_old from the field before the predictor (the
+    // pending corrector applied on the fly); PML_displ_new is formed by the C-PML kernel
+    m->pml_t1 = *deltatover2_F * (1.0f - 2.0f * 0.125f);
+    m->pml_t2 = *deltatsqover2_F * (1.0f - 0.125f);
+    launch_pml_store_old(m->stream, m->pml_npts, m->pml_pts, w.displ, w.veloc, w.accel, m->rmass,
+                         m->pending_k3_el[f], m->pending_dt2_el[f], m->pml_t1, m->pml_t2, m->pml_old);
+  }
   if (m->pending_k3_el[f]) {
     launch_corrector_predictor(m->stream, n, w.displ + o, w.veloc + o, w.accel + o, m->rmass + o,
                                m->pending_dt2_el[f], *deltat_F, *deltatover2_F, *deltatsqover2_F);
@@ -271,6 +320,81 @@ extern "C" void FC_FUNC_(acoustic_enforce_free_surf_cuda, ACOUSTIC_ENFORCE_FREE_
   }
 }
 
+// ---------------------------------------------------------------- C-PML
+// pml_compute_accel_cuda.cu:245 -- Dirichlet condition on the outer faces of the C-PML layers: u, v, a
+// set to zero at their points (the reference also zeroes PML_displ_old/new there, which the next
+// predictor overwrites before any use)
+extern "C" void FC_FUNC_(pml_impose_boundary_condition_elastic_cuda,
+                         PML_IMPOSE_BOUNDARY_CONDITION_ELASTIC_CUDA)(long* Mesh_pointer) {
+  Mesh* m = from_handle(Mesh_pointer);
+  if (!m->pml_conditions || m->pml_nbc == 0) return;
+  flush_el(m, 0);
+  launch_zero_points(m->stream, m->pml_nbc, m->pml_bc_pts, m->el[0].displ, m->el[0].veloc, m->el[0].accel);
+}
+
+// transfer_fields_cuda.cu -- PML_displ_old/new(NDIM, NGLL3, NSPEC_CPML). The library keeps
+// PML_displ_old per global point and forms PML_displ_new from u and v inside the C-PML kernel, so
+// "from" returns the stored old values and new = u + deltatover2 (1 - 2 theta) v + deltatsqover2
+// (1 - theta) a of the current field (the reference's array between the predictor and the
+// stiffness); "to" sets the old values (equal at shared points) and ignores new, which the next
+// predictor recomputes in the reference as well.
+static void pml_displ_check(Mesh* m, int* size, const char* where) {
+  if (!m->pml_state) fatal(where, "C-PML arrays not prepared");
+  if (*size != 3 * 125 * m->nspec_cpml) fatal(where, "size must be NDIM*NGLL3*NSPEC_CPML");
+}
+
+extern "C" void FC_FUNC_(transfer_pml_displ_from_device, TRANSFER_PML_DISPL_FROM_DEVICE)(int* size,
+                                                                                       realw* PML_displ_old,
+                                                                                       realw* PML_displ_new,
+                                                                                       long* Mesh_pointer) {
+  Mesh* m = from_handle(Mesh_pointer);
+  pml_displ_check(m, size, "transfer_pml_displ_from_device");
+  flush_el(m, 0);
+  const size_t n3 = 3 * (size_t)m->nglob;
+  std::vector<realw> u(n3), v(n3), a(n3), od(n3);
+  SF_CHECK(hipMemcpyAsync(u.data(), m->el[0].displ, sizeof(realw) * n3, hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipMemcpyAsync(v.data(), m->el[0].veloc, sizeof(realw) * n3, hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipMemcpyAsync(a.data(), m->el[0].accel, sizeof(realw) * n3, hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipMemcpyAsync(od.data(), m->pml_old, sizeof(realw) * n3, hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  const std::vector<int>& ib = host_ibool(m);
+  std::vector<int> el(m->pml_n), cp(m->pml_n);
+  SF_CHECK(hipMemcpy(el.data(), m->pml_elems, sizeof(int) * el.size(), hipMemcpyDeviceToHost));
+  SF_CHECK(hipMemcpy(cp.data(), m->pml_cpml, sizeof(int) * cp.size(), hipMemcpyDeviceToHost));
+  for (int q = 0; q < m->pml_n; q++)
+    for (int p = 0; p < 125; p++) {
+      const int64_t g = ib[(int64_t)el[q] * 125 + p];
+      for (int c = 0; c < 3; c++) {
+        const int64_t d = ((int64_t)cp[q] * 125 + p) * 3 + c;
+        PML_displ_old[d] = od[3 * g + c];
+        PML_displ_new[d] = u[3 * g + c] + m->pml_t1 * v[3 * g + c] + m->pml_t2 * a[3 * g + c];
+      }
+    }
+}
+
+extern "C" void FC_FUNC_(transfer_pml_displ_to_device, TRANSFER_PML_DISPL_TO_DEVICE)(int* size, realw* PML_displ_old,
+                                                                                   realw* PML_displ_new,
+                                                                                   long* Mesh_pointer) {
+  Mesh* m = from_handle(Mesh_pointer);
+  (void)PML_displ_new;
+  pml_displ_check(m, size, "transfer_pml_displ_to_device");
+  const size_t n3 = 3 * (size_t)m->nglob;
+  std::vector<realw> od(n3);
+  SF_CHECK(hipMemcpyAsync(od.data(), m->pml_old, sizeof(realw) * n3, hipMemcpyDeviceToHost, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+  const std::vector<int>& ib = host_ibool(m);
+  std::vector<int> el(m->pml_n), cp(m->pml_n);
+  SF_CHECK(hipMemcpy(el.data(), m->pml_elems, sizeof(int) * el.size(), hipMemcpyDeviceToHost));
+  SF_CHECK(hipMemcpy(cp.data(), m->pml_cpml, sizeof(int) * cp.size(), hipMemcpyDeviceToHost));
+  for (int q = 0; q < m->pml_n; q++)
+    for (int p = 0; p < 125; p++) {
+      const int64_t g = ib[(int64_t)el[q] * 125 + p];
+      for (int c = 0; c < 3; c++) od[3 * g + c] = PML_displ_old[((int64_t)cp[q] * 125 + p) * 3 + c];
+    }
+  SF_CHECK(hipMemcpyAsync(m->pml_old, od.data(), sizeof(realw) * n3, hipMemcpyHostToDevice, m->stream));
+  SF_CHECK(hipStreamSynchronize(m->stream));
+}
+
 // ---------------------------------------------------------------- boundaries
 // compute_stacey_viscoelastic_cuda.cu:39. Forward field: apply (and, when saving the forward
 // run, fill b_absorb_field(3,25,nfaces) for the host to write). Backward field of a kernel
@@ -284,7 +408,8 @@ extern "C" void FC_FUNC_(compute_stacey_viscoelastic_cuda, COMPUTE_STACEY_VISCOE
   if (foa == 0 || foa == 1) {
     flush_el(m, 0);
     const bool store = m->simulation_type == 1 && m->save_forward && b_absorb_field;
-    launch_stacey_elastic(m->stream, m->abs_el, m->el[0].veloc, m->el[0].accel, m->b_absorb_field, store ? 1 : 0);
+    launch_stacey_elastic(m->stream, m->abs_el, m->el[0].veloc, m->el[0].accel, m->b_absorb_field, store ? 1 : 0,
+                          face_atomics(m));
     if (store) {
       SF_CHECK(hipMemcpyAsync(b_absorb_field, m->b_absorb_field, sizeof(realw) * rec_len, hipMemcpyDeviceToHost,
                               m->stream));
@@ -312,7 +437,7 @@ extern "C" void FC_FUNC_(compute_stacey_viscoelastic_undoatt_cuda, COMPUTE_STACE
     if (foa == 3 && f == 0) continue;
     if (!m->el[f].displ) continue;
     flush_el(m, f);
-    launch_stacey_elastic(m->stream, m->abs_el, m->el[f].veloc, m->el[f].accel, nullptr, 0);
+    launch_stacey_elastic(m->stream, m->abs_el, m->el[f].veloc, m->el[f].accel, nullptr, 0, face_atomics(m));
   }
 }
 
@@ -327,7 +452,8 @@ extern "C" void FC_FUNC_(compute_stacey_acoustic_cuda, COMPUTE_STACEY_ACOUSTIC_C
   if (foa == 0 || foa == 1) {
     flush_ac(m, 0);
     const bool store = m->simulation_type == 1 && m->save_forward && h_b_absorb_potential;
-    launch_stacey_acoustic(m->stream, m->abs_ac, m->ac[0].potd, m->ac[0].potdd, m->b_absorb_potential, store ? 1 : 0);
+    launch_stacey_acoustic(m->stream, m->abs_ac, m->ac[0].potd, m->ac[0].potdd, m->b_absorb_potential, store ? 1 : 0,
+                           face_atomics(m));
     if (store) {
       SF_CHECK(hipMemcpyAsync(h_b_absorb_potential, m->b_absorb_potential, sizeof(realw) * rec_len,
                               hipMemcpyDeviceToHost, m->stream));
@@ -353,7 +479,7 @@ extern "C" void FC_FUNC_(compute_stacey_acoustic_undoatt_cuda, COMPUTE_STACEY_AC
     if (foa == 3 && f == 0) continue;
     if (!m->ac[f].pot) continue;
     flush_ac(m, f);
-    launch_stacey_acoustic(m->stream, m->abs_ac, m->ac[f].potd, m->ac[f].potdd, nullptr, 0);
+    launch_stacey_acoustic(m->stream, m->abs_ac, m->ac[f].potd, m->ac[f].potdd, nullptr, 0, face_atomics(m));
   }
 }
 
@@ -370,7 +496,7 @@ extern "C" void FC_FUNC_(compute_coupling_ac_el_cuda, COMPUTE_COUPLING_AC_EL_CUD
     if (!m->ac[f].pot || !m->el[f].displ) continue;
     flush_ac(m, f);
     const int flip = (m->simulation_type != 1 && f == 0) ? 1 : 0;
-    launch_coupling_ac_el(m->stream, m->coupling, m->el[f].displ, m->ac[f].potdd, flip);
+    launch_coupling_ac_el(m->stream, m->coupling, m->el[f].displ, m->ac[f].potdd, flip, face_atomics(m));
   }
 }
 
@@ -388,7 +514,7 @@ extern "C" void FC_FUNC_(compute_coupling_el_ac_cuda, COMPUTE_COUPLING_EL_AC_CUD
     flush_ac(m, f);
     flush_el(m, f);
     const int flip = (m->simulation_type != 1 && f == 0) ? 1 : 0;
-    launch_coupling_el_ac(m->stream, m->coupling, m->ac[f].potdd, m->el[f].accel, flip);
+    launch_coupling_el_ac(m->stream, m->coupling, m->ac[f].potdd, m->el[f].accel, flip, face_atomics(m));
   }
 }
 

@@ -831,6 +831,7 @@ extern "C" long sf_batch_create(long* Mesh_pointer, int nshots, const int* nstep
   Mesh* m = from_handle(Mesh_pointer);
   if (nshots <= 0) fatal("sf_batch_create", "need at least one shot");
   if (!m->rmass || !m->kappamu || m->nspec_elastic <= 0) fatal("sf_batch_create", "mesh has no prepared elastic domain");
+  if (m->pml_n > 0) fatal("sf_batch_create", "C-PML meshes are not supported by the batched engine");
   if (m->nspec_acoustic > 0 && (!m->ac[0].pot || !m->rmass_ac))
     fatal("sf_batch_create", "mesh has fluid elements but prepare_fields_acoustic_device was not called");
   if (m->attenuation && !m->fc_index && !m->factor_common)

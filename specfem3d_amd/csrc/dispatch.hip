@@ -21,6 +21,8 @@ void launch_element_strain_exact(hipStream_t, const ElasticArgs&, realw*, realw*
 void launch_element_strain_fast(hipStream_t, const ElasticArgs&, realw*, realw*, bool);
 void launch_elastic_batch_exact(hipStream_t, const BatchArgs&, int, int, int);
 void launch_elastic_batch_fast(hipStream_t, const BatchArgs&, int, int, int);
+void launch_elastic_pml_exact(hipStream_t, const PmlArgs&, int);
+void launch_elastic_pml_fast(hipStream_t, const PmlArgs&, int);
 void launch_acoustic_exact(hipStream_t, const AcousticArgs&, int);
 void launch_acoustic_fast(hipStream_t, const AcousticArgs&, int);
 
@@ -38,6 +40,13 @@ void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, real
     launch_element_strain_exact(s, a, eps, eps_tr3, with_trace);
   else
     launch_element_strain_fast(s, a, eps, eps_tr3, with_trace);
+}
+
+void launch_elastic_pml(hipStream_t s, const PmlArgs& a, bool exact, int mode) {
+  if (exact)
+    launch_elastic_pml_exact(s, a, mode);
+  else
+    launch_elastic_pml_fast(s, a, mode);
 }
 
 void launch_visco_batch_exact(hipStream_t, const BatchArgs&, int, int);

@@ -6,7 +6,9 @@
 // the host's face order). One lane per face point; one launch per round updates with plain
 // read-modify-writes (no two lanes of a round touch the same point), so the sums at points
 // shared by several faces are accumulated in the reference's face order and are
-// deterministic. Built with -ffp-contract=off (reference rounding).
+// deterministic. When the element stiffness itself scatters with float atomics (so the sums
+// have no fixed order anyway), the rounds run as one launch with atomic adds (ATOMIC).
+// Built with -ffp-contract=off (reference rounding).
 #include "kernels.h"
 
 namespace sf {
@@ -14,6 +16,7 @@ namespace sf {
 // Stacey elastic (compute_stacey_viscoelastic.F90:85-118):
 //   t = rho vp (v.n) n + rho vs (v - (v.n) n);  a -= t w
 // `store` (optional) receives t w at the point's record slot (SAVE_FORWARD boundary record).
+template <bool ATOMIC>
 __global__ void k_stacey_el(int n, const int* __restrict__ ig, const realw* __restrict__ nrm,
                             const realw* __restrict__ jw, const realw* __restrict__ rvp,
                             const realw* __restrict__ rvs, const int* __restrict__ slot,
@@ -30,9 +33,15 @@ __global__ void k_stacey_el(int n, const int* __restrict__ ig, const realw* __re
   const realw tz = p * vn * nz + s * (vz - vn * nz);
   const realw w = jw[t];
   const realw fx = tx * w, fy = ty * w, fz = tz * w;
-  accel[3 * g] = accel[3 * g] - fx;
-  accel[3 * g + 1] = accel[3 * g + 1] - fy;
-  accel[3 * g + 2] = accel[3 * g + 2] - fz;
+  if (ATOMIC) {
+    atomicAdd(accel + 3 * g, -fx);
+    atomicAdd(accel + 3 * g + 1, -fy);
+    atomicAdd(accel + 3 * g + 2, -fz);
+  } else {
+    accel[3 * g] = accel[3 * g] - fx;
+    accel[3 * g + 1] = accel[3 * g + 1] - fy;
+    accel[3 * g + 2] = accel[3 * g + 2] - fz;
+  }
   if (store) {
     const int q = slot[t];
     store[3 * q] = fx;
@@ -55,6 +64,7 @@ __global__ void k_stacey_el_replay(int n, const int* __restrict__ ig, const int*
 
 // Stacey acoustic (compute_stacey_acoustic.f90:96-112): chi'' -= chi' w / c / rho.
 // imp_p holds rho, imp_s holds c = sqrt(kappa/rho), both per face point.
+template <bool ATOMIC>
 __global__ void k_stacey_ac(int n, const int* __restrict__ ig, const realw* __restrict__ jw,
                             const realw* __restrict__ rho, const realw* __restrict__ cp, const int* __restrict__ slot,
                             const realw* __restrict__ potd, realw* potdd, realw* __restrict__ store) {
@@ -62,7 +72,8 @@ __global__ void k_stacey_ac(int n, const int* __restrict__ ig, const realw* __re
   if (t >= n) return;
   const int g = ig[t];
   const realw absorbl = potd[g] * jw[t] / cp[t] / rho[t];
-  potdd[g] = potdd[g] - absorbl;
+  if (ATOMIC) atomicAdd(potdd + g, -absorbl);
+  else potdd[g] = potdd[g] - absorbl;
   if (store) store[slot[t]] = absorbl;
 }
 
@@ -75,6 +86,7 @@ __global__ void k_stacey_ac_replay(int n, const int* __restrict__ ig, const int*
 }
 
 // acoustic side of the coupling (compute_coupling_acoustic_el.f90:84-151): chi'' += w (u.n)
+template <bool ATOMIC>
 __global__ void k_coupling_ac_el(int n, const int* __restrict__ ig, const realw* __restrict__ nrm,
                                  const realw* __restrict__ jw, const realw* __restrict__ displ, realw* potdd,
                                  int flip) {
@@ -84,10 +96,12 @@ __global__ void k_coupling_ac_el(int n, const int* __restrict__ ig, const realw*
   realw ux = displ[3 * g], uy = displ[3 * g + 1], uz = displ[3 * g + 2];
   if (flip) { ux = -ux; uy = -uy; uz = -uz; }
   const realw un = ux * nrm[3 * t] + uy * nrm[3 * t + 1] + uz * nrm[3 * t + 2];
-  potdd[g] = potdd[g] + jw[t] * un;
+  if (ATOMIC) atomicAdd(potdd + g, jw[t] * un);
+  else potdd[g] = potdd[g] + jw[t] * un;
 }
 
 // elastic side (compute_coupling_viscoelastic_ac.f90:84-160): a += w n (-chi'')
+template <bool ATOMIC>
 __global__ void k_coupling_el_ac(int n, const int* __restrict__ ig, const realw* __restrict__ nrm,
                                  const realw* __restrict__ jw, const realw* __restrict__ potdd, realw* accel,
                                  int flip) {
@@ -97,9 +111,15 @@ __global__ void k_coupling_el_ac(int n, const int* __restrict__ ig, const realw*
   realw pr = -potdd[g];
   if (flip) pr = -pr;
   const realw w = jw[t];
-  accel[3 * g] = accel[3 * g] + w * nrm[3 * t] * pr;
-  accel[3 * g + 1] = accel[3 * g + 1] + w * nrm[3 * t + 1] * pr;
-  accel[3 * g + 2] = accel[3 * g + 2] + w * nrm[3 * t + 2] * pr;
+  if (ATOMIC) {
+    atomicAdd(accel + 3 * g, w * nrm[3 * t] * pr);
+    atomicAdd(accel + 3 * g + 1, w * nrm[3 * t + 1] * pr);
+    atomicAdd(accel + 3 * g + 2, w * nrm[3 * t + 2] * pr);
+  } else {
+    accel[3 * g] = accel[3 * g] + w * nrm[3 * t] * pr;
+    accel[3 * g + 1] = accel[3 * g + 1] + w * nrm[3 * t + 1] * pr;
+    accel[3 * g + 2] = accel[3 * g + 2] + w * nrm[3 * t + 2] * pr;
+  }
 }
 
 __global__ void k_free_surface(int n, const int* __restrict__ ig, realw* pot, realw* potd, realw* potdd) {
@@ -120,10 +140,16 @@ __global__ void k_free_surface(int n, const int* __restrict__ ig, realw* pot, re
   }
 
 void launch_stacey_elastic(hipStream_t s, const FacePoints& f, const realw* veloc, realw* accel, realw* b_absorb,
-                           int mode) {
-  SF_ROUNDS(f, hipLaunchKernelGGL(k_stacey_el, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
-                                  f.normal + 3 * o, f.jacw + o, f.imp_p + o, f.imp_s + o, f.slot + o, veloc, accel,
-                                  mode ? b_absorb : (realw*)nullptr));
+                           int mode, bool atomic) {
+  if (atomic) {
+    if (f.n > 0)
+      hipLaunchKernelGGL(k_stacey_el<true>, sf_grid(f.n, 128), dim3(128), 0, s, f.n, f.iglob, f.normal, f.jacw,
+                         f.imp_p, f.imp_s, f.slot, veloc, accel, mode ? b_absorb : (realw*)nullptr);
+  } else {
+    SF_ROUNDS(f, hipLaunchKernelGGL(k_stacey_el<false>, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
+                                    f.normal + 3 * o, f.jacw + o, f.imp_p + o, f.imp_s + o, f.slot + o, veloc, accel,
+                                    mode ? b_absorb : (realw*)nullptr));
+  }
   SF_LAUNCH_CHECK("k_stacey_el");
 }
 
@@ -134,10 +160,16 @@ void launch_stacey_elastic_replay(hipStream_t s, const FacePoints& f, const real
 }
 
 void launch_stacey_acoustic(hipStream_t s, const FacePoints& f, const realw* potd, realw* potdd, realw* b_absorb,
-                            int mode) {
-  SF_ROUNDS(f, hipLaunchKernelGGL(k_stacey_ac, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o, f.jacw + o,
-                                  f.imp_p + o, f.imp_s + o, f.slot + o, potd, potdd,
-                                  mode ? b_absorb : (realw*)nullptr));
+                            int mode, bool atomic) {
+  if (atomic) {
+    if (f.n > 0)
+      hipLaunchKernelGGL(k_stacey_ac<true>, sf_grid(f.n, 128), dim3(128), 0, s, f.n, f.iglob, f.jacw, f.imp_p,
+                         f.imp_s, f.slot, potd, potdd, mode ? b_absorb : (realw*)nullptr);
+  } else {
+    SF_ROUNDS(f, hipLaunchKernelGGL(k_stacey_ac<false>, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
+                                    f.jacw + o, f.imp_p + o, f.imp_s + o, f.slot + o, potd, potdd,
+                                    mode ? b_absorb : (realw*)nullptr));
+  }
   SF_LAUNCH_CHECK("k_stacey_ac");
 }
 
@@ -147,15 +179,29 @@ void launch_stacey_acoustic_replay(hipStream_t s, const FacePoints& f, const rea
   SF_LAUNCH_CHECK("k_stacey_ac_replay");
 }
 
-void launch_coupling_ac_el(hipStream_t s, const FacePoints& f, const realw* displ, realw* potdd, int flip) {
-  SF_ROUNDS(f, hipLaunchKernelGGL(k_coupling_ac_el, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
-                                  f.normal + 3 * o, f.jacw + o, displ, potdd, flip));
+void launch_coupling_ac_el(hipStream_t s, const FacePoints& f, const realw* displ, realw* potdd, int flip,
+                           bool atomic) {
+  if (atomic) {
+    if (f.n > 0)
+      hipLaunchKernelGGL(k_coupling_ac_el<true>, sf_grid(f.n, 128), dim3(128), 0, s, f.n, f.iglob, f.normal, f.jacw,
+                         displ, potdd, flip);
+  } else {
+    SF_ROUNDS(f, hipLaunchKernelGGL(k_coupling_ac_el<false>, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
+                                    f.normal + 3 * o, f.jacw + o, displ, potdd, flip));
+  }
   SF_LAUNCH_CHECK("k_coupling_ac_el");
 }
 
-void launch_coupling_el_ac(hipStream_t s, const FacePoints& f, const realw* potdd, realw* accel, int flip) {
-  SF_ROUNDS(f, hipLaunchKernelGGL(k_coupling_el_ac, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
-                                  f.normal + 3 * o, f.jacw + o, potdd, accel, flip));
+void launch_coupling_el_ac(hipStream_t s, const FacePoints& f, const realw* potdd, realw* accel, int flip,
+                           bool atomic) {
+  if (atomic) {
+    if (f.n > 0)
+      hipLaunchKernelGGL(k_coupling_el_ac<true>, sf_grid(f.n, 128), dim3(128), 0, s, f.n, f.iglob, f.normal, f.jacw,
+                         potdd, accel, flip);
+  } else {
+    SF_ROUNDS(f, hipLaunchKernelGGL(k_coupling_el_ac<false>, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
+                                    f.normal + 3 * o, f.jacw + o, potdd, accel, flip));
+  }
   SF_LAUNCH_CHECK("k_coupling_el_ac");
 }
 

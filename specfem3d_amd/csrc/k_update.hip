@@ -86,6 +86,43 @@ __global__ void k_corrector_predictor(int64_t n4, int64_t n, realw* __restrict__
   }
 }
 
+// PML_displ_old (update_displ_elastic_PML, update_displacement_scheme.f90:417-455) at the global points
+// of the C-PML elements, from the field before the predictor: u + t1 v + t2 a with
+// t1 = deltatover2 (1 - 2 theta), t2 = deltatsqover2 (1 - theta). With a corrector still pending (it
+// is fused into the predictor that follows), the corrected v and a are formed first with the
+// corrector's own expressions.
+__global__ void k_pml_store_old(int n, const int* __restrict__ pts, const realw* __restrict__ u,
+                                const realw* __restrict__ v, const realw* __restrict__ a,
+                                const realw* __restrict__ rm, int pending, realw dt2p, realw t1, realw t2,
+                                realw* __restrict__ old) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t g = pts[t];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    realw ac = a[3 * g + c], vc = v[3 * g + c];
+    if (pending) {
+      ac = ac * rm[3 * g + c];
+      vc = vc + dt2p * ac;
+    }
+    old[3 * g + c] = u[3 * g + c] + t1 * vc + t2 * ac;
+  }
+}
+
+// Dirichlet condition on the outer faces of the C-PML layers (pml_impose_boundary_condition_elastic)
+__global__ void k_zero_points(int n, const int* __restrict__ pts, realw* __restrict__ u, realw* __restrict__ v,
+                              realw* __restrict__ a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t g = pts[t];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    u[3 * g + c] = 0.f;
+    v[3 * g + c] = 0.f;
+    a[3 * g + c] = 0.f;
+  }
+}
+
 static dim3 grid_for(int64_t n4, int block) {
   int64_t b = (n4 + block - 1) / block;
   if (b < 1) b = 1;
@@ -113,6 +150,20 @@ void launch_corrector_predictor(hipStream_t s, int64_t n, realw* u, realw* v, re
   hipLaunchKernelGGL(k_corrector_predictor, grid_for(n4, 256), dim3(256), 0, s, n4, n, u, v, a, rmass, dt2_prev,
                      dt, dt2, dtsq2);
   SF_LAUNCH_CHECK("k_corrector_predictor");
+}
+
+void launch_pml_store_old(hipStream_t s, int n, const int* pts, const realw* u, const realw* v, const realw* a,
+                          const realw* rmass, int pending, realw dt2p, realw t1, realw t2, realw* old) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pml_store_old, grid_for(n, 256), dim3(256), 0, s, n, pts, u, v, a, rmass, pending, dt2p, t1,
+                     t2, old);
+  SF_LAUNCH_CHECK("k_pml_store_old");
+}
+
+void launch_zero_points(hipStream_t s, int n, const int* pts, realw* u, realw* v, realw* a) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_points, grid_for(n, 256), dim3(256), 0, s, n, pts, u, v, a);
+  SF_LAUNCH_CHECK("k_zero_points");
 }
 
 }  // namespace sf

@@ -74,6 +74,39 @@ void launch_elastic(hipStream_t s, const ElasticArgs& a, bool strain, bool att, 
 void launch_element_strain(hipStream_t s, const ElasticArgs& a, realw* eps, realw* eps_tr3, bool with_trace,
                            bool exact);
 
+// C-PML elements (k_pml.hip): stiffness with the convolution memory variables of
+// compute_forces_viscoelastic_PML; per C-PML element blocks of 125-point planes
+constexpr int SF_PML_NSTATE = 48;
+constexpr int SF_PML_NCOEF = 42;
+struct PmlArgs {
+  int nelem;
+  const int* elems;        // element ids ...
+  const int* cpml;         // ... and their 0-based C-PML index
+  const int* ibool;
+  const int* irregular;
+  realw xix_regular, jacobian_regular;
+  const realw* jac;        // 10 planes
+  int64_t jac_plane;
+  const float2* kappamu;
+  const realw* displ;
+  const realw* veloc;
+  realw* accel;
+  const realw* pml_old;    // 3*nglob
+  realw* state;            // SF_PML_NSTATE planes per C-PML element
+  const realw* coef;       // SF_PML_NCOEF planes per C-PML element
+  realw t1, t2;            // deltatover2 (1 - 2 theta), deltatsqover2 (1 - theta)
+  realw* local_out;        // ordered assembly: 3 planes of nloc
+  int64_t nloc;
+};
+// mode as launch_elastic
+void launch_elastic_pml(hipStream_t s, const PmlArgs& a, bool exact, int mode);
+// PML_displ_old at the listed points: u + t1 v + t2 a, with the pending corrector (a *= rmass,
+// v += dt2p a) applied first when dt2p_pending != 0
+void launch_pml_store_old(hipStream_t s, int n, const int* pts, const realw* u, const realw* v, const realw* a,
+                          const realw* rmass, int pending, realw dt2p, realw t1, realw t2, realw* old);
+// zero u, v, a at the listed points (Dirichlet condition of the C-PML outer faces)
+void launch_zero_points(hipStream_t s, int n, const int* pts, realw* u, realw* v, realw* a);
+
 // batched many-shot stiffness (k_batch.hip): fields interleaved shot-fastest,
 // value (g, c, s) at ((int64_t)g * 3 + c) * S + s
 struct BatchArgs {
@@ -159,14 +192,18 @@ void launch_ordered_gather(hipStream_t s, int nglob, int ncomp, const int* ptr, 
                            const realw* local, int64_t nloc, realw* out);
 
 // boundary terms (k_boundary.hip)
+// atomic: one launch with float atomics instead of one launch per round (used when the element
+// stiffness scatters with atomics, so the sums have no fixed order anyway)
 void launch_stacey_elastic(hipStream_t s, const FacePoints& f, const realw* veloc, realw* accel, realw* b_absorb,
-                           int mode);
+                           int mode, bool atomic);
 void launch_stacey_elastic_replay(hipStream_t s, const FacePoints& f, const realw* rec, realw* b_accel);
 void launch_stacey_acoustic(hipStream_t s, const FacePoints& f, const realw* potd, realw* potdd, realw* b_absorb,
-                            int mode);
+                            int mode, bool atomic);
 void launch_stacey_acoustic_replay(hipStream_t s, const FacePoints& f, const realw* rec, realw* b_potdd);
-void launch_coupling_ac_el(hipStream_t s, const FacePoints& f, const realw* displ, realw* potdd, int flip);
-void launch_coupling_el_ac(hipStream_t s, const FacePoints& f, const realw* potdd, realw* accel, int flip);
+void launch_coupling_ac_el(hipStream_t s, const FacePoints& f, const realw* displ, realw* potdd, int flip,
+                           bool atomic);
+void launch_coupling_el_ac(hipStream_t s, const FacePoints& f, const realw* potdd, realw* accel, int flip,
+                           bool atomic);
 void launch_free_surface(hipStream_t s, const FacePoints& f, realw* pot, realw* potd, realw* potdd);
 
 // sources / receivers (k_sources.hip); FacePoints.ispec holds the owning source / receiver

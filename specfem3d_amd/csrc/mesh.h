@@ -126,6 +126,31 @@ struct Mesh {
   int num_abs_boundary_faces = 0;
   int* h_abs_face_point_el = nullptr;  // host map from record slot (face*25+igll) to abs_el index
 
+  // C-PML (elastic; prepare_fields_elastic_pml): the stiffness kernel skips the C-PML elements and
+  // a second kernel computes them with their convolution memory variables
+  int pml_conditions = 0;          // PML_CONDITIONS of prepare_constants_device
+  int nspec_cpml = 0;
+  int pml_n = 0;                   // listed C-PML elements (elastic ones, processing order) ...
+  int* pml_elems = nullptr;        // ... their element ids ...
+  int* pml_cpml = nullptr;         // ... and their 0-based C-PML index
+  int pml_ncolor = 0;              // coloured assembly: the C-PML list regrouped by colour
+  int pml_color_off[33] = {0};
+  int* pml_color_elems = nullptr;
+  int* pml_color_cpml = nullptr;
+  // per C-PML element blocks of 125-point planes: state (48 planes: 27 three-term strain memory
+  // variables, 12 one-term ones, 9 displacement memory variables) and coefficients (42 planes:
+  // alpha 9, beta 9, abar 5, strain 18, wgll_cube * rho)
+  realw* pml_state = nullptr;
+  realw* pml_coef = nullptr;
+  // PML_displ_old at every global point (3*nglob; it is a function of the point's u, v, a before the
+  // predictor); PML_displ_new is recomputed in the kernel from u and v after the predictor (a = 0)
+  realw* pml_old = nullptr;
+  int pml_npts = 0;                // global points of the C-PML elements (store list)
+  int* pml_pts = nullptr;
+  int pml_nbc = 0;                 // global points on the outer faces of the layers (Dirichlet list)
+  int* pml_bc_pts = nullptr;
+  realw pml_t1 = 0.f, pml_t2 = 0.f;  // deltatover2 (1 - 2 theta), deltatsqover2 (1 - theta)
+
   // sources
   int nsources = 0;
   FacePoints src_pts;              // (source, point) pairs of sources in elastic elements

@@ -64,9 +64,7 @@ SF_UNSUPPORTED(initialize_fault_data_gpu, "fault simulation", long*, int*, int*,
 SF_UNSUPPORTED(initialize_fault_solver_gpu, "fault simulation", long*, int*, realw*, realw*, int*)
 SF_UNSUPPORTED(make_displ_rand, "noise tomography", long*, realw*)
 SF_UNSUPPORTED(noise_read_add_surface_movie_cu, "noise tomography", long*, realw*, int*)
-SF_UNSUPPORTED(pml_impose_boundary_condition_elastic_cuda, "C-PML", long*)
 SF_UNSUPPORTED(prepare_fault_device, "fault simulation", long*, int*, realw*)
-SF_UNSUPPORTED(prepare_fields_elastic_pml, "C-PML", long*, int*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*)
 SF_UNSUPPORTED(prepare_fields_gravity_device, "gravity", long*, int*, realw*, realw*, realw*, realw*)
 SF_UNSUPPORTED(prepare_fields_noise_device, "noise tomography", long*, int*, int*, int*, int*, int*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*)
 SF_UNSUPPORTED(prepare_smooth_gpu, "kernel smoothing", long*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*, int*)
@@ -78,8 +76,6 @@ SF_UNSUPPORTED(transfer_fault_data_to_host, "fault simulation", long*, int*, int
 SF_UNSUPPORTED(transfer_kernels_hess_ac_tohost, "Hessian kernels", long*, realw*, realw*, realw*, int*)
 SF_UNSUPPORTED(transfer_kernels_hess_el_tohost, "Hessian kernels", long*, realw*, realw*, realw*, realw*, int*)
 SF_UNSUPPORTED(transfer_kernels_noise_to_host, "noise tomography", long*, realw*, int*)
-SF_UNSUPPORTED(transfer_pml_displ_from_device, "C-PML", int*, realw*, realw*, long*)
-SF_UNSUPPORTED(transfer_pml_displ_to_device, "C-PML", int*, realw*, realw*, long*)
 SF_UNSUPPORTED(transfer_rsf_data_todevice, "fault simulation", long*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, int*)
 SF_UNSUPPORTED(transfer_rsf_data_tohost, "fault simulation", long*, int*, int*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*, realw*)
 SF_UNSUPPORTED(transfer_sigma_from_device, "noise tomography", int*, realw*, long*)

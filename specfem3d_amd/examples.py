@@ -81,3 +81,19 @@ def attenuation_benchmark_example(path: str | None = None, full_box: bool = Fals
     st = S.read_stations(io.StringIO(str(z["stations"])))
     rec = S.setup_receivers(db, st, z_coordinates=True)
     return db, src, rec, att, {k: z[k] for k in z.files}
+
+
+def cpml5_example(path: str | None = None):
+    """(db, sources, receivers, ref_seis (nstep, nrec, 3), DT, fixture dict) of the reference example
+    CPML_examples/homogeneous_halfspace_HEX8_elastic_absorbing_CPML_5sides: C-PML layers (three elements
+    thick) on the four sides and the bottom of a 36 x 36 x 16 CUBIT mesh, free top, a point force
+    (1e10 N along (1, -2, -1), Ricker f0 = 0.128 Hz) at 30 km depth, four surface receivers, 2400 steps
+    of 0.11 s, f0_FOR_PML = 0.05555 Hz."""
+    z = np.load(path or os.path.join(GOLDEN, "cpml5_example.npz"), allow_pickle=False)
+    dt = float(z["DT"])
+    db = build_from_arrays({k: z[k] for k in z.files}, dt, stacey=False, pml=dict(f0=float(z["f0_FOR_PML"])))
+    forces = S.read_forcesolution(io.StringIO(str(z["forcesolution"])))
+    src = S.setup_force_sources(db, forces, dt, ricker=True)
+    st = S.read_stations(io.StringIO(str(z["stations"])))
+    rec = S.setup_receivers(db, st)
+    return db, src, rec, z["ref_seis"], dt, {k: z[k] for k in z.files}

@@ -115,6 +115,9 @@ class Database:
     qkappa: np.ndarray | None = None
     dt: float | None = None
     stacey: bool = True
+    # C-PML elements and their damping (specfem3d_amd.pml.CPML) or None; with C-PML the absorbing
+    # faces are the outer faces of the layers (Dirichlet condition) and stacey is False
+    pml: object | None = None
     meta: dict = field(default_factory=dict)
 
     # -- derived helpers --------------------------------------------------------
@@ -160,10 +163,21 @@ class Database:
             jac[irr] = self.jacobian[self.irregular_element_number[irr] - 1]
         # elastic mass: per-term real(dble(jac)*w*dble(rho)) summed in float32 in element order
         parts = {}
+        if self.pml is not None:
+            from ..pml import convolution_coefficients
+            if self.pml.dt != float(dt):
+                convolution_coefficients(self.pml, dt)
+            if (self.pml.n and not self.ispec_is_elastic[self.pml.cpml_to_spec].all()):
+                raise ValueError("C-PML elements in acoustic domains are not supported (elastic C-PML only)")
         if self.nspec_elastic:
             el = self.ispec_is_elastic
-            terms = (jac[el].astype(np.float64) * w3[None, :] * self.rhostore[el].astype(np.float64)).astype(np.float32)
-            rmass = _ordered_scatter_add(nglob, ib[el], terms)
+            if self.pml is not None:
+                # define_mass_matrices_pml_elastic: C-PML elements carry K + d dt/2
+                from ..pml import pml_elastic_mass
+                rmass = pml_elastic_mass(self, self.pml, dt)
+            else:
+                terms = (jac[el].astype(np.float64) * w3[None, :] * self.rhostore[el].astype(np.float64)).astype(np.float32)
+                rmass = _ordered_scatter_add(nglob, ib[el], terms)
             parts["rmass"] = rmass
             if stacey and self.abs_faces.n:
                 sx, sy, sz = self._stacey_mass_elastic(dt)
@@ -237,7 +251,8 @@ class Database:
                     nspec_elastic=self.nspec_elastic, nspec_acoustic=self.nspec_acoustic,
                     n_abs_faces=self.abs_faces.n, n_free_faces=self.free_faces.n,
                     n_coupling_faces=self.coupling_faces.n,
-                    attenuation=self.attenuation is not None)
+                    attenuation=self.attenuation is not None,
+                    nspec_cpml=0 if self.pml is None else self.pml.n)
 
 
 def _ordered_scatter_add(n: int, idx: np.ndarray, vals: np.ndarray) -> np.ndarray:

@@ -6,9 +6,12 @@ length markers, arrays in Fortran (column-major) order, ``CUSTOM_REAL`` float32,
 integers. The record sequence is the one ``save_arrays_solver_mesh`` writes
 (src/generate_databases/save_arrays_solver.F90:112-330) and ``read_mesh_databases`` reads back
 (src/specfem3D/read_mesh_databases.F90:158-1100); attenuation.bin as
-src/specfem3D/prepare_attenuation.f90:93-111 reads it. Records that depend on run-time switches
-(ocean load, C-PML, anisotropy, GPU mesh colouring) are not written and refused on reading; records
-without a counterpart here (poroelastic domains, MPI interfaces) are written empty.
+src/specfem3D/prepare_attenuation.f90:93-111 reads it. The C-PML records (save_arrays_solver.F90:
+186-218: widths, separation distance, regions, element lists, damping profiles) are written when the
+database has C-PML layers and read with ``pml=True``; the interface-point lists that only adjoint runs
+with C-PML use are not. Records of other run-time switches (ocean load, anisotropy, GPU mesh
+colouring) are not written and refused on reading; records without a counterpart here (poroelastic
+domains, MPI interfaces) are written empty.
 
 ``write_solver_database`` also fills the bookkeeping the solver only uses outside the time loop:
 boundary element lists ``ibelm_*`` from the faces' ids, the external-surface flags (faces of one
@@ -153,11 +156,26 @@ def write_solver_database(db: Database, path: str):
         w.rec(mp["rmass"])
         w.rec(loc(db.rho_vp))
         w.rec(loc(db.rho_vs))
+    pml = db.pml
+    if pml is not None:
+        w.i(pml.n)
+        for c in range(3):
+            w.rec(np.float32(pml.width[c]))
+        w.rec(np.float32(pml.min_distance))
+        if pml.n:
+            w.rec(pml.regions.astype(np.int32))
+            w.rec(pml.cpml_to_spec.astype(np.int32) + 1)
+            w.rec(_logical(pml.is_cpml(nspec)))
+            for arr in (pml.d, pml.K, pml.alpha):
+                for c in range(3):
+                    w.rec(loc(arr[c]))
     f = db.abs_faces
-    w.i(f.n if db.stacey else 0)
-    if db.stacey and f.n:
+    with_faces = db.stacey or pml is not None
+    w.i(f.n if with_faces else 0)
+    if with_faces and f.n:
         for a in _faces_fortran(f):
             w.rec(a)
+    if db.stacey and pml is None and f.n:
         if elastic:
             w.rec(mp["rmassx"]); w.rec(mp["rmassy"]); w.rec(mp["rmassz"])
         if acoustic:
@@ -165,7 +183,7 @@ def write_solver_database(db: Database, path: str):
     w.i(9998)
     # boundary element lists by side (xmin, xmax, ymin, ymax, bottom, top)
     sides = {s: [] for s in range(1, 7)}
-    for e, ijk in zip(f.ispec if db.stacey else [], f.ijk if db.stacey else []):
+    for e, ijk in zip(f.ispec if with_faces else [], f.ijk if with_faces else []):
         sides[_face_id(ijk)].append(int(e) + 1)
     for e, ijk in zip(db.free_faces.ispec, db.free_faces.ijk):
         sides[_face_id(ijk)].append(int(e) + 1)
@@ -231,8 +249,10 @@ def read_solver_database(path: str, dt: float, stacey: bool = True, ocean_load: 
     """Read proc000000_external_mesh.bin (read_mesh_databases.F90) into a ``Database``; the switches are
     the Par_file values that decide which records exist. HEX8 anchors are not in the file: ``anchors``
     holds each element's 8 corner GLL points, which is what the locator and face geometry use."""
-    if ocean_load or pml or anisotropy or mesh_coloring:
-        raise NotImplementedError("ocean load, C-PML, anisotropy and GPU mesh colouring records are not supported")
+    if ocean_load or anisotropy or mesh_coloring:
+        raise NotImplementedError("ocean load, anisotropy and GPU mesh colouring records are not supported")
+    if pml and stacey:
+        raise ValueError("C-PML and Stacey conditions exclude each other")
     r = FortranReader(path)
     nspec, nglob, nirr = r.i()[0], r.i()[0], r.i()[0]
     ibool = r.arr(np.int32, (5, 5, 5, nspec)).transpose(3, 2, 1, 0) - 1           # [e, k, j, i]
@@ -256,12 +276,29 @@ def read_solver_database(path: str, dt: float, stacey: bool = True, ocean_load: 
     if is_el.any():
         parts["rmass"] = r.arr(np.float32)
         rho_vp, rho_vs = per_el(r.arr(np.float32), nspec), per_el(r.arr(np.float32), nspec)
+    cpml = None
+    if pml:
+        from ..pml import CPML
+        ncp = r.i()[0]
+        width = np.array([r.f() for _ in range(3)], np.float32)
+        md = np.float32(r.f())
+        if ncp:
+            regions = r.arr(np.int32)
+            cts = r.arr(np.int32) - 1
+            r.arr(np.int32)                                   # is_CPML (follows from CPML_to_spec)
+            stores = [per_el(r.arr(np.float32), ncp) for _ in range(9)]
+        else:
+            regions = cts = np.zeros(0, np.int32)
+            stores = [np.zeros((0, 125), np.float32)] * 9
+        cpml = CPML(cpml_to_spec=cts.astype(np.int32), regions=regions.astype(np.int32), f0=float("nan"),
+                    instead_of_free_surface=False, d=np.stack(stores[0:3]), K=np.stack(stores[3:6]),
+                    alpha=np.stack(stores[6:9]), width=width, min_distance=md)
     nabs = r.i()[0]
     abs_faces = Faces.empty()
     if nabs:
         abs_faces = _faces_from_fortran(r.arr(np.int32), r.arr(np.int32, (3, 25, nabs)), r.arr(np.float32, (25, nabs)),
                                         r.arr(np.float32, (3, 25, nabs)))
-        if stacey:
+        if stacey and not pml:
             if is_el.any():
                 parts["rmassx"], parts["rmassy"], parts["rmassz"] = (r.arr(np.float32) for _ in range(3))
             if is_ac.any():
@@ -299,6 +336,9 @@ def read_solver_database(path: str, dt: float, stacey: bool = True, ocean_load: 
     db.stacey = bool(stacey and nabs)
     db.mass_parts = parts
     _combine_mass(db)
+    if cpml is not None:
+        from ..pml import convolution_coefficients
+        db.pml = convolution_coefficients(cpml, dt)
     db.meta.update(kind="database file")
     return db
 

@@ -63,14 +63,23 @@ def read_mesh_dir(mesh_dir: str) -> dict:
     p = os.path.join(mesh_dir, "free_or_absorbing_surface_file_zmax")
     if os.path.exists(p):
         d["free_zmax"] = _read_table(p, np.int64)
+    p = os.path.join(mesh_dir, "absorbing_cpml_file")
+    if os.path.exists(p):
+        d["cpml"] = _read_table(p, np.int64)
     return d
 
 
-def read_external_mesh(mesh_dir: str, dt: float, stacey: bool = True) -> Database:
-    return build_from_arrays(read_mesh_dir(mesh_dir), dt, stacey=stacey)
+def read_external_mesh(mesh_dir: str, dt: float, stacey: bool = True, pml: dict | None = None) -> Database:
+    return build_from_arrays(read_mesh_dir(mesh_dir), dt, stacey=stacey, pml=pml)
 
 
-def build_from_arrays(d, dt: float, stacey: bool = True) -> Database:
+def build_from_arrays(d, dt: float, stacey: bool = True, pml: dict | None = None) -> Database:
+    """``pml`` = dict(f0=f0_FOR_PML, instead_of_free_surface=PML_INSTEAD_OF_FREE_SURFACE) turns the
+    ``absorbing_cpml_file`` elements (``ispec region`` lines, region codes 1..7) into C-PML layers: the
+    absorbing faces then carry the Dirichlet condition of the layers' outer edges and no Stacey term
+    (``pml_set_local_dampingcoeff.f90``; C-PML and Stacey exclude each other)."""
+    if pml is not None and stacey:
+        raise ValueError("C-PML and Stacey conditions exclude each other")
     nodes = np.asarray(d["nodes"], np.float64)
     node_xyz = np.zeros((int(nodes[:, 0].max()) + 1, 3))
     node_xyz[nodes[:, 0].astype(np.int64)] = nodes[:, 1:4]
@@ -152,14 +161,16 @@ def build_from_arrays(d, dt: float, stacey: bool = True) -> Database:
                 raise ValueError(f"face {row} not found on element {e + 1}")
         return np.array(isp, np.int32), np.array(ifc, np.int32)
 
-    if stacey:
+    if stacey or pml is not None:
         parts = [faces_of(d["abs_" + nm]) for nm in STACEY_SIDES if ("abs_" + nm) in d]
+        if pml is not None and pml.get("instead_of_free_surface") and "free_zmax" in d:
+            parts.append(faces_of(d["free_zmax"]))
         if parts:
             isp = np.concatenate([p[0] for p in parts])
             ifc = np.concatenate([p[1] for p in parts])
             ijk, nrm, jw = face_geometry(anchors, isp, ifc, xstore, ystore, zstore, ibool)
             db.abs_faces = Faces(isp, ijk, nrm, jw)
-    if "free_zmax" in d:
+    if "free_zmax" in d and not (pml is not None and pml.get("instead_of_free_surface")):
         isp, ifc = faces_of(d["free_zmax"])
         if len(isp):
             ijk, nrm, jw = face_geometry(anchors, isp, ifc, xstore, ystore, zstore, ibool)
@@ -169,6 +180,14 @@ def build_from_arrays(d, dt: float, stacey: bool = True) -> Database:
         if len(isp):
             ijk, nrm, jw = face_geometry(anchors, isp, ifc, xstore, ystore, zstore, ibool)
             db.coupling_faces = Faces(isp, ijk, nrm, jw)
+    if pml is not None:
+        from ..pml import set_damping
+        if "cpml" not in d:
+            raise ValueError("C-PML requested but the mesh has no absorbing_cpml_file")
+        rows = np.asarray(d["cpml"], np.int64)
+        rows = rows[np.argsort(rows[:, 0], kind="stable")]
+        db.pml = set_damping(db, (rows[:, 0] - 1).astype(np.int32), rows[:, 1].astype(np.int32), pml["f0"],
+                             bool(pml.get("instead_of_free_surface", False)))
     db.finalize(dt, stacey=stacey)
     return db
 

@@ -120,7 +120,9 @@ class GPUSolver:
         g = get_gll()
         nirr = db.nspec_irregular
         jac = [db.jac9[:, t, :] for t in range(9)] if nirr else [np.zeros(1, np.float32)] * 9
-        f = db.abs_faces if db.stacey else None
+        # absorbing faces: Stacey faces, or with C-PML the outer faces of the layers (Dirichlet)
+        f = db.abs_faces if (db.stacey or db.pml is not None) else None
+        pml = db.pml
         nsrc = self.sources.n if self.sources is not None else 0
         nrec = self.receivers.n if self.receivers is not None else 0
         if nsrc:
@@ -158,7 +160,7 @@ class GPUSolver:
             A.i(0), A.i(int(save_forward)), A.fa(hx.ravel()), A.fa(hy.ravel()), A.fa(hz.ravel()),
             A.da(nu.ravel()), A.da(np.zeros(9 * max(nsrc, 1))), A.ia(np.zeros(max(nrec, 1), np.int32)),
             A.i(self.nlength), A.i(sv("d")), A.i(sv("v")), A.i(sv("a")), A.i(sv("p")), A.i(1), A.i(0),
-            A.i(int(self.undo_att)), A.i(0))
+            A.i(int(self.undo_att)), A.i(int(pml is not None)))
         stage("prepare_constants_device")
         L.sf_set_options(self.MP, C.c_int(int(exact)), C.c_int(int(assembly)))
         for name, value in self.tuning.items():
@@ -221,6 +223,16 @@ class GPUSolver:
                         self.MP, A.i(nrec), A.i(self.nstep), A.fa(R.hxir.astype(np.float32).ravel()),
                         A.fa(R.hetar.astype(np.float32).ravel()), A.fa(R.hgammar.astype(np.float32).ravel()),
                         A.i(nrec), A.ia(np.zeros(nrec, np.int32)), A.ia(rec_ispec))
+        if self.elastic and pml is not None:
+            # prepare_gpu.f90:182-205
+            g = get_gll()
+            w = np.asarray(g.wgll, np.float64)
+            wc = np.array([w[i] * w[j] * w[k] for k in range(5) for j in range(5) for i in range(5)], np.float64)
+            L.prepare_fields_elastic_pml_(
+                self.MP, A.i(pml.n), A.ia(pml.is_cpml(db.nspec).astype(np.int32)), A.ia(pml.cpml_to_spec + 1),
+                A.ia(pml.spec_to_cpml(db.nspec)), A.fa(pml.coef_alpha.ravel()), A.fa(pml.coef_beta.ravel()),
+                A.fa(pml.coef_abar.ravel()), A.fa(pml.coef_strain.ravel()), A.fa(wc.astype(np.float32)),
+                A.fa(db.rhostore.ravel()))
         if self.elastic and self.element_order:
             if self.element_order == "morton":
                 order = morton_order(db, db.elastic_elements())
@@ -304,6 +316,9 @@ class GPUSolver:
                     if self.stf_table is not None:
                         stf = np.ascontiguousarray(self.stf_table[it - 1], np.float64)
                         L.compute_add_sources_el_cuda_(MP, stf.ctypes.data_as(C.c_void_p), C.byref(C.c_int(len(stf))))
+            if self.db.pml is not None:
+                # Dirichlet condition on the outer edges of the C-PML layers
+                L.pml_impose_boundary_condition_elastic_cuda_(MP)
             L.kernel_3_a_cuda_(MP, dt2, b_dt2, zero, one)
         if self.receivers is not None and self.receivers.n:
             self.seismo_current += 1

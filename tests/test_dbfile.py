@@ -109,3 +109,23 @@ def test_mass_and_stacey_terms_closed_form():
     kappa = 1020.0 * 1500.0 ** 2
     assert abs(q["rmass_acoustic"].astype(np.float64).sum() / (V / kappa) - 1) < 1e-6
     assert abs(q["rmassz_acoustic"].astype(np.float64).sum() / (0.5 * dt * (ax + ay + ab) / (1020.0 * 1500.0)) - 1) < 1e-6
+
+
+def test_database_file_roundtrip_with_cpml(tmp_path):
+    """C-PML records (save_arrays_solver.F90:186-218): widths, separation distance, regions, element lists
+    and damping profiles come back bit for bit, the absorbing faces are kept without Stacey mass terms,
+    and the convolution coefficients recomputed from them equal the original ones."""
+    from conftest import small_pml_box
+    from specfem3d_amd.mesh.dbfile import read_solver_database, write_solver_database
+    db, _, _, _ = small_pml_box(n=5, layers=2, warp=None)
+    p = str(tmp_path / "proc000000_external_mesh.bin")
+    write_solver_database(db, p)
+    rd = read_solver_database(p, db.dt, stacey=False, pml=True)
+    a, b = db.pml, rd.pml
+    assert np.array_equal(a.cpml_to_spec, b.cpml_to_spec) and np.array_equal(a.regions, b.regions)
+    assert np.array_equal(a.width, b.width) and a.min_distance == b.min_distance
+    for x, y in ((a.d, b.d), (a.K, b.K), (a.alpha, b.alpha), (a.coef_alpha, b.coef_alpha),
+                 (a.coef_beta, b.coef_beta), (a.coef_abar, b.coef_abar), (a.coef_strain, b.coef_strain)):
+        assert np.array_equal(x, y)
+    assert rd.abs_faces.n == db.abs_faces.n and not rd.stacey
+    assert np.array_equal(rd.rmassx, db.rmassx)
