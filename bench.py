@@ -303,13 +303,15 @@ def homogeneous_box_measurement(args) -> dict:
 def cpml_example_measurement(args) -> dict:
     """The reference example CPML_examples/homogeneous_halfspace_HEX8_elastic_absorbing_CPML_5sides
     (36 x 36 x 16 irregular elements, 1,366,625 points, 9036 C-PML elements, 2400 steps) with the
-    library defaults, and its seismograms against the example's REF_SEIS."""
+    library defaults (FMA kernels, coloured assembly), and its seismograms against the example's
+    REF_SEIS."""
+    import ctypes as C
     import numpy as np
     from specfem3d_amd.examples import cpml5_example
     from specfem3d_amd.solver import GPUSolver
     db, src, rec, ref, dt, _ = cpml5_example()
     n = ref.shape[0]
-    g = GPUSolver(db, src, rec, nstep=n)
+    g = GPUSolver(db, src, rec, nstep=n, exact=False, assembly=args.cpml_assembly)
     warm = 20
     for _ in range(warm):
         g.step()
@@ -544,6 +546,8 @@ def main():
                     help="1: also run the 20^3 homogeneous box single-shot figures")
     ap.add_argument("--cpml-example", type=int, default=1,
                     help="1: also run the reference's C-PML example (2400 steps) and compare with its REF_SEIS")
+    ap.add_argument("--cpml-assembly", type=int, default=-1,
+                    help="assembly of the C-PML example run (-1: library default, coloured without attenuation)")
     ap.add_argument("--coupled-shots", type=int, default=8,
                     help="shots of the batched fluid-over-viscoelastic workload (0: skip)")
     ap.add_argument("--coupled-steps", type=int, default=300, help="steps of each batched coupled shot")
@@ -611,6 +615,7 @@ def main():
                "batched_viscoelastic": batched_visco_measurement(args) if args.visco_shots > 0 else None,
                "adjoint": adjoint_measurement(args) if args.adjoint_steps > 0 else None,
                "coupled_batch": coupled_batch_measurement(args) if args.coupled_shots > 0 else None,
+               "cpml_example": cpml_example_measurement(args) if args.cpml_example else None,
                "large": large_measurement(args) if args.large_steps > 0 else None}
         print(json.dumps(out), flush=True)
         return

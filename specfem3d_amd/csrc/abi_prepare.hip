@@ -467,7 +467,6 @@ extern "C" void FC_FUNC_(prepare_constants_device, PREPARE_CONSTANTS_DEVICE)(
 
 namespace sf {
 static void build_elastic_assembly(Mesh* m);
-static int color_elements(int nglob, int nelem, const int* ibool0, const int* elems, int* col);
 
 // Distinct (factor_common, factor_common_kappa) rows, compared bit for bit, and a 16-bit row index
 // per local point; gives up (planes only) beyond 65535 distinct rows.
@@ -784,31 +783,6 @@ static void build_elastic_assembly(Mesh* m) {
   m->asm_ready = 1;
 }
 
-// C-PML list regrouped by colour (coloured assembly): same greedy colouring as the stiffness list
-void build_pml_colors(Mesh* m) {
-  if (m->pml_n <= 0) return;
-  HostSide* H = host_of(m);
-  std::vector<int> el(m->pml_n), cp(m->pml_n), col(m->pml_n);
-  SF_CHECK(hipMemcpy(el.data(), m->pml_elems, sizeof(int) * el.size(), hipMemcpyDeviceToHost));
-  SF_CHECK(hipMemcpy(cp.data(), m->pml_cpml, sizeof(int) * cp.size(), hipMemcpyDeviceToHost));
-  const int nc = color_elements(m->nglob, m->pml_n, H->ibool.data(), el.data(), col.data());
-  if (nc < 0) fatal("build_pml_colors", "element colouring needs more than 32 colours");
-  int* off = m->pml_color_off;
-  for (int c = 0; c <= 32; c++) off[c] = 0;
-  for (int q = 0; q < m->pml_n; q++) off[col[q] + 1]++;
-  for (int c = 0; c < 32; c++) off[c + 1] += off[c];
-  std::vector<int> pos(off, off + 32), le(m->pml_n), lc(m->pml_n);
-  for (int q = 0; q < m->pml_n; q++) {
-    const int d = pos[col[q]]++;
-    le[d] = el[q];
-    lc[d] = cp[q];
-  }
-  dfree(m->pml_color_elems);
-  dfree(m->pml_color_cpml);
-  m->pml_color_elems = dupload(le.data(), (int64_t)le.size());
-  m->pml_color_cpml = dupload(lc.data(), (int64_t)lc.size());
-  m->pml_ncolor = nc;
-}
 
 }  // namespace sf
 
@@ -989,6 +963,36 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_pml, PREPARE_FIELDS_ELASTIC_PML)
       }
     }
     m->pml_coef = dupload(cf.data(), (int64_t)cf.size());
+    // Per element: coefficient planes that hold one value (read once), and state planes whose
+    // multiplier plane is zero over the element (their memory variables never reach the stress or
+    // the acceleration, so they are not kept). Face regions damp one direction: about a third of the
+    // state and half of the coefficients are live there.
+    std::vector<uint64_t> mk(2 * (size_t)ncp);
+    for (int ic = 0; ic < ncp; ic++) {
+      const realw* b = &cf[(size_t)ic * blk];
+      uint64_t um = 0, nz = 0;
+      for (int m = 0; m < SF_PML_NCOEF; m++) {
+        bool uni = true, any = false;
+        for (int p = 0; p < 125; p++) {
+          uni = uni && b[m * 125 + p] == b[m * 125];
+          any = any || b[m * 125 + p] != 0.f;
+        }
+        if (uni) um |= 1ull << m;
+        if (any) nz |= 1ull << m;
+      }
+      uint64_t live = 0;
+      for (int a = 0; a < 9; a++)
+        for (int s2 = 0; s2 < 3; s2++)
+          if (nz >> (23 + 4 * (a / 3) + 1 + s2) & 1) live |= 1ull << (a * 3 + s2);
+      for (int bb = 0; bb < 12; bb++)
+        if (nz >> (23 + 13 + 2 * (bb / 4)) & 1) live |= 1ull << (27 + bb);
+      for (int c = 0; c < 3; c++)
+        for (int s2 = 0; s2 < 3; s2++)
+          if (nz >> (20 + s2) & 1) live |= 1ull << (39 + 3 * c + s2);
+      mk[2 * (size_t)ic] = um;
+      mk[2 * (size_t)ic + 1] = live;
+    }
+    m->pml_masks = dupload(mk.data(), (int64_t)mk.size());
   }
   const int64_t nst = (int64_t)ncp * SF_PML_NSTATE * 125;
   m->pml_state = dmalloc<realw>(nst);
@@ -1030,7 +1034,6 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_pml, PREPARE_FIELDS_ELASTIC_PML)
       SF_CHECK(hipMemcpy(order.data(), m->elastic_launch, sizeof(int) * order.size(), hipMemcpyDeviceToHost));
     }
     build_colors(m, order, m->el_color_elems, m->el_color_off, m->el_ncolor);
-    build_pml_colors(m);
   }
   prep_log("C-PML arrays prepared");
 }
@@ -1181,8 +1184,8 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   dfree(m->seismo_d); dfree(m->seismo_v); dfree(m->seismo_a); dfree(m->seismo_p);
   dfree(m->rho_kl); dfree(m->mu_kl); dfree(m->kappa_kl); dfree(m->rho_ac_kl); dfree(m->kappa_ac_kl);
   dfree(m->asm_ptr); dfree(m->asm_idx); dfree(m->asm_buf);
-  dfree(m->pml_elems); dfree(m->pml_cpml); dfree(m->pml_color_elems); dfree(m->pml_color_cpml);
-  dfree(m->pml_state); dfree(m->pml_coef); dfree(m->pml_old); dfree(m->pml_pts); dfree(m->pml_bc_pts);
+  dfree(m->pml_elems); dfree(m->pml_cpml);
+  dfree(m->pml_state); dfree(m->pml_coef); dfree(m->pml_masks); dfree(m->pml_old); dfree(m->pml_pts); dfree(m->pml_bc_pts);
   dfree(m->asm_ac_ptr); dfree(m->asm_ac_idx); dfree(m->asm_ac_buf);
   dfree(m->d_norm);
   if (m->h_norm) (void)hipHostFree(m->h_norm);
@@ -1220,7 +1223,6 @@ extern "C" void sf_set_options(long* Mesh_pointer, int exact, int assembly) {
   }
   if (assembly == 2) {
     if (!m->el_color_elems) ensure_element_colors(m);
-    if (!m->pml_color_elems && m->pml_n > 0) build_pml_colors(m);
     if (!m->ac_color_elems && m->nspec_acoustic > 0 && m->acoustic_elems) {
       std::vector<int> lst(m->nspec_acoustic);
       SF_CHECK(hipMemcpy(lst.data(), m->acoustic_elems, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));

@@ -23,7 +23,10 @@ const std::vector<int>& host_ibool(Mesh* m);
 static inline int field_of(int foa) { return foa == 3 ? 1 : 0; }
 
 // face-point terms in one atomic launch when the element stiffness scatters with atomics (its
-// sums have no fixed order then), else one launch per round in the reference's face order
+// sums have no fixed order then), else one launch per round in the reference's face order. The
+// elastic Stacey term keeps its rounds: on the 96x96x48 workload its 0.63 M points took 65 us in
+// one atomic launch against 37 us in three rounds, while the fluid Stacey term and both coupling
+// directions were faster in one launch (4.7 / 15.1 / 5.5 us against 18.9 / 19.2 / 18.7 us).
 static inline bool face_atomics(const Mesh* m) { return m->assembly == 0; }
 
 static void flush_el(Mesh* m, int f) {
@@ -121,20 +124,15 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
     P.pml_old = m->pml_old;
     P.state = m->pml_state;
     P.coef = m->pml_coef;
+    P.masks = m->pml_masks;
     P.t1 = m->pml_t1;
     P.t2 = m->pml_t2;
     P.local_out = m->asm_buf;
     P.nloc = a.nloc;
-    if (m->assembly == 2 && m->pml_color_elems) {
-      for (int c = 0; c < m->pml_ncolor; c++) {
-        P.elems = m->pml_color_elems + m->pml_color_off[c];
-        P.cpml = m->pml_color_cpml + m->pml_color_off[c];
-        P.nelem = m->pml_color_off[c + 1] - m->pml_color_off[c];
-        launch_elastic_pml(m->stream, P, m->exact != 0, 2);
-      }
-    } else {
-      launch_elastic_pml(m->stream, P, m->exact != 0, ordered ? 1 : 0);
-    }
+    // one launch, float atomics also under coloured assembly: split into its colours (about 1100
+    // elements each on the reference's C-PML example) the launches were too small to fill the chip
+    // (8 x 35 us against 240 us for one atomic launch before the dead-plane masks)
+    launch_elastic_pml(m->stream, P, m->exact != 0, ordered ? 1 : 0);
   }
   if (timed) {
     SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count + 1], m->stream));
@@ -409,7 +407,7 @@ extern "C" void FC_FUNC_(compute_stacey_viscoelastic_cuda, COMPUTE_STACEY_VISCOE
     flush_el(m, 0);
     const bool store = m->simulation_type == 1 && m->save_forward && b_absorb_field;
     launch_stacey_elastic(m->stream, m->abs_el, m->el[0].veloc, m->el[0].accel, m->b_absorb_field, store ? 1 : 0,
-                          face_atomics(m));
+                          false);
     if (store) {
       SF_CHECK(hipMemcpyAsync(b_absorb_field, m->b_absorb_field, sizeof(realw) * rec_len, hipMemcpyDeviceToHost,
                               m->stream));
@@ -437,7 +435,7 @@ extern "C" void FC_FUNC_(compute_stacey_viscoelastic_undoatt_cuda, COMPUTE_STACE
     if (foa == 3 && f == 0) continue;
     if (!m->el[f].displ) continue;
     flush_el(m, f);
-    launch_stacey_elastic(m->stream, m->abs_el, m->el[f].veloc, m->el[f].accel, nullptr, 0, face_atomics(m));
+    launch_stacey_elastic(m->stream, m->abs_el, m->el[f].veloc, m->el[f].accel, nullptr, 0, false);
   }
 }
 

@@ -1,9 +1,9 @@
 // k_acoustic.hip -- acoustic (fluid potential) element stiffness for NGLL=5.
 //
 // Restates compute_forces_acoustic.F90:35-726: chi'' -= div((1/rho) grad chi) in weak form.
-// Point-per-lane tiling like k_elastic_pt (one element per 128-lane workgroup, the element's
-// tiles in LDS). Compiled twice like k_elastic.hip (SF_EXACT = 1: no fused multiply-add, the
-// reference's rounding).
+// Same tiling as the elastic kernel (5 elements per 128-lane workgroup, one GLL column per
+// lane, gamma contractions in registers, xi/eta through LDS). Compiled twice like
+// k_elastic.hip (SF_EXACT = 1: no fused multiply-add, the reference's rounding).
 #include "kernels.h"
 
 #ifndef SF_EXACT
@@ -20,92 +20,100 @@
 namespace sf {
 namespace SF_VNS {
 
-// One element per 128-lane workgroup; lane p owns GLL point p = i + 5 j + 25 k (lanes 125..127
-// shadow point 124 and store nothing). The potential tile and the three products go through LDS;
-// every sum runs in the reference's order (the five terms of each contraction left to right, then
-// the three weighted divergence terms), the exact build's rounding of the CPU solver.
-__device__ __forceinline__ int xcd_remap_a(int b, int nb) {
-  const int q = nb >> 3, r = nb & 7, x = b & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
-}
-
 template <int MODE>
 __global__ void __launch_bounds__(128) k_acoustic(AcousticArgs A, GLLk G) {
-  __shared__ realw sC[128];               // potential tile
-  __shared__ realw sT[3][128];            // products t1, t2, t3
-  __shared__ realw sG[75];                // hprime, hprimewgll, wgllwgll
-  const int p = threadIdx.x;
-  const bool active = p < 125;
-  if (p < 25) sG[p] = G.hp[p];
-  else if (p < 50) sG[p] = G.hpw[p - 25];
-  else if (p < 75) sG[p] = G.wxy[p - 50];
-  const int ie = xcd_remap_a(blockIdx.x, gridDim.x);
-  const int e = A.elems[ie];
-  const int q = active ? p : 124;
-  const int k = q / 25, j = (q / 5) % 5, i = q % 5;
-  const int64_t ep = (int64_t)e * 125 + q;
-  const int g = A.ibool[ep];
-  const int irr = A.irregular[e];
-  const realw rho_invl = (realw)1 / A.rho[ep];
-  realw xixl = 0, xiyl = 0, xizl = 0, etaxl = 0, etayl = 0, etazl = 0, gaxl = 0, gayl = 0, gazl = 0, jl = 0;
-  if (irr) {
-    const int64_t qq = (int64_t)(irr - 1) * 125 + q;
-    const int64_t P = A.jac_plane;
-    const realw* J = A.jac;
-    xixl = J[qq]; xiyl = J[P + qq]; xizl = J[2 * P + qq];
-    etaxl = J[3 * P + qq]; etayl = J[4 * P + qq]; etazl = J[5 * P + qq];
-    gaxl = J[6 * P + qq]; gayl = J[7 * P + qq]; gazl = J[8 * P + qq];
-    jl = J[9 * P + qq];
-  }
-  sC[p] = A.pot[g];
-  __syncthreads();
-  const int b1 = 5 * j + 25 * k, b2 = i + 25 * k, b3 = i + 5 * j;
-  realw a1 = sG[i * 5] * sC[b1];
-  realw a2 = sG[j * 5] * sC[b2];
-  realw a3 = sG[k * 5] * sC[b3];
+  __shared__ realw s_buf[5][2][128];
+  const int tid = threadIdx.x;
+  const int el = tid / 25;
+  const int c = tid - el * 25;
+  const int i = c % 5, j = c / 5;
+  const int ie = blockIdx.x * 5 + el;
+  const bool active = (tid < 125) && (ie < A.nelem);
+  const int e = active ? A.elems[ie] : 0;
+  const int elx = el < 5 ? el : 4;
+
+  int g[5];
+  realw chi[5];
+  if (active) {
 #pragma unroll
-  for (int l = 1; l < 5; l++) {
-    a1 = a1 + sG[i * 5 + l] * sC[b1 + l];
-    a2 = a2 + sG[j * 5 + l] * sC[b2 + 5 * l];
-    a3 = a3 + sG[k * 5 + l] * sC[b3 + 25 * l];
+    for (int k = 0; k < 5; k++) {
+      g[k] = A.ibool[(int64_t)e * 125 + c + 25 * k];
+      chi[k] = A.pot[g[k]];
+      s_buf[el][0][c + 25 * k] = chi[k];
+    }
   }
-  realw t1, t2, t3;
-  if (irr) {
-    const realw dpdx = xixl * a1 + etaxl * a2 + gaxl * a3;
-    const realw dpdy = xiyl * a1 + etayl * a2 + gayl * a3;
-    const realw dpdz = xizl * a1 + etazl * a2 + gazl * a3;
-    t1 = rho_invl * jl * (xixl * dpdx + xiyl * dpdy + xizl * dpdz);
-    t2 = rho_invl * jl * (etaxl * dpdx + etayl * dpdy + etazl * dpdz);
-    t3 = rho_invl * jl * (gaxl * dpdx + gayl * dpdy + gazl * dpdz);
-  } else {
-    const realw jr = A.jacobian_regular, xr = A.xix_regular;
-    t1 = rho_invl * jr * xr * xr * a1;
-    t2 = rho_invl * jr * xr * xr * a2;
-    t3 = rho_invl * jr * xr * xr * a3;
+  __syncthreads();
+  const int irr = active ? A.irregular[e] : 0;
+  realw t1[5], t2[5], t3[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    if (!active) continue;
+    const int p = c + 25 * k;
+    const realw* s = s_buf[elx][0];
+    const int b1 = 5 * j + 25 * k, b2 = i + 25 * k;
+    realw a1 = G.hp[i * 5] * s[b1];
+    realw a2 = G.hp[j * 5] * s[b2];
+    realw a3 = G.hp[k * 5] * chi[0];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      a1 = a1 + G.hp[i * 5 + l] * s[b1 + l];
+      a2 = a2 + G.hp[j * 5 + l] * s[b2 + 5 * l];
+      a3 = a3 + G.hp[k * 5 + l] * chi[l];
+    }
+    const int64_t ep = (int64_t)e * 125 + p;
+    const realw rho_invl = (realw)1 / A.rho[ep];
+    if (irr) {
+      const int64_t q = (int64_t)(irr - 1) * 125 + p;
+      const int64_t P = A.jac_plane;
+      const realw* J = A.jac;
+      const realw xixl = J[q], xiyl = J[P + q], xizl = J[2 * P + q], etaxl = J[3 * P + q], etayl = J[4 * P + q];
+      const realw etazl = J[5 * P + q], gaxl = J[6 * P + q], gayl = J[7 * P + q], gazl = J[8 * P + q];
+      const realw jl = J[9 * P + q];
+      const realw dpdx = xixl * a1 + etaxl * a2 + gaxl * a3;
+      const realw dpdy = xiyl * a1 + etayl * a2 + gayl * a3;
+      const realw dpdz = xizl * a1 + etazl * a2 + gazl * a3;
+      t1[k] = rho_invl * jl * (xixl * dpdx + xiyl * dpdy + xizl * dpdz);
+      t2[k] = rho_invl * jl * (etaxl * dpdx + etayl * dpdy + etazl * dpdz);
+      t3[k] = rho_invl * jl * (gaxl * dpdx + gayl * dpdy + gazl * dpdz);
+    } else {
+      const realw jl = A.jacobian_regular, xr = A.xix_regular;
+      t1[k] = rho_invl * jl * xr * xr * a1;
+      t2[k] = rho_invl * jl * xr * xr * a2;
+      t3[k] = rho_invl * jl * xr * xr * a3;
+    }
   }
-  sT[0][p] = t1;
-  sT[1][p] = t2;
-  sT[2][p] = t3;
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      s_buf[el][0][c + 25 * k] = t1[k];
+      s_buf[el][1][c + 25 * k] = t2[k];
+    }
+  }
   __syncthreads();
   if (!active) return;
-  const realw* hpw = sG + 25;
-  realw n1 = hpw[i] * sT[0][b1];
-  realw n2 = hpw[j] * sT[1][b2];
-  realw n3 = hpw[k] * sT[2][b3];
 #pragma unroll
-  for (int l = 1; l < 5; l++) {
-    n1 = n1 + hpw[l * 5 + i] * sT[0][b1 + l];
-    n2 = n2 + hpw[l * 5 + j] * sT[1][b2 + 5 * l];
-    n3 = n3 + hpw[l * 5 + k] * sT[2][b3 + 25 * l];
-  }
-  const realw* wxy = sG + 50;
-  const realw f = wxy[j * 5 + k] * n1 + wxy[i * 5 + k] * n2 + wxy[i * 5 + j] * n3;
-  if (MODE == 1) {
-    A.local_out[ep] = f;
-  } else if (MODE == 2) {
-    A.potdd[g] = A.potdd[g] + (-f);
-  } else {
-    atomicAdd(A.potdd + g, -f);
+  for (int k = 0; k < 5; k++) {
+    const int b1 = 5 * j + 25 * k, b2 = i + 25 * k;
+    const realw* s1 = s_buf[el][0];
+    const realw* s2 = s_buf[el][1];
+    realw n1 = G.hpw[0 * 5 + i] * s1[b1];
+    realw n2 = G.hpw[0 * 5 + j] * s2[b2];
+    realw n3 = G.hpw[0 * 5 + k] * t3[0];
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
+      n1 = n1 + G.hpw[l * 5 + i] * s1[b1 + l];
+      n2 = n2 + G.hpw[l * 5 + j] * s2[b2 + 5 * l];
+      n3 = n3 + G.hpw[l * 5 + k] * t3[l];
+    }
+    const realw f = G.wxy[j * 5 + k] * n1 + G.wxy[i * 5 + k] * n2 + G.wxy[i * 5 + j] * n3;
+    if (MODE == 1) {
+      A.local_out[(int64_t)e * 125 + c + 25 * k] = f;
+    } else if (MODE == 2) {
+      A.potdd[g[k]] = A.potdd[g[k]] + (-f);
+    } else {
+      atomicAdd(A.potdd + g[k], -f);
+    }
   }
 }
 
@@ -114,7 +122,7 @@ using namespace SF_VNS;
 
 void SFX(launch_acoustic)(hipStream_t s, const AcousticArgs& a, int mode) {
   if (a.nelem <= 0) return;
-  dim3 grid(a.nelem), block(128);
+  dim3 grid((a.nelem + 4) / 5), block(128);
   if (mode == 1)
     hipLaunchKernelGGL(k_acoustic<1>, grid, block, 0, s, a, g_gll_host);
   else if (mode == 2)

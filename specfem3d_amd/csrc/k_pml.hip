@@ -9,9 +9,12 @@
 //     global point (it depends only on the point's u, v, a before the predictor); PML_displ_new is
 //     recomputed from u and v (the predictor leaves a = 0), so neither is stored per local point;
 //   * the 48 memory variables and 42 coefficients of a point are 125-point planes of per-element
-//     blocks: each access of a wavefront is one coalesced run;
-//   * the scatter is selected by MODE as in k_elastic_pt (0 atomic, 1 local store for the ordered
-//     gather, 2 read-modify-write within one colour).
+//     blocks: each access of a wavefront is one coalesced run. Per element, coefficient planes
+//     holding one value are read at one address, and memory variables whose multiplier is zero over
+//     the element (most of them in face and edge regions) are not kept at all;
+//   * the scatter is selected by MODE: 0 float atomics, 1 a local store for the ordered gather
+//     (one launch in both cases: the C-PML elements alone are too few to fill the chip colour by
+//     colour).
 // Every sum keeps the reference's operand order; compiled twice (SF_EXACT = 1: no FMA contraction,
 // the CPU reference's rounding; SF_EXACT = 0: FMA, with the regular-element mapping folded into
 // the GLL tables as in k_elastic_pt).
@@ -89,9 +92,13 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
   const Du9 O = map_gradient(tile_gradient<1>(sF[3], sF[4], sF[5], hpg, i, j, k), irr != 0, J, A.xix_regular);
   const Du9 N = map_gradient(tile_gradient<1>(sF[6], sF[7], sF[8], hpg, i, j, k), irr != 0, J, A.xix_regular);
 
-  const realw* cf = A.coef + ic * (SF_PML_NCOEF * 125) + q;
+  const realw* cf = A.coef + ic * (SF_PML_NCOEF * 125);
   realw* st = A.state + ic * (SF_PML_NSTATE * 125) + q;
-#define CF(m) cf[(m) * 125]
+  // element-uniform masks: coefficient planes with one value are read at point 0 (one address per
+  // wavefront), dead state planes are skipped and enter as zero (their multipliers are zero)
+  const uint64_t um = A.masks[2 * ic], lv = A.masks[2 * ic + 1];
+#define CF(m) cf[(m) * 125 + ((um >> (m)) & 1 ? 0 : q)]
+#define LIVE(pl) ((lv >> (pl)) & 1)
 
   // three-term memory variables: derivative direction d, component c; coefficient sets
   // (alpha_z, alpha_y, beta_x) for d/dx, (alpha_x, alpha_z, beta_y) for d/dy, (alpha_x, alpha_y, beta_z) for d/dz
@@ -105,9 +112,13 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
 #pragma unroll
     for (int s = 0; s < 3; s++) {
       const int b = cset[d][s];
-      const realw r = pml_conv(st[(a * 3 + s) * 125], nv[a], ov[a], CF(b), CF(b + 1), CF(b + 2));
+      const int pl = a * 3 + s;
+      realw r = 0;
+      if (LIVE(pl)) {
+        r = pml_conv(st[pl * 125], nv[a], ov[a], CF(b), CF(b + 1), CF(b + 2));
+        if (active) st[pl * 125] = r;
+      }
       r3[a][s] = r;
-      if (active) st[(a * 3 + s) * 125] = r;
     }
   }
   // one-term memory variables (alpha_x for 0..3, alpha_y for 4..7, alpha_z for 8..11)
@@ -117,9 +128,12 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
 #pragma unroll
   for (int b = 0; b < 12; b++) {
     const int cb = 3 * (b / 4);
-    const realw r = pml_conv(st[(27 + b) * 125], n1[b], o1[b], CF(cb), CF(cb + 1), CF(cb + 2));
+    realw r = 0;
+    if (LIVE(27 + b)) {
+      r = pml_conv(st[(27 + b) * 125], n1[b], o1[b], CF(cb), CF(cb + 1), CF(cb + 2));
+      if (active) st[(27 + b) * 125] = r;
+    }
     r1[b] = r;
-    if (active) st[(27 + b) * 125] = r;
   }
   // strain of the PML stress (A_6..A_23 = plane 23 + 0..17)
 #define AS(m) CF(23 + (m))
@@ -196,14 +210,19 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
 #pragma unroll
       for (int s = 0; s < 3; s++) {
         const int b = 3 * s;
-        const realw r = pml_conv(st[(39 + 3 * c + s) * 125], nw[c], od[c], CF(b), CF(b + 1), CF(b + 2));
+        const int pl = 39 + 3 * c + s;
+        realw r = 0;
+        if (LIVE(pl)) {
+          r = pml_conv(st[pl * 125], nw[c], od[c], CF(b), CF(b + 1), CF(b + 2));
+          if (active) st[pl * 125] = r;
+        }
         rr[s] = r;
-        if (active) st[(39 + 3 * c + s) * 125] = r;
       }
       cpml[c] = w * (A1 * v[c] + A2 * u[c] + A3 * rr[0] + A4 * rr[1] + A5 * rr[2]);
     }
   }
 #undef CF
+#undef LIVE
   lds_barrier();
   if (!active) return;
   realw f[3];
@@ -213,11 +232,6 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
     A.local_out[ep] = tot0;
     A.local_out[A.nloc + ep] = tot1;
     A.local_out[2 * A.nloc + ep] = tot2;
-  } else if (MODE == 2) {
-    realw* a = A.accel + 3 * (int64_t)g;
-    a[0] = a[0] + (-tot0);
-    a[1] = a[1] + (-tot1);
-    a[2] = a[2] + (-tot2);
   } else {
     realw* a = A.accel + 3 * (int64_t)g;
     atomicAdd(a + 0, -tot0);
@@ -235,7 +249,6 @@ void SFX(launch_elastic_pml)(hipStream_t s, const PmlArgs& a, int mode) {
   if (SF_FOLD) fold_gll(G, a.xix_regular, a.jacobian_regular);
   dim3 grid(a.nelem), block(128);
   if (mode == 1) hipLaunchKernelGGL(k_elastic_pml<1>, grid, block, 0, s, a, G);
-  else if (mode == 2) hipLaunchKernelGGL(k_elastic_pml<2>, grid, block, 0, s, a, G);
   else hipLaunchKernelGGL(k_elastic_pml<0>, grid, block, 0, s, a, G);
   SF_LAUNCH_CHECK("k_elastic_pml");
 }

@@ -94,11 +94,15 @@ struct PmlArgs {
   const realw* pml_old;    // 3*nglob
   realw* state;            // SF_PML_NSTATE planes per C-PML element
   const realw* coef;       // SF_PML_NCOEF planes per C-PML element
+  // per C-PML element: [2 ic] bit m set if coefficient plane m is one value over the element (read at
+  // point 0), [2 ic + 1] bit m set if state plane m is live (its multiplier plane is not all zero;
+  // dead planes are neither read nor written and enter as zero)
+  const uint64_t* masks;
   realw t1, t2;            // deltatover2 (1 - 2 theta), deltatsqover2 (1 - theta)
   realw* local_out;        // ordered assembly: 3 planes of nloc
   int64_t nloc;
 };
-// mode as launch_elastic
+// mode: 0 atomic scatter, 1 store local contributions (ordered gather)
 void launch_elastic_pml(hipStream_t s, const PmlArgs& a, bool exact, int mode);
 // PML_displ_old at the listed points: u + t1 v + t2 a, with the pending corrector (a *= rmass,
 // v += dt2p a) applied first when dt2p_pending != 0
