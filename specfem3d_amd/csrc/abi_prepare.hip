@@ -962,11 +962,12 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_pml, PREPARE_FIELDS_ELASTIC_PML)
         b[41 * 125] = h_wgll_cube[p] * rhostore[e * 125 + p];
       }
     }
-    m->pml_coef = dupload(cf.data(), (int64_t)cf.size());
-    // Per element: coefficient planes that hold one value (read once), and state planes whose
-    // multiplier plane is zero over the element (their memory variables never reach the stress or
-    // the acceleration, so they are not kept). Face regions damp one direction: about a third of the
-    // state and half of the coefficients are live there.
+    // Per element: coefficient planes that hold one value (kept as that value), and state planes
+    // whose multiplier plane is zero over the element (their memory variables never reach the
+    // stress or the acceleration, so they are not kept). Face regions damp one direction: about a
+    // third of the state and of the coefficients are live there. The rest is packed: the
+    // non-uniform coefficient planes and the live state planes of an element are contiguous, in
+    // plane order, at per-element offsets.
     std::vector<uint64_t> mk(2 * (size_t)ncp);
     for (int ic = 0; ic < ncp; ic++) {
       const realw* b = &cf[(size_t)ic * blk];
@@ -992,11 +993,26 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_pml, PREPARE_FIELDS_ELASTIC_PML)
       mk[2 * (size_t)ic] = um;
       mk[2 * (size_t)ic + 1] = live;
     }
+    std::vector<int64_t> off(2 * (size_t)ncp);
+    std::vector<realw> packed;
+    int64_t nst = 0;
+    for (int ic = 0; ic < ncp; ic++) {
+      const realw* b = &cf[(size_t)ic * blk];
+      off[2 * (size_t)ic] = (int64_t)packed.size();
+      off[2 * (size_t)ic + 1] = nst;
+      for (int m2 = 0; m2 < SF_PML_NCOEF; m2++) {
+        const int len = (mk[2 * (size_t)ic] >> m2 & 1) ? 1 : 125;
+        packed.insert(packed.end(), b + m2 * 125, b + m2 * 125 + len);
+      }
+      packed.resize((packed.size() + 31) / 32 * 32, (realw)0);  // elements start on 128-byte lines
+      nst += (125 * (int64_t)__builtin_popcountll(mk[2 * (size_t)ic + 1]) + 31) / 32 * 32;
+    }
+    m->pml_coef = dupload(packed.data(), (int64_t)packed.size());
     m->pml_masks = dupload(mk.data(), (int64_t)mk.size());
+    m->pml_off = dupload(off.data(), (int64_t)off.size());
+    m->pml_state = dmalloc<realw>(std::max<int64_t>(1, nst));
+    SF_CHECK(hipMemset(m->pml_state, 0, sizeof(realw) * std::max<int64_t>(1, nst)));
   }
-  const int64_t nst = (int64_t)ncp * SF_PML_NSTATE * 125;
-  m->pml_state = dmalloc<realw>(nst);
-  SF_CHECK(hipMemset(m->pml_state, 0, sizeof(realw) * nst));
   m->pml_old = dmalloc<realw>(3 * (int64_t)m->nglob);
   SF_CHECK(hipMemset(m->pml_old, 0, sizeof(realw) * 3 * (size_t)m->nglob));
   // global points of the C-PML elements, and those on the outer faces of the layers
@@ -1185,7 +1201,8 @@ extern "C" void FC_FUNC_(prepare_cleanup_device, PREPARE_CLEANUP_DEVICE)(long* M
   dfree(m->rho_kl); dfree(m->mu_kl); dfree(m->kappa_kl); dfree(m->rho_ac_kl); dfree(m->kappa_ac_kl);
   dfree(m->asm_ptr); dfree(m->asm_idx); dfree(m->asm_buf);
   dfree(m->pml_elems); dfree(m->pml_cpml);
-  dfree(m->pml_state); dfree(m->pml_coef); dfree(m->pml_masks); dfree(m->pml_old); dfree(m->pml_pts); dfree(m->pml_bc_pts);
+  dfree(m->pml_state); dfree(m->pml_coef); dfree(m->pml_masks); dfree(m->pml_off);
+  dfree(m->pml_old); dfree(m->pml_pts); dfree(m->pml_bc_pts);
   dfree(m->asm_ac_ptr); dfree(m->asm_ac_idx); dfree(m->asm_ac_buf);
   dfree(m->d_norm);
   if (m->h_norm) (void)hipHostFree(m->h_norm);

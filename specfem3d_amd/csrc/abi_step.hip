@@ -125,6 +125,7 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
     P.state = m->pml_state;
     P.coef = m->pml_coef;
     P.masks = m->pml_masks;
+    P.off = m->pml_off;
     P.t1 = m->pml_t1;
     P.t2 = m->pml_t2;
     P.local_out = m->asm_buf;

@@ -8,10 +8,10 @@
 //   * three element tiles sit in LDS: u, PML_displ_old and PML_displ_new. PML_displ_old is read per
 //     global point (it depends only on the point's u, v, a before the predictor); PML_displ_new is
 //     recomputed from u and v (the predictor leaves a = 0), so neither is stored per local point;
-//   * the 48 memory variables and 42 coefficients of a point are 125-point planes of per-element
-//     blocks: each access of a wavefront is one coalesced run. Per element, coefficient planes
-//     holding one value are read at one address, and memory variables whose multiplier is zero over
-//     the element (most of them in face and edge regions) are not kept at all;
+//   * the 48 memory variables and 42 coefficients of a point are 125-point planes: each access of a
+//     wavefront is one coalesced run. Per element, coefficient planes holding one value are read at
+//     one address, memory variables whose multiplier is zero over the element (most of them in
+//     face and edge regions) are not kept at all, and the remaining planes are packed;
 //   * the scatter is selected by MODE: 0 float atomics, 1 a local store for the ordered gather
 //     (one launch in both cases: the C-PML elements alone are too few to fill the chip colour by
 //     colour).
@@ -45,8 +45,8 @@ __device__ __forceinline__ realw pml_conv(realw r, realw nw, realw od, realw c0,
   return c0 * r + nw * c1 + od * c2;
 }
 
-// coefficient planes: alpha_x 0..2, alpha_y 3..5, alpha_z 6..8 (coef0, coef1, coef2), beta likewise
-// at 9..17, A_1..A_5 at 18..22, A_6..A_23 at 23..40, wgll_cube * rho at 41.
+// coefficient planes (before packing): alpha_x 0..2, alpha_y 3..5, alpha_z 6..8 (coef0, coef1,
+// coef2), beta likewise at 9..17, A_1..A_5 at 18..22, A_6..A_23 at 23..40, wgll_cube * rho at 41.
 // state planes: three-term memory variables (a * 3 + s) for a = dux_dxl_x, duy_dxl_y, duz_dxl_z,
 // dux_dyl_x, duy_dyl_y, duz_dyl_z, dux_dzl_x, duy_dzl_y, duz_dzl_z at 0..26; one-term ones at 27..38
 // (duz_dzl_y, duz_dyl_y, duy_dzl_z, duy_dyl_z | duz_dzl_x, duz_dxl_x, dux_dzl_z, dux_dxl_z |
@@ -92,13 +92,18 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
   const Du9 O = map_gradient(tile_gradient<1>(sF[3], sF[4], sF[5], hpg, i, j, k), irr != 0, J, A.xix_regular);
   const Du9 N = map_gradient(tile_gradient<1>(sF[6], sF[7], sF[8], hpg, i, j, k), irr != 0, J, A.xix_regular);
 
-  const realw* cf = A.coef + ic * (SF_PML_NCOEF * 125);
-  realw* st = A.state + ic * (SF_PML_NSTATE * 125) + q;
-  // element-uniform masks: coefficient planes with one value are read at point 0 (one address per
-  // wavefront), dead state planes are skipped and enter as zero (their multipliers are zero)
+  // element-uniform masks and packed planes: the coefficient planes of an element sit in plane
+  // order at its offset, a uniform one as its single value (one address per wavefront), the others
+  // as 125 values; the live state planes likewise, 125 values each. A plane's position is counted
+  // from the mask bits below it; dead state planes enter as zero (their multipliers are zero)
   const uint64_t um = A.masks[2 * ic], lv = A.masks[2 * ic + 1];
-#define CF(m) cf[(m) * 125 + ((um >> (m)) & 1 ? 0 : q)]
+  const realw* cf = A.coef + A.off[2 * ic];
+  realw* st = A.state + A.off[2 * ic + 1];
+#define BELOW(m) ((1ull << (m)) - 1ull)
+#define CF(m) cf[__builtin_popcountll(~um & BELOW(m)) * 125 + __builtin_popcountll(um & BELOW(m)) + \
+                 (((um >> (m)) & 1) ? 0 : q)]
 #define LIVE(pl) ((lv >> (pl)) & 1)
+#define ST(pl) st[__builtin_popcountll(lv & BELOW(pl)) * 125 + q]
 
   // three-term memory variables: derivative direction d, component c; coefficient sets
   // (alpha_z, alpha_y, beta_x) for d/dx, (alpha_x, alpha_z, beta_y) for d/dy, (alpha_x, alpha_y, beta_z) for d/dz
@@ -115,8 +120,8 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
       const int pl = a * 3 + s;
       realw r = 0;
       if (LIVE(pl)) {
-        r = pml_conv(st[pl * 125], nv[a], ov[a], CF(b), CF(b + 1), CF(b + 2));
-        if (active) st[pl * 125] = r;
+        r = pml_conv(ST(pl), nv[a], ov[a], CF(b), CF(b + 1), CF(b + 2));
+        if (active) ST(pl) = r;
       }
       r3[a][s] = r;
     }
@@ -130,8 +135,8 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
     const int cb = 3 * (b / 4);
     realw r = 0;
     if (LIVE(27 + b)) {
-      r = pml_conv(st[(27 + b) * 125], n1[b], o1[b], CF(cb), CF(cb + 1), CF(cb + 2));
-      if (active) st[(27 + b) * 125] = r;
+      r = pml_conv(ST(27 + b), n1[b], o1[b], CF(cb), CF(cb + 1), CF(cb + 2));
+      if (active) ST(27 + b) = r;
     }
     r1[b] = r;
   }
@@ -213,8 +218,8 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
         const int pl = 39 + 3 * c + s;
         realw r = 0;
         if (LIVE(pl)) {
-          r = pml_conv(st[pl * 125], nw[c], od[c], CF(b), CF(b + 1), CF(b + 2));
-          if (active) st[pl * 125] = r;
+          r = pml_conv(ST(pl), nw[c], od[c], CF(b), CF(b + 1), CF(b + 2));
+          if (active) ST(pl) = r;
         }
         rr[s] = r;
       }
@@ -223,6 +228,8 @@ __global__ void __launch_bounds__(128) k_elastic_pml(PmlArgs A, GLLk G) {
   }
 #undef CF
 #undef LIVE
+#undef ST
+#undef BELOW
   lds_barrier();
   if (!active) return;
   realw f[3];

@@ -92,12 +92,13 @@ struct PmlArgs {
   const realw* veloc;
   realw* accel;
   const realw* pml_old;    // 3*nglob
-  realw* state;            // SF_PML_NSTATE planes per C-PML element
-  const realw* coef;       // SF_PML_NCOEF planes per C-PML element
-  // per C-PML element: [2 ic] bit m set if coefficient plane m is one value over the element (read at
-  // point 0), [2 ic + 1] bit m set if state plane m is live (its multiplier plane is not all zero;
-  // dead planes are neither read nor written and enter as zero)
+  // per C-PML element: [2 ic] bit m set if coefficient plane m is one value over the element,
+  // [2 ic + 1] bit m set if state plane m is live (its multiplier plane is not all zero; dead planes
+  // are not stored and enter as zero)
   const uint64_t* masks;
+  const int64_t* off;      // [2 ic]: offset of the element's packed coefficient planes, [2 ic + 1]: of its state
+  realw* state;            // live state planes, 125 values each, packed per element in plane order
+  const realw* coef;       // coefficient planes packed per element in plane order: 1 value if uniform, else 125
   realw t1, t2;            // deltatover2 (1 - 2 theta), deltatsqover2 (1 - theta)
   realw* local_out;        // ordered assembly: 3 planes of nloc
   int64_t nloc;

@@ -133,12 +133,14 @@ struct Mesh {
   int pml_n = 0;                   // listed C-PML elements (elastic ones, processing order) ...
   int* pml_elems = nullptr;        // ... their element ids ...
   int* pml_cpml = nullptr;         // ... and their 0-based C-PML index
-  // per C-PML element blocks of 125-point planes: state (48 planes: 27 three-term strain memory
-  // variables, 12 one-term ones, 9 displacement memory variables) and coefficients (42 planes:
-  // alpha 9, beta 9, abar 5, strain 18, wgll_cube * rho)
+  // per C-PML element, packed planes: the live state planes (of 48: 27 three-term strain memory
+  // variables, 12 one-term ones, 9 displacement memory variables), 125 values each, and the
+  // coefficient planes (42: alpha 9, beta 9, abar 5, strain 18, wgll_cube * rho), one value for a
+  // plane uniform over the element, else 125
   realw* pml_state = nullptr;
   realw* pml_coef = nullptr;
   uint64_t* pml_masks = nullptr;   // 2 per C-PML element: uniform coefficient planes, live state planes
+  int64_t* pml_off = nullptr;      // 2 per C-PML element: offsets (floats) of its coefficient and state planes
   // PML_displ_old at every global point (3*nglob; it is a function of the point's u, v, a before the
   // predictor); PML_displ_new is recomputed in the kernel from u and v after the predictor (a = 0)
   realw* pml_old = nullptr;
