@@ -161,17 +161,21 @@ extern "C" void FC_FUNC_(update_displacement_cuda, UPDATE_DISPLACMENT_CUDA)(long
   check_field(m, f, "update_displacement_cuda");
   const int64_t n = 3 * m->el_n, o = 3 * m->el_g0;  // only the elastic range changes
   Wavefield& w = m->el[f];
-  if (f == 0 && m->pml_n > 0) {
-    // update_displacement_cuda.cu:92-124: PML_displ_old from the field before the predictor (the
-    // pending corrector applied on the fly); PML_displ_new is formed by the C-PML kernel
+  const bool pml = f == 0 && m->pml_n > 0;
+  if (pml) {
+    // update_displacement_cuda.cu:92-124: PML_displ_old from the field before the predictor;
+    // PML_displ_new is formed by the C-PML kernel. With a pending corrector the fused
+    // corrector-predictor stores it (for every elastic point), else the C-PML points are gathered.
     m->pml_t1 = *deltatover2_F * (1.0f - 2.0f * 0.125f);
     m->pml_t2 = *deltatsqover2_F * (1.0f - 0.125f);
-    launch_pml_store_old(m->stream, m->pml_npts, m->pml_pts, w.displ, w.veloc, w.accel, m->rmass,
-                         m->pending_k3_el[f], m->pending_dt2_el[f], m->pml_t1, m->pml_t2, m->pml_old);
+    if (!m->pending_k3_el[f])
+      launch_pml_store_old(m->stream, m->pml_npts, m->pml_pts, w.displ, w.veloc, w.accel, m->rmass, 0, 0.f,
+                           m->pml_t1, m->pml_t2, m->pml_old);
   }
   if (m->pending_k3_el[f]) {
     launch_corrector_predictor(m->stream, n, w.displ + o, w.veloc + o, w.accel + o, m->rmass + o,
-                               m->pending_dt2_el[f], *deltat_F, *deltatover2_F, *deltatsqover2_F);
+                               m->pending_dt2_el[f], *deltat_F, *deltatover2_F, *deltatsqover2_F,
+                               pml ? m->pml_old + o : nullptr, m->pml_t1, m->pml_t2);
     m->pending_k3_el[f] = 0;
   } else {
     launch_predictor(m->stream, n, w.displ + o, w.veloc + o, w.accel + o, *deltat_F, *deltatover2_F,

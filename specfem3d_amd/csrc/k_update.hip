@@ -56,18 +56,23 @@ __global__ void k_corrector(int64_t n4, int64_t n, realw* __restrict__ v, realw*
   }
 }
 
+// OLD = 1 also stores PML_displ_old = u + t1 v + t2 a of the corrected field before the predictor
+// (the arithmetic of k_pml_store_old), for every point of the range
+template <int OLD>
 __global__ void k_corrector_predictor(int64_t n4, int64_t n, realw* __restrict__ u, realw* __restrict__ v,
                                       realw* __restrict__ a, const realw* __restrict__ rm, realw dt2p, realw dt,
-                                      realw dt2, realw dtsq2) {
+                                      realw dt2, realw dtsq2, realw* __restrict__ old, realw t1, realw t2) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n4) {
     float4 U = reinterpret_cast<float4*>(u)[t];
     float4 V = reinterpret_cast<float4*>(v)[t];
     float4 A = reinterpret_cast<float4*>(a)[t];
     const float4 M = reinterpret_cast<const float4*>(rm)[t];
+    float4 O;
 #define SF_CP(c)                               \
   A.c = A.c * M.c;                             \
   V.c = V.c + dt2p * A.c;                      \
+  if (OLD) O.c = U.c + t1 * V.c + t2 * A.c;    \
   U.c = U.c + dt * V.c + dtsq2 * A.c;          \
   V.c = V.c + dt2 * A.c;
     SF_CP(x) SF_CP(y) SF_CP(z) SF_CP(w)
@@ -75,11 +80,13 @@ __global__ void k_corrector_predictor(int64_t n4, int64_t n, realw* __restrict__
     reinterpret_cast<float4*>(u)[t] = U;
     reinterpret_cast<float4*>(v)[t] = V;
     reinterpret_cast<float4*>(a)[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (OLD) reinterpret_cast<float4*>(old)[t] = O;
   }
   const int64_t r = 4 * n4 + t;
   if (t < 4 && r < n) {
     realw ar = a[r] * rm[r];
     realw vr = v[r] + dt2p * ar;
+    if (OLD) old[r] = u[r] + t1 * vr + t2 * ar;
     u[r] = u[r] + dt * vr + dtsq2 * ar;
     v[r] = vr + dt2 * ar;
     a[r] = 0.f;
@@ -144,11 +151,15 @@ void launch_corrector(hipStream_t s, int64_t n, realw* v, realw* a, const realw*
 }
 
 void launch_corrector_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, const realw* rmass,
-                                realw dt2_prev, realw dt, realw dt2, realw dtsq2) {
+                                realw dt2_prev, realw dt, realw dt2, realw dtsq2, realw* old, realw t1, realw t2) {
   if (n <= 0) return;
   const int64_t n4 = n / 4;
-  hipLaunchKernelGGL(k_corrector_predictor, grid_for(n4, 256), dim3(256), 0, s, n4, n, u, v, a, rmass, dt2_prev,
-                     dt, dt2, dtsq2);
+  if (old)
+    hipLaunchKernelGGL(k_corrector_predictor<1>, grid_for(n4, 256), dim3(256), 0, s, n4, n, u, v, a, rmass,
+                       dt2_prev, dt, dt2, dtsq2, old, t1, t2);
+  else
+    hipLaunchKernelGGL(k_corrector_predictor<0>, grid_for(n4, 256), dim3(256), 0, s, n4, n, u, v, a, rmass,
+                       dt2_prev, dt, dt2, dtsq2, nullptr, 0.f, 0.f);
   SF_LAUNCH_CHECK("k_corrector_predictor");
 }
 

@@ -36,8 +36,10 @@ void set_gll_constants(const realw* hprime, const realw* hprimewgll, const realw
 // Newmark pieces (k_update.hip)
 void launch_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, realw dt, realw dt2, realw dtsq2);
 void launch_corrector(hipStream_t s, int64_t n, realw* v, realw* a, const realw* rmass, realw dt2);
+// with `old`, also PML_displ_old = u + t1 v + t2 a of the corrected field (before the predictor)
 void launch_corrector_predictor(hipStream_t s, int64_t n, realw* u, realw* v, realw* a, const realw* rmass,
-                                realw dt2_prev, realw dt, realw dt2, realw dtsq2);
+                                realw dt2_prev, realw dt, realw dt2, realw dtsq2, realw* old = nullptr,
+                                realw t1 = 0.f, realw t2 = 0.f);
 
 // element stiffness (k_elastic.hip / k_acoustic.hip)
 struct ElasticArgs {

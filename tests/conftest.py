@@ -82,28 +82,32 @@ def smooth_warp(L, amp=0.06):
     return w
 
 
-def small_pml_box(n=6, layers=2, warp=None, dt=0.004, nrec=6, seed=0, attenuation=False):
+def small_pml_box(n=6, layers=2, warp=None, dt=0.004, nrec=6, seed=0, attenuation=False, top=False):
     """Gradient box with C-PML layers ``layers`` elements thick on the four sides and the bottom (free
-    top), one CMT source and receivers inside the layers' interior (test helper)."""
+    top; with ``top`` a layer replaces the free surface too, PML_INSTEAD_OF_FREE_SURFACE), one CMT
+    source and receivers inside the layers' interior (test helper)."""
     from specfem3d_amd.mesh.box import build_box, gradient_grf_model
     from specfem3d_amd import pml as P, sources as S
     L = 500.0 * n
     ext = (0.0, L, 0.0, L, -L, 0.0)
     db = build_box(n, n, n, ext, material=gradient_grf_model(ext, seed=seed, corr_len=700.0),
-                   stacey_faces=(1, 2, 3, 4, 5), warp=warp)
-    cs, rg = P.box_regions(db, layers)
-    db.pml = P.set_damping(db, cs, rg, f0=4.0)
+                   stacey_faces=(1, 2, 3, 4, 5, 6) if top else (1, 2, 3, 4, 5), warp=warp)
+    faces = ("xmin", "xmax", "ymin", "ymax", "bottom") + (("top",) if top else ())
+    cs, rg = P.box_regions(db, layers, faces)
+    db.pml = P.set_damping(db, cs, rg, f0=4.0, instead_of_free_surface=top)
     db.finalize(dt, stacey=False)
     att = None
     if attenuation:
         from specfem3d_amd.attenuation import setup_attenuation
         att = setup_attenuation(db, qmu=60.0, qkappa=200.0, f0_ref=4.0)
     rng = np.random.default_rng(seed)
+    zs = (0.42, 0.58) if top else (0.3, 0.5)
+    zr = (0.4, 0.6) if top else (0.0, 0.3)
     cmt = S.CMTSource(x=float(rng.uniform(0.42, 0.58) * L), y=float(rng.uniform(0.42, 0.58) * L),
-                      depth_km=float(rng.uniform(0.3, 0.5) * L / 1000.0), tshift=0.0, hdur=4.0,
+                      depth_km=float(rng.uniform(*zs) * L / 1000.0), tshift=0.0, hdur=4.0,
                       Mxx=1e13, Myy=-2e13, Mzz=0.7e13, Mxy=0.3e13, Mxz=-0.5e13, Myz=0.2e13)
     src = S.setup_cmt_sources(db, [cmt], dt, ricker=True)
     st = [S.Station(f"R{i}", "T", float(rng.uniform(0.1, 0.9) * L), float(rng.uniform(0.1, 0.9) * L), 0.0,
-                    float(rng.uniform(0.0, 0.3) * L)) for i in range(nrec)]
+                    float(rng.uniform(*zr) * L)) for i in range(nrec)]
     rec = S.setup_receivers(db, st)
     return db, src, rec, att

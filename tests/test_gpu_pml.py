@@ -1,8 +1,8 @@
 """C-PML layers on the GPU (elastic elements, forward runs) through the C-ABI.
 
 * exact build + ordered assembly equals the FP32 oracle bit for bit: seismograms and the whole
-  displacement field, on regular and on warped (irregular) elements, and with 3-SLS attenuation in the
-  elements inside the layers;
+  displacement field, on regular and on warped (irregular) elements, with 3-SLS attenuation in the
+  elements inside the layers, and with a layer in place of the free surface;
 * the FMA build with atomic and with coloured assembly stays within 1e-5 of the oracle;
 * the reference example CPML_examples/homogeneous_halfspace_HEX8_elastic_absorbing_CPML_5sides (2400
   steps, irregular CUBIT mesh) against its REF_SEIS and against the maximum displacement norms its log
@@ -37,10 +37,11 @@ def oracle_run(oracle_mod, db, src, rec, nstep, att=None):
     return seis, o.displ.copy(), o.veloc.copy(), o
 
 
-@pytest.mark.parametrize("case", ["regular", "warped", "attenuation"])
+@pytest.mark.parametrize("case", ["regular", "warped", "attenuation", "six_sides"])
 def test_pml_exact_ordered_bitwise(oracle_mod, case):
     warp = smooth_warp(3000.0) if case == "warped" else None
-    db, src, rec, att = small_pml_box(n=6, layers=2, warp=warp, attenuation=(case == "attenuation"))
+    db, src, rec, att = small_pml_box(n=6, layers=2, warp=warp, attenuation=(case == "attenuation"),
+                                      top=(case == "six_sides"))
     nstep = 250
     rs, ru, rv, _ = oracle_run(oracle_mod, db, src, rec, nstep, att)
     gs, gu, gv = gpu_run(db, src, rec, nstep, True, 1, att)

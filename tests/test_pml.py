@@ -7,7 +7,9 @@
 * the acceleration and strain coefficients against the partial-fraction expansions they are
   (s^2 s_x s_y s_z and s_i s_j / s_k with s_i = K_i + d_i / (alpha_i + s)), region by region;
 * the C-PML mass matrix: without damping it is the plain mass, and it grows with d;
-* the oracle: energy leaves a box through its C-PML layers and stays in the same box without them.
+* the oracle: energy leaves a box through its C-PML layers and stays in the same box without them;
+  a layer in place of the free surface (PML_INSTEAD_OF_FREE_SURFACE) takes out the energy the free
+  surface keeps, and its damping profile mirrors the bottom layer's.
 """
 import numpy as np
 import pytest
@@ -160,3 +162,48 @@ def test_oracle_energy_leaves_through_the_layers(oracle_mod):
     print("kinetic energy no layers", out[0], "with layers", out[2])
     assert out[2][-1] < 0.2 * out[0][-1]
     assert out[2][-1] < out[2][0]
+
+
+def test_top_layer_mirrors_bottom_layer():
+    """PML_INSTEAD_OF_FREE_SURFACE: the top layer's z profile is measured from the upper interface
+    (pml_set_local_dampingcoeff.f90, the z_origin_top branch). On a box with equal top and bottom
+    layers, d_z and alpha_z at a point of the top layer equal those at its mirror image in the
+    bottom layer, and both vanish at the interfaces and peak on the outer faces."""
+    db, _, _, _ = small_pml_box(n=6, layers=2, top=True)
+    p = db.pml
+    assert p.instead_of_free_surface and p.origins["z_top"] is not None
+    ib = db.ibool.reshape(db.nspec, 125)[p.cpml_to_spec]
+    z = db.zstore[ib].astype(np.float64)
+    zmin, zmax = float(db.zstore.min()), float(db.zstore.max())
+    damped = np.array([P._HAS[int(r)][2] for r in p.regions], bool)
+    zz, dz, az = z[damped].ravel(), p.d[2][damped].ravel(), p.alpha[2][damped].ravel()
+    top = zz > 0.5 * (zmin + zmax)
+    assert top.any() and (~top).any()
+    # distance into the layer from the nearer outer face, rounded to the mesh's float32 resolution;
+    # the points at one distance carry the same values on both sides (alpha also after the separation
+    # of nearly equal parameters, which moves some of them: compared as sorted sets)
+    key = np.round(np.where(top, zmax - zz, zz - zmin), 3)
+    for k in np.unique(key):
+        for v in (dz, az):
+            a, b = np.sort(v[top & (key == k)]), np.sort(v[~top & (key == k)])
+            assert a.size == b.size > 0
+            np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6)
+    assert dz[top].max() == pytest.approx(dz.max(), rel=2e-5) and dz[top & (key == key.max())].max() == 0
+    assert np.all(dz[top & (key == 0)] == dz[top].max())
+
+
+def test_oracle_top_layer_takes_out_surface_energy(oracle_mod):
+    """Kinetic energy 900 steps after the source: with a C-PML layer in place of the free surface it
+    falls far below the five-sided box's, where the free surface keeps waves in the box."""
+    out = {}
+    for top in (False, True):
+        db, src, rec, _ = small_pml_box(n=6, layers=2, top=top)
+        o = oracle_mod.OracleSolver(db, src, rec)
+        mass = 1.0 / db.rmassx.astype(np.float64)
+        for _ in range(900):
+            o.step()
+        v = o.veloc.reshape(-1, 3).astype(np.float64)
+        out[top] = 0.5 * float((mass * (v * v).sum(1)).sum())
+        assert np.isfinite(o.displ).all()
+    print("kinetic energy five sides", out[False], "six sides", out[True])
+    assert out[True] < 0.05 * out[False]
