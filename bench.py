@@ -313,28 +313,89 @@ def cpml_example_measurement(args) -> dict:
     n = ref.shape[0]
     g = GPUSolver(db, src, rec, nstep=n, exact=False, assembly=args.cpml_assembly)
     warm = 20
+    half = (n - warm) // 2
     for _ in range(warm):
         g.step()
     g.synchronize()
-    g.L.sf_timing_start(g.MP, C.c_int(n - warm))
+    tot, cnt = C.c_double(0), C.c_int(0)
+    ptot, pcnt = C.c_double(0), C.c_int(0)
+    # HIP events on the mesh stream: the whole elastic stiffness call over the first half of the timed
+    # steps, its C-PML launch alone over the second half (timing_scope 1)
+    g.L.sf_timing_start(g.MP, C.c_int(half))
     t = time.perf_counter()
-    for _ in range(n - warm):
+    for it in range(n - warm):
+        if it == half:
+            g.L.sf_timing_read(g.MP, C.byref(tot), C.byref(cnt))
+            g.L.sf_set_tuning(g.MP, b"timing_scope", C.c_int(1))
+            g.L.sf_timing_start(g.MP, C.c_int(n - warm - half))
         g.step()
     g.synchronize()
     wall = time.perf_counter() - t
-    tot = C.c_double(0)
-    cnt = C.c_int(0)
-    g.L.sf_timing_read(g.MP, C.byref(tot), C.byref(cnt))
+    g.L.sf_timing_read(g.MP, C.byref(ptot), C.byref(pcnt))
     out = g.seismograms("d").copy()
     g.close()
     e = float(np.sqrt(((out.astype(np.float64) - ref) ** 2).sum() / (ref.astype(np.float64) ** 2).sum()))
     value = db.nglob * (n - warm) / wall
+    stiff_ms = tot.value / max(cnt.value, 1)
+    pml_ms = ptot.value / max(pcnt.value, 1)
+    nbytes = pml_bytes_per_launch(db)
+    achieved = nbytes / (pml_ms * 1e-3) / 1e9
     log(f"[bench] C-PML example: {value:.3e} GLL-pt updates/s ({wall / (n - warm) * 1e3:.3f} ms/step, stiffness "
-        f"incl. C-PML {tot.value / max(cnt.value, 1):.3f} ms), rel-L2 vs REF_SEIS {e:.2e}")
+        f"incl. C-PML {stiff_ms:.3f} ms, C-PML launch {pml_ms * 1e3:.1f} us = {achieved:.0f} GB/s algorithmic), "
+        f"rel-L2 vs REF_SEIS {e:.2e}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        ora = oracle.OracleSolver(db, src, rec, "f32")
+        ns = 12
+        tc = time.perf_counter()
+        for _ in range(ns):
+            ora.step()
+        cpu = {"value": db.nglob * ns / (time.perf_counter() - tc), "unit": "GLL-point updates/s", "cores": 1,
+               "kind": "port", "sample": f"first {ns} of the {n} steps (FP32 C oracle with C-PML, one thread)"}
+        log(f"[bench] C-PML example oracle: {cpu['value']:.3e} GLL-pt updates/s on one core")
     return {"workload": "CPML_examples/homogeneous_halfspace_HEX8_elastic_absorbing_CPML_5sides", "nglob": db.nglob,
             "nspec": db.nspec, "nspec_cpml": db.pml.n, "value": value, "unit": "GLL-point updates/s",
             "ms_per_step": wall / (n - warm) * 1e3, "steps": n - warm,
-            "stiffness_ms": tot.value / max(cnt.value, 1), "rel_l2_vs_ref_seis": e}
+            "stiffness_ms": stiff_ms, "rel_l2_vs_ref_seis": e,
+            "roofline": {"kernel": "k_elastic_pml (C-PML elements)", "bound": "hbm", "achieved": round(achieved, 1),
+                         "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                         "bytes_per_launch": nbytes, "avg_launch_ms": pml_ms,
+                         "traffic": 400.6e6, "traffic_source": "profiles/k_elastic_pml_cpml5_example.md"},
+            "cpu_baseline": cpu}
+
+
+def pml_bytes_per_launch(db) -> float:
+    """Algorithmic HBM bytes of one C-PML stiffness launch: every value moved once. Per C-PML element,
+    the live memory-variable planes read and written (a plane is live when the coefficient plane that
+    multiplies it is not zero over the element) and the coefficient planes read (125 values, or one
+    for a plane uniform over the element); per local point ibool, kappa/mu and, on irregular
+    elements, the ten Jacobian terms; per global point of the C-PML elements u, v and PML_displ_old
+    read and accel read and written."""
+    import numpy as np
+    from specfem3d_amd.gll import get_gll
+    p = db.pml
+    n = p.n
+    w = get_gll().wgll
+    w3 = (w[None, None, :] * w[None, :, None] * w[:, None, None]).ravel().astype(np.float32)
+    rho = db.rhostore.reshape(db.nspec, 125)[p.cpml_to_spec]
+    cf = np.concatenate([p.coef_alpha.reshape(n, 125, 9), p.coef_beta.reshape(n, 125, 9),
+                         p.coef_abar.reshape(n, 125, 5), p.coef_strain.reshape(n, 125, 18),
+                         (w3[None, :] * rho)[:, :, None]], axis=2)
+    uni = np.all(cf == cf[:, :1, :], axis=1)
+    nz = np.any(cf != 0, axis=1)
+    live = 0
+    for a in range(9):
+        for s in range(3):
+            live += int(nz[:, 23 + 4 * (a // 3) + 1 + s].sum())
+    for b in range(12):
+        live += int(nz[:, 23 + 13 + 2 * (b // 4)].sum())
+    live += 3 * int(nz[:, 20:23].sum())
+    ib = db.ibool.reshape(db.nspec, 125)[p.cpml_to_spec]
+    nirr = int((db.irregular_element_number[p.cpml_to_spec] != 0).sum())
+    return float(live * 125 * 8 + int((~uni).sum()) * 125 * 4 + int(uni.sum()) * 4
+                 + n * 125 * 12 + nirr * 125 * 40 + np.unique(ib).size * 60)
 
 
 def coupled_batch_measurement(args) -> dict:

@@ -345,15 +345,13 @@ void transfer_swf_data_tohost_(long* p1, int* p2, int* p3, realw* p4, realw* p5,
    order (deterministic, the CPU solver's order); 2: one launch per colour of a greedy element
    colouring with a plain read-modify-write scatter (deterministic, no atomics). */
 void sf_set_options(long* Mesh_pointer, int exact, int assembly);
-/* tuning switch by name: "elastic_shape" 0 = column-per-lane stiffness kernel (five elements per
-   workgroup), 1 = point-per-lane (one element per workgroup, the default); "att_variant" 0..5 =
-   scheduling of the attenuation state in the point kernel (same results; 4, the default, loads it
-   straight into LDS; 5 = persistent workgroups that load the next element while computing one);
-   "pipe_wgs" >= 1 = workgroups of schedule 5 (default 2048); "acoustic_kernel_field"
-   0 = compute_kernels_acoustic_cuda pairs the adjoint potential with the reconstructed field as the
-   reference does (compute_kernels.f90:269,325), 1 = it uses the adjoint potential_dot_dot instead
-   (the misfit gradient when the adjoint field enters the fluid through the coupling);
-   unknown names are fatal */
+/* tuning switch by name: "att_variant" 0..4 = scheduling of the attenuation state in the point
+   stiffness kernel (same results; 4, the default, loads it straight into LDS);
+   "acoustic_kernel_field" 0 = compute_kernels_acoustic_cuda pairs the adjoint potential with the
+   reconstructed field as the reference does (compute_kernels.f90:269,325), 1 = it uses the adjoint
+   potential_dot_dot instead (the misfit gradient when the adjoint field enters the fluid through
+   the coupling); "timing_scope" 0 = sf_timing_* time each elastic stiffness call, 1 = only its
+   C-PML launch; unknown names are fatal */
 void sf_set_tuning(long* Mesh_pointer, const char* name, int value);
 /* device-resident checkpoints for checkpointed kernel runs (the files of
    save_forward_arrays_undoatt / read_forward_arrays_undoatt, save_forward_arrays.f90:102-179 and

@@ -1262,6 +1262,11 @@ extern "C" void sf_set_tuning(long* Mesh_pointer, const char* name, int value) {
     // 0 = potential_acoustic (compute_kernels.f90:269 and :325), 1 = potential_dot_dot_acoustic
     if (value < 0 || value > 1) fatal("sf_set_tuning", "acoustic_kernel_field must be 0 or 1");
     m->ac_kernel_dd = value;
+  } else if (std::strcmp(name, "timing_scope") == 0) {
+    // what sf_timing_start/sf_timing_read time in each elastic stiffness call: 0 = the whole call,
+    // 1 = its C-PML launch alone (calls without one are not counted)
+    if (value < 0 || value > 1) fatal("sf_set_tuning", "timing_scope must be 0 or 1");
+    m->timing_scope = value;
   } else {
     fatal("sf_set_tuning", "unknown tuning switch");
   }

@@ -92,8 +92,10 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
   ElasticArgs a = elastic_args(m, f, deltat);
   const bool ordered = m->assembly == 1 && m->asm_ready;
   const bool colored = m->assembly == 2 && m->el_color_elems;
-  const bool timed = m->timing_count < m->timing_max;
-  if (timed) SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count], m->stream));
+  const bool pml_call = f == 0 && m->pml_n > 0;
+  const bool whole = m->timing_scope == 0;
+  const bool timed = m->timing_count < m->timing_max && (whole || pml_call);
+  if (timed && whole) SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count], m->stream));
   if (colored) {
     for (int c = 0; c < m->el_ncolor; c++) {
       a.elems = m->el_color_elems + m->el_color_off[c];
@@ -103,7 +105,7 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
   } else {
     launch_elastic(m->stream, a, strain, att, m->exact != 0, ordered ? 1 : 0, m->att_variant);
   }
-  if (f == 0 && m->pml_n > 0) {
+  if (pml_call) {
     // C-PML elements (pml_kernel_2_impl of compute_forces_viscoelastic_cuda), after the others: their
     // contributions are summed after those of the stiffness elements
     PmlArgs P;
@@ -133,6 +135,7 @@ void elastic_forces(Mesh* m, int f, realw deltat, bool strain, bool att) {
     // one launch, float atomics also under coloured assembly: split into its colours (about 1100
     // elements each on the reference's C-PML example) the launches were too small to fill the chip
     // (8 x 35 us against 240 us for one atomic launch before the dead-plane masks)
+    if (timed && !whole) SF_CHECK(hipEventRecord(m->timing_ev[2 * m->timing_count], m->stream));
     launch_elastic_pml(m->stream, P, m->exact != 0, ordered ? 1 : 0);
   }
   if (timed) {

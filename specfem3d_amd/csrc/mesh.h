@@ -238,6 +238,7 @@ struct Mesh {
   // optional HIP-event timing of the element-stiffness launches (sf_timing_*)
   int timing_max = 0;
   int timing_count = 0;
+  int timing_scope = 0;            // 0: the whole elastic stiffness call, 1: its C-PML launch only
   hipEvent_t* timing_ev = nullptr; // 2 * timing_max events (start, stop)
 };
 
