@@ -900,6 +900,8 @@ extern "C" void FC_FUNC_(prepare_fields_elastic_pml, PREPARE_FIELDS_ELASTIC_PML)
   if (m->pml_state || m->pml_n) fatal(where, "C-PML arrays are already prepared");
   if (m->simulation_type != 1)
     fatal(where, "C-PML runs are forward only (SIMULATION_TYPE = 1), as the reference's GPU C-PML kernel");
+  if (m->nspec_acoustic > 0)  // compute_forces_acoustic_calling_routine.f90:97
+    fatal(where, "C-PML conditions with fluid elements are not supported on the GPU, as in the reference");
   const int ncp = *NSPEC_CPML;
   m->nspec_cpml = ncp;
   H->is_cpml.assign(m->nspec, 0);

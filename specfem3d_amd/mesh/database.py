@@ -169,6 +169,10 @@ class Database:
                 convolution_coefficients(self.pml, dt)
             if (self.pml.n and not self.ispec_is_elastic[self.pml.cpml_to_spec].all()):
                 raise ValueError("C-PML elements in acoustic domains are not supported (elastic C-PML only)")
+            if self.nspec_acoustic:
+                # the reference's GPU path stops on any fluid domain with C-PML
+                # (compute_forces_acoustic_calling_routine.f90:97)
+                raise ValueError("C-PML conditions with fluid elements are not supported (as on the reference's GPUs)")
         if self.nspec_elastic:
             el = self.ispec_is_elastic
             if self.pml is not None:

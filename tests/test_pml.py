@@ -117,6 +117,18 @@ def test_strain_coefficients_partial_fractions(region, index):
             assert abs(_s(float(K[axis]), float(d[axis]), float(a[axis]), s) - (A0 + A1 / (s + float(a[axis])))) < 1e-5
 
 
+def test_pml_with_fluid_elements_is_refused():
+    """The reference's GPU path stops on C-PML conditions in a mesh with fluid elements
+    (compute_forces_acoustic_calling_routine.f90:97); the database does the same."""
+    from conftest import small_coupled_box
+    db, _, _ = small_coupled_box(n=4)
+    cs, rg = P.box_regions(db, 1, ("xmin", "xmax", "ymin", "ymax", "bottom"))
+    solid = db.ispec_is_elastic[cs]
+    db.pml = P.set_damping(db, cs[solid], rg[solid], f0=4.0)
+    with pytest.raises(ValueError, match="fluid elements"):
+        db.finalize(db.dt, stacey=False)
+
+
 def test_pml_mass_matrix():
     db, _, _, _ = small_pml_box(n=4, layers=1)
     p = db.pml
