@@ -423,7 +423,8 @@ int sf_color_elements(int nglob, int nelem, const int* ibool0, const int* elems,
 void sf_set_jacobian(long* Mesh_pointer, const realw* jacobianstore);
 /* run any deferred corrector and wait for the mesh stream */
 void sf_synchronize(long* Mesh_pointer);
-/* HIP-event timing of the element-stiffness launches */
+/* HIP-event timing on the mesh stream of each elastic stiffness call (sf_set_tuning "timing_scope"
+   0), or of its C-PML launch alone ("timing_scope" 1), for the next max_launches calls */
 void sf_timing_start(long* Mesh_pointer, int max_launches);
 void sf_timing_read(long* Mesh_pointer, double* total_ms, int* count);
 /* message of the last fatal error */

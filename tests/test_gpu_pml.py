@@ -114,3 +114,24 @@ def test_cpml5_reference_example():
     print("[gpu] norms", {s: (f"{norms[s]:.4e}", f"{lg[s]:.4e}") for s in steps}, flush=True)
     assert e < 2e-3 and cc > 0.9999
     assert np.all(np.abs(ratio - 1) < 5e-3)
+
+
+def test_pml_launch_timing_scope():
+    """sf_set_tuning("timing_scope", 1) times the C-PML launch alone: one event pair per step, each
+    shorter than the whole elastic stiffness call that contains it (timing_scope 0)."""
+    from specfem3d_amd.solver import GPUSolver
+    db, src, rec, _ = small_pml_box(n=6, layers=2)
+    g = GPUSolver(db, src, rec, nstep=60, exact=False, assembly=0)
+    out = {}
+    for scope in (0, 1):
+        g.L.sf_set_tuning(g.MP, b"timing_scope", C.c_int(scope))
+        g.L.sf_timing_start(g.MP, C.c_int(20))
+        for _ in range(20):
+            g.step()
+        tot, cnt = C.c_double(0), C.c_int(0)
+        g.L.sf_timing_read(g.MP, C.byref(tot), C.byref(cnt))
+        out[scope] = (tot.value, cnt.value)
+    g.close()
+    print("[gpu] stiffness call / C-PML launch (ms, launches):", out, flush=True)
+    assert out[0][1] == 20 and out[1][1] == 20
+    assert 0 < out[1][0] < out[0][0]
