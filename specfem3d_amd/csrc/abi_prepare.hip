@@ -91,7 +91,7 @@ static HostSide* host_of(Mesh* m);
 static void build_face_points(Mesh* m, const RawFaces& rf, const unsigned char* flag, const int* ibool0,
                               FacePoints& fp, std::vector<int>* hp_ispec_out, std::vector<int>* hp_ploc_out) {
   const int nf = (int)rf.ispec.size();
-  struct P { int g, e, p, slot, round; realw n[3]; realw w; };
+  struct P { int g, e, p, slot, round, orig; realw n[3]; realw w; };
   std::vector<P> pts;
   pts.reserve((size_t)nf * 25);
   std::vector<int> occ;  // occurrence counter per global point (sparse via sort later)
@@ -107,20 +107,19 @@ static void build_face_points(Mesh* m, const RawFaces& rf, const unsigned char* 
       q.p = p;
       q.slot = f * 25 + a;
       q.round = 0;
+      q.orig = (int)pts.size();
       for (int c = 0; c < 3; c++) q.n[c] = rf.normal.empty() ? 0.f : rf.normal[((size_t)f * 25 + a) * 3 + c];
       q.w = rf.jacw.empty() ? 0.f : rf.jacw[(size_t)f * 25 + a];
       pts.push_back(q);
     }
   }
   // round index = number of earlier occurrences of the same global point
-  {
-    std::vector<std::pair<int, int>> key(pts.size());
-    for (size_t i = 0; i < pts.size(); i++) key[i] = {pts[i].g, (int)i};
-    std::sort(key.begin(), key.end());
-    for (size_t i = 0; i < key.size(); i++) {
-      if (i > 0 && key[i].first == key[i - 1].first)
-        pts[key[i].second].round = pts[key[i - 1].second].round + 1;
-    }
+  std::vector<std::pair<int, int>> key(pts.size());
+  for (size_t i = 0; i < pts.size(); i++) key[i] = {pts[i].g, (int)i};
+  std::sort(key.begin(), key.end());
+  for (size_t i = 0; i < key.size(); i++) {
+    if (i > 0 && key[i].first == key[i - 1].first)
+      pts[key[i].second].round = pts[key[i - 1].second].round + 1;
   }
   int nr = 0;
   for (auto& q : pts) nr = std::max(nr, q.round + 1);
@@ -135,6 +134,11 @@ static void build_face_points(Mesh* m, const RawFaces& rf, const unsigned char* 
       while (r <= pts[i].round) fp.round_off[r++] = i;
     while (r <= nr) fp.round_off[r++] = fp.n;
   }
+  // occurrence chains: the next occurrence of a point (key is in face order within each point)
+  std::vector<int> pos(fp.n), nx(fp.n, -1);
+  for (int i = 0; i < fp.n; i++) pos[pts[i].orig] = i;
+  for (size_t i = 0; i + 1 < key.size(); i++)
+    if (key[i + 1].first == key[i].first) nx[pos[key[i].second]] = pos[key[i + 1].second];
   std::vector<int> g(fp.n), e(fp.n), pl(fp.n), sl(fp.n);
   std::vector<realw> n3(3 * (size_t)fp.n), w(fp.n);
   for (int i = 0; i < fp.n; i++) {
@@ -148,12 +152,13 @@ static void build_face_points(Mesh* m, const RawFaces& rf, const unsigned char* 
   fp.slot = dupload(sl.data(), fp.n);
   fp.normal = dupload(n3.data(), 3 * (int64_t)fp.n);
   fp.jacw = dupload(w.data(), fp.n);
+  fp.next = dupload(nx.data(), fp.n);
   if (hp_ispec_out) *hp_ispec_out = e;
   if (hp_ploc_out) *hp_ploc_out = pl;
 }
 
 static void free_face_points(FacePoints& f) {
-  dfree(f.iglob); dfree(f.ispec); dfree(f.ploc); dfree(f.slot);
+  dfree(f.iglob); dfree(f.ispec); dfree(f.ploc); dfree(f.slot); dfree(f.next);
   dfree(f.normal); dfree(f.jacw); dfree(f.imp_p); dfree(f.imp_s);
   f.n = 0; f.nrounds = 0;
 }

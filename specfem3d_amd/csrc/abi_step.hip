@@ -24,9 +24,11 @@ static inline int field_of(int foa) { return foa == 3 ? 1 : 0; }
 
 // face-point terms in one atomic launch when the element stiffness scatters with atomics (its
 // sums have no fixed order then), else one launch per round in the reference's face order. The
-// elastic Stacey term keeps its rounds: on the 96x96x48 workload its 0.63 M points took 65 us in
-// one atomic launch against 37 us in three rounds, while the fluid Stacey term and both coupling
-// directions were faster in one launch (4.7 / 15.1 / 5.5 us against 18.9 / 19.2 / 18.7 us).
+// elastic Stacey term never uses atomics: one launch walks each point's occurrences in round order
+// (k_stacey_el_chain, the rounds' sums bit for bit). On the 96x96x48 workload its 0.63 M points take
+// 21.7 us so, against 37 us in three rounds and 65 us in one atomic launch; the fluid Stacey term
+// and both coupling directions were faster in one atomic launch (4.7 / 15.1 / 5.5 us against
+// 18.9 / 19.2 / 18.7 us in rounds).
 static inline bool face_atomics(const Mesh* m) { return m->assembly == 0; }
 
 static void flush_el(Mesh* m, int f) {

@@ -16,7 +16,7 @@ namespace sf {
 // Stacey elastic (compute_stacey_viscoelastic.F90:85-118):
 //   t = rho vp (v.n) n + rho vs (v - (v.n) n);  a -= t w
 // `store` (optional) receives t w at the point's record slot (SAVE_FORWARD boundary record).
-template <bool ATOMIC>
+// Float atomics (atomic assembly with the elastic Stacey term in one launch).
 __global__ void k_stacey_el(int n, const int* __restrict__ ig, const realw* __restrict__ nrm,
                             const realw* __restrict__ jw, const realw* __restrict__ rvp,
                             const realw* __restrict__ rvs, const int* __restrict__ slot,
@@ -33,21 +33,53 @@ __global__ void k_stacey_el(int n, const int* __restrict__ ig, const realw* __re
   const realw tz = p * vn * nz + s * (vz - vn * nz);
   const realw w = jw[t];
   const realw fx = tx * w, fy = ty * w, fz = tz * w;
-  if (ATOMIC) {
-    atomicAdd(accel + 3 * g, -fx);
-    atomicAdd(accel + 3 * g + 1, -fy);
-    atomicAdd(accel + 3 * g + 2, -fz);
-  } else {
-    accel[3 * g] = accel[3 * g] - fx;
-    accel[3 * g + 1] = accel[3 * g + 1] - fy;
-    accel[3 * g + 2] = accel[3 * g + 2] - fz;
-  }
+  atomicAdd(accel + 3 * g, -fx);
+  atomicAdd(accel + 3 * g + 1, -fy);
+  atomicAdd(accel + 3 * g + 2, -fz);
   if (store) {
     const int q = slot[t];
     store[3 * q] = fx;
     store[3 * q + 1] = fy;
     store[3 * q + 2] = fz;
   }
+}
+
+// The same term in one launch without atomics: a lane per point of round 0 (the first occurrence of
+// each global point) walks the point's occurrence chain in round order and applies each face's term
+// to a register copy of accel, which is the read-modify-write sequence of the rounds, in their
+// order, so the sum has the same bits.
+__global__ void k_stacey_el_chain(int n0, const int* __restrict__ ig, const realw* __restrict__ nrm,
+                                  const realw* __restrict__ jw, const realw* __restrict__ rvp,
+                                  const realw* __restrict__ rvs, const int* __restrict__ slot,
+                                  const int* __restrict__ next, const realw* __restrict__ veloc, realw* accel,
+                                  realw* __restrict__ store) {
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= n0) return;
+  const int64_t g = ig[t0];
+  const realw vx = veloc[3 * g], vy = veloc[3 * g + 1], vz = veloc[3 * g + 2];
+  realw ax = accel[3 * g], ay = accel[3 * g + 1], az = accel[3 * g + 2];
+  for (int t = t0; t >= 0; t = next[t]) {
+    const realw nx = nrm[3 * t], ny = nrm[3 * t + 1], nz = nrm[3 * t + 2];
+    const realw vn = vx * nx + vy * ny + vz * nz;
+    const realw p = rvp[t], s = rvs[t];
+    const realw tx = p * vn * nx + s * (vx - vn * nx);
+    const realw ty = p * vn * ny + s * (vy - vn * ny);
+    const realw tz = p * vn * nz + s * (vz - vn * nz);
+    const realw w = jw[t];
+    const realw fx = tx * w, fy = ty * w, fz = tz * w;
+    ax = ax - fx;
+    ay = ay - fy;
+    az = az - fz;
+    if (store) {
+      const int q = slot[t];
+      store[3 * q] = fx;
+      store[3 * q + 1] = fy;
+      store[3 * q + 2] = fz;
+    }
+  }
+  accel[3 * g] = ax;
+  accel[3 * g + 1] = ay;
+  accel[3 * g + 2] = az;
 }
 
 // replay of a stored boundary record on the backward field (compute_stacey_viscoelastic.F90:274-283)
@@ -143,12 +175,12 @@ void launch_stacey_elastic(hipStream_t s, const FacePoints& f, const realw* velo
                            int mode, bool atomic) {
   if (atomic) {
     if (f.n > 0)
-      hipLaunchKernelGGL(k_stacey_el<true>, sf_grid(f.n, 128), dim3(128), 0, s, f.n, f.iglob, f.normal, f.jacw,
+      hipLaunchKernelGGL(k_stacey_el, sf_grid(f.n, 128), dim3(128), 0, s, f.n, f.iglob, f.normal, f.jacw,
                          f.imp_p, f.imp_s, f.slot, veloc, accel, mode ? b_absorb : (realw*)nullptr);
-  } else {
-    SF_ROUNDS(f, hipLaunchKernelGGL(k_stacey_el<false>, sf_grid(cnt, 128), dim3(128), 0, s, cnt, f.iglob + o,
-                                    f.normal + 3 * o, f.jacw + o, f.imp_p + o, f.imp_s + o, f.slot + o, veloc, accel,
-                                    mode ? b_absorb : (realw*)nullptr));
+  } else if (f.nrounds > 0 && f.round_off[1] > 0) {
+    const int n0 = f.round_off[1];
+    hipLaunchKernelGGL(k_stacey_el_chain, sf_grid(n0, 128), dim3(128), 0, s, n0, f.iglob, f.normal, f.jacw, f.imp_p,
+                       f.imp_s, f.slot, f.next, veloc, accel, mode ? b_absorb : (realw*)nullptr);
   }
   SF_LAUNCH_CHECK("k_stacey_el");
 }

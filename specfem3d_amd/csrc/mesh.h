@@ -55,6 +55,7 @@ struct FacePoints {          // flattened boundary/coupling face points
   int nrounds = 0;
   int round_off[17] = {0};
   int* slot = nullptr;       // original host slot (face*25 + igll) of each point, for records
+  int* next = nullptr;       // index of the same global point's next occurrence (next round), -1 after the last
 };
 
 struct Snapshot {            // one saved state of a wavefield (save_forward_arrays_undoatt)
